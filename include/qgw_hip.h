@@ -1,0 +1,156 @@
+/*
+ * qgw_hip.h -- the device C-ABI of the MI355X timestep hot path
+ * (libqgw_hip.so).  Plain C types only.
+ *
+ * The reference exposes its hot path as global C functions called from main
+ * (src/qg.c:153-200) over process-global state.  Here the same calls take an
+ * explicit context so that several runs (an ensemble) can live in one
+ * process; each entry point names the reference function it replaces.
+ *
+ * Host arrays always use the reference storage layout
+ *   idx(i,j,k) = k*(n+2)*(m+2) + j*(n+2) + i        (src/domain.h:62)
+ * with a one-point ring (i = 0, n+1 and j = 0, m+1).  Device arrays are owned
+ * by the context and are never exposed.
+ *
+ * All calls return QGW_OK (0) or a negative status (qgw_host.h); none calls
+ * exit().  Calls on one context are not thread-safe; calls on different
+ * contexts are independent (each owns a HIP stream).
+ */
+#ifndef QGW_HIP_H
+#define QGW_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+#include "qgw_params.h"
+#include "qgw_host.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Largest layer count the device library accepts (the layer<->mode
+ * projections of the row passes are register-resident up to this count). */
+#define QGW_DEV_NL_MAX 16
+
+/* Arithmetic forms of the step. */
+#define QGW_FP64 0        /* every field and pass in FP64 */
+#define QGW_FP32 1        /* FP32 fields and passes, AB3 increment with Kahan compensation */
+#define QGW_FP32_PLAIN 2  /* FP32 fields and passes, uncompensated AB3 */
+
+/* Fields for qgw_upload / qgw_download (full storage layout, nl layers
+ * unless noted). */
+#define QGW_FIELD_Q 1      /* potential vorticity q */
+#define QGW_FIELD_PSI 2    /* streamfunction psi */
+#define QGW_FIELD_TOPO 3   /* bottom topography, one layer */
+#define QGW_FIELD_F0 4     /* newest AB3 tendency (f0_ab, src/timestep.h:10) */
+#define QGW_FIELD_F1 5     /* previous tendency (f1_ab) */
+#define QGW_FIELD_F2 6     /* tendency two steps back (f2_ab) */
+
+typedef struct qgw_ctx qgw_ctx;
+
+/* Create a run on the current HIP device: the grid and arrays of
+ * init_domain/init_vars (src/domain.h:64,178), the time-step state of
+ * init_timestep/check_timestep (src/timestep.h:56,18) and the transform set-up
+ * of init_elliptic (src/elliptic.h:29).  `precision` is QGW_FP64, QGW_FP32 or
+ * QGW_FP32_PLAIN.  q, psi and the tendency history start at zero, topography
+ * from the bathtub formula.  The vertical modes must be supplied with
+ * qgw_set_modes before the first inversion.  Returns NULL and sets *status
+ * on failure (bounded domains only: bc_fac >= 0; 1 <= nl <= QGW_DEV_NL_MAX;
+ * 2(n+1) and 2(m+1) in {2^a: 8..8192} or {3*2^a: 24..6144}; n_hyper even
+ * and <= 8; no stochastic (k_f) or file (dt_forc_period) forcing). */
+qgw_ctx *qgw_create(const qgw_params *p, int precision, int *status);
+
+/* Free the run (clean_fft, clean_timestep, clean_eigmode: src/elliptic.h:157,
+ * src/timestep.h:169, src/eigmode.h:28). */
+int qgw_destroy(qgw_ctx *ctx);
+
+/* Install the vertical modes computed on the host (outputs of compute_eigmode,
+ * src/eigmode.h:87-271; see qgwh_eigmode). cl2m[k*nl+l], cm2l[l*nl+k], iRd2[k]. */
+int qgw_set_modes(qgw_ctx *ctx, const double *cl2m, const double *cm2l, const double *iRd2);
+
+/* Copy a host array (reference layout, FP64) to the device, or back.  In the
+ * FP32 forms the upload rounds to FP32 and, for q, keeps the rounding residual
+ * in the compensation array, so a download returns q to ~2^-48 relative. */
+int qgw_upload(qgw_ctx *ctx, int field, const double *host);
+int qgw_download(qgw_ctx *ctx, int field, double *host);
+
+/* psi from q: layer->mode projection, 2-D sine transform, Helmholtz division,
+ * inverse transform, mode->layer projection, then the wall ring of psi and q
+ * (invert_pv, src/elliptic.h:76-155). */
+int qgw_invert_pv(qgw_ctx *ctx);
+
+/* Right-hand side into the newest history slot F0 without advancing time:
+ * invert_pv, then -J(psi,q) - beta psi_x + nu lap q + nu_hyper lap^(n_hyper+2) psi,
+ * wind on layer 0, Ekman drag and topography on layer nl-1
+ * (rhs + calc_diff, src/dynamics.h:47-73, 13-44). */
+int qgw_rhs(qgw_ctx *ctx);
+
+/* CFL-limited time step from the current psi with the reference's doubling,
+ * DT_max cap and snap to the next output time (adjust_timestep,
+ * src/timestep.h:69-111).  Updates the context's dt and returns it in *dt_out.
+ * With dt_fixed > 0 it returns dt_fixed and changes nothing. */
+int qgw_adjust_timestep(qgw_ctx *ctx, double *dt_out);
+
+/* One step: rotate the tendency history, rhs, new dt, variable-step AB3 update
+ * of the interior of q, t += dt (timestep, src/timestep.h:114-167). */
+int qgw_timestep(qgw_ctx *ctx);
+
+/* nsteps consecutive qgw_timestep calls.  With a fixed dt the whole sequence
+ * is enqueued without host synchronisation (the tendency buffers rotate by
+ * pointer), and is replayed from a captured HIP graph when nsteps allows. */
+int qgw_run(qgw_ctx *ctx, int nsteps);
+
+/* Energy and enstrophy of the current (psi, q) pair, accumulated in FP64:
+ * per layer E_layer[l] = -sum psi*q*Delta^2/Lx^2 and Z_layer[l] = sum q^2*Delta^2/Lx^2
+ * over all stored points (test/test_energy_injection.py:24-25), and the
+ * thickness-weighted totals E = sum_l dh_l E_l/(2H), Z = sum_l dh_l Z_l/H.
+ * Any output pointer may be NULL.  Call after qgw_invert_pv for the state at
+ * the current time. */
+int qgw_diagnostics(qgw_ctx *ctx, double *E_layer, double *Z_layer, double *E_tot, double *Z_tot);
+
+/* Time-step state (the reference globals t, dt, dt0, dt1, it; src/qg.c:63-70,
+ * src/timestep.h:14). */
+int qgw_get_time(qgw_ctx *ctx, double *t, double *dt, double *dt0, double *dt1, int *it);
+int qgw_set_time(qgw_ctx *ctx, double t, double dt, double dt0, double dt1, int it);
+
+/* Time of the next output: the variable step is shortened so that an integer
+ * number of steps lands on it (t_out, src/timestep.h:98-109, advanced by the
+ * driver at each output, src/qg.c:185-188). */
+int qgw_set_next_output(qgw_ctx *ctx, double t_out);
+
+/* Wait for all work queued on the context's stream. */
+int qgw_synchronize(qgw_ctx *ctx);
+
+/* Time nsteps steps of qgw_run between two HIP events recorded on the
+ * context's stream; *ms is the elapsed time. */
+int qgw_time_run(qgw_ctx *ctx, int nsteps, double *ms);
+
+/* Per-pass timing: while enabled, every kernel launch of the step is
+ * bracketed by HIP events on the context's stream.  qgw_profile_read returns,
+ * per pass id (QGW_PASS_*), the accumulated milliseconds and launch counts,
+ * and resets them. */
+#define QGW_PASS_P1 0      /* projection + forward row transform */
+#define QGW_PASS_T1 1      /* transpose to column-major spectral layout */
+#define QGW_PASS_P2 2      /* column transform, Helmholtz division, inverse column transform */
+#define QGW_PASS_T2 3      /* transpose back */
+#define QGW_PASS_P3 4      /* inverse row transform + back-projection + wall ring */
+#define QGW_PASS_P4 5      /* tendency + AB3 update */
+#define QGW_PASS_CFL 6     /* CFL reduction (adaptive dt only) */
+#define QGW_PASS_DIAG 7    /* diagnostics reduction */
+#define QGW_NPASS 8
+int qgw_profile_enable(qgw_ctx *ctx, int on);
+int qgw_profile_read(qgw_ctx *ctx, double *ms, int *count, int npass);
+
+/* Bytes of device memory held by the context. */
+int qgw_device_bytes(qgw_ctx *ctx, size_t *bytes);
+
+/* Human-readable text for a status code; the last HIP error text of the
+ * context is appended by qgw_last_error. */
+const char *qgw_strerror(int status);
+const char *qgw_last_error(qgw_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QGW_HIP_H */

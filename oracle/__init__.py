@@ -1,0 +1,241 @@
+"""TEST INFRASTRUCTURE ONLY -- ctypes view of the oracle (liboracle.so).
+
+The oracle is a CPU FP64 restatement of the reference's timestep hot path
+(see qgw_oracle.h for the function -> reference file:line map and the pins).
+Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg may load
+it, and only as the checker; the product never does.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+NL_MAX = 1000
+Q, PSI, TOPO, F0, F1, F2, LAPN = 1, 2, 3, 4, 5, 6, 7
+
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+class OrcParams(ctypes.Structure):
+    _fields_ = [
+        ("NX", ctypes.c_int), ("NY", ctypes.c_int), ("nl", ctypes.c_int), ("n_hyper", ctypes.c_int),
+        ("Lx", ctypes.c_double), ("dh", ctypes.c_double * NL_MAX), ("N2", ctypes.c_double * NL_MAX),
+        ("tau0", ctypes.c_double), ("forc_mode", ctypes.c_double), ("f0", ctypes.c_double),
+        ("beta", ctypes.c_double), ("nu", ctypes.c_double), ("nu_hyper", ctypes.c_double),
+        ("hEkb", ctypes.c_double), ("Ld", ctypes.c_double), ("h_topo", ctypes.c_double),
+        ("w_topo", ctypes.c_double), ("bc_fac", ctypes.c_double), ("cfl", ctypes.c_double),
+        ("sigma_f", ctypes.c_double), ("tend", ctypes.c_double), ("dt_out", ctypes.c_double),
+        ("dt_fixed", ctypes.c_double),
+    ]
+
+
+_lib = None
+
+
+def build():
+    import subprocess
+    subprocess.run(["make", "-B", "-C", HERE], check=True, stdout=subprocess.DEVNULL)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        path = os.path.join(HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        L = ctypes.CDLL(path)
+        vp = ctypes.c_void_p
+        ip = ctypes.POINTER(ctypes.c_int)
+        L.orc_create.argtypes = [ctypes.POINTER(OrcParams), ip]
+        L.orc_create.restype = vp
+        L.orc_destroy.argtypes = [vp]
+        L.orc_modes.argtypes = [vp, _dp, _dp, _dp]
+        L.orc_set_modes.argtypes = [vp, _dp, _dp, _dp]
+        L.orc_field.argtypes = [vp, ctypes.c_int]
+        L.orc_field.restype = _dp
+        L.orc_sizes.argtypes = [vp, ip, ip, ip]
+        for f in ("orc_invert_pv", "orc_calc_diff", "orc_timestep"):
+            getattr(L, f).argtypes = [vp]
+        L.orc_adjust_bc.argtypes = [vp, _dp, ctypes.c_int]
+        L.orc_rhs.argtypes = [vp, ctypes.c_int]
+        L.orc_adjust_timestep.argtypes = [vp]
+        L.orc_adjust_timestep.restype = ctypes.c_double
+        L.orc_run.argtypes = [vp, ctypes.c_int]
+        L.orc_get_time.argtypes = [vp, _dp, _dp, _dp, _dp, ip, _dp]
+        L.orc_set_time.argtypes = [vp, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                   ctypes.c_int, ctypes.c_double]
+        L.orc_diagnostics.argtypes = [vp, _dp, _dp, _dp, _dp]
+        L.orc_ab3_coeffs.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp]
+        L.orc_nu_hyper.argtypes = [vp]
+        L.orc_nu_hyper.restype = ctypes.c_double
+        L.orc_dst1d.argtypes = [_dp, ctypes.c_int]
+        L.orc_dst2d.argtypes = [_dp, ctypes.c_int, ctypes.c_int]
+        L.orc_fft.argtypes = [_dp, _dp, ctypes.c_int]
+        L.orc_eig_sym.argtypes = [ctypes.c_int, _dp, _dp, _dp]
+        L.orc_check_timestep.argtypes = [ctypes.POINTER(OrcParams), _dp, _dp]
+        L.orc_jacobian_field.argtypes = [vp, _dp, _dp, _dp]
+        L.orc_laplacian_field.argtypes = [vp, _dp, _dp]
+        L.orc_threads.argtypes = [ctypes.c_int]
+        L.orc_threads.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(_dp)
+
+
+def params_from(p):
+    """OrcParams from a qgw_amd.Params (same meanings) or a dict."""
+    d = p.as_dict() if hasattr(p, "as_dict") else dict(p)
+    o = OrcParams()
+    for name, _ in OrcParams._fields_:
+        if name in ("dh", "N2"):
+            arr = getattr(o, name)
+            for i, v in enumerate(d.get(name, [])):
+                arr[i] = v
+        elif name in d:
+            setattr(o, name, d[name])
+    return o
+
+
+class Oracle:
+    def __init__(self, p):
+        self.L = lib()
+        self.op = params_from(p)
+        st = ctypes.c_int(0)
+        self.s = self.L.orc_create(ctypes.byref(self.op), ctypes.byref(st))
+        if not self.s:
+            raise RuntimeError("orc_create failed (%d)" % st.value)
+        n, m, nl = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        self.L.orc_sizes(self.s, ctypes.byref(n), ctypes.byref(m), ctypes.byref(nl))
+        self.n, self.m, self.nl = n.value, m.value, nl.value
+        self.shape = (self.nl, self.m + 2, self.n + 2)
+
+    def close(self):
+        if self.s:
+            self.L.orc_destroy(self.s)
+            self.s = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def field(self, which):
+        """A numpy view of an oracle field (shares memory; copy to keep)."""
+        ptr = self.L.orc_field(self.s, which)
+        size = self.shape[1] * self.shape[2] * (1 if which == TOPO else self.nl)
+        a = np.ctypeslib.as_array(ptr, shape=(size,))
+        return a.reshape((self.shape[1], self.shape[2]) if which == TOPO else self.shape)
+
+    def modes(self):
+        nl = self.nl
+        a, b, c = np.zeros(nl * nl), np.zeros(nl * nl), np.zeros(nl)
+        self.L.orc_modes(self.s, _p(a), _p(b), _p(c))
+        return a, b, c
+
+    def set_modes(self, cl2m, cm2l, iRd2):
+        a = [np.ascontiguousarray(x, dtype=np.float64) for x in (cl2m, cm2l, iRd2)]
+        self.L.orc_set_modes(self.s, _p(a[0]), _p(a[1]), _p(a[2]))
+
+    def set_q(self, q):
+        self.field(Q)[...] = q
+
+    def invert_pv(self):
+        self.L.orc_invert_pv(self.s)
+
+    def calc_diff(self):
+        self.L.orc_calc_diff(self.s)
+
+    def rhs(self, which=F0):
+        self.L.orc_rhs(self.s, which)
+        return self.field(which).copy()
+
+    def adjust_timestep(self):
+        return self.L.orc_adjust_timestep(self.s)
+
+    def timestep(self):
+        self.L.orc_timestep(self.s)
+
+    def run(self, nsteps):
+        self.L.orc_run(self.s, int(nsteps))
+
+    def get_time(self):
+        t, dt, dt0, dt1, DT = (ctypes.c_double() for _ in range(5))
+        it = ctypes.c_int()
+        self.L.orc_get_time(self.s, ctypes.byref(t), ctypes.byref(dt), ctypes.byref(dt0), ctypes.byref(dt1),
+                            ctypes.byref(it), ctypes.byref(DT))
+        return dict(t=t.value, dt=dt.value, dt0=dt0.value, dt1=dt1.value, it=it.value, DT_max=DT.value)
+
+    def set_time(self, t, dt, dt0, dt1, it, t_out):
+        self.L.orc_set_time(self.s, t, dt, dt0, dt1, it, t_out)
+
+    def diagnostics(self):
+        El, Zl = np.zeros(self.nl), np.zeros(self.nl)
+        Et, Zt = ctypes.c_double(), ctypes.c_double()
+        self.L.orc_diagnostics(self.s, _p(El), _p(Zl), ctypes.byref(Et), ctypes.byref(Zt))
+        return El, Zl, Et.value, Zt.value
+
+    def jacobian(self, p, q):
+        p = np.ascontiguousarray(p, dtype=np.float64)
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        out = np.zeros(self.shape)
+        self.L.orc_jacobian_field(self.s, _p(p), _p(q), _p(out))
+        return out
+
+    def laplacian(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float64)
+        out = np.zeros(self.shape)
+        self.L.orc_laplacian_field(self.s, _p(p), _p(out))
+        return out
+
+    def nu_hyper(self):
+        return self.L.orc_nu_hyper(self.s)
+
+
+def dst1d(x):
+    a = np.ascontiguousarray(x, dtype=np.float64).copy()
+    lib().orc_dst1d(_p(a), a.size)
+    return a
+
+
+def dst2d(a):
+    b = np.ascontiguousarray(a, dtype=np.float64).copy()
+    m, n = b.shape
+    lib().orc_dst2d(_p(b), n, m)
+    return b
+
+
+def fft(z):
+    re = np.ascontiguousarray(z.real, dtype=np.float64).copy()
+    im = np.ascontiguousarray(z.imag, dtype=np.float64).copy()
+    lib().orc_fft(_p(re), _p(im), re.size)
+    return re + 1j * im
+
+
+def eig_sym(A):
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    n = A.shape[0]
+    w, V = np.zeros(n), np.zeros((n, n))
+    lib().orc_eig_sym(n, _p(A), _p(w), _p(V))
+    return w, V
+
+
+def ab3_coeffs(dt, dt0, dt1):
+    c = np.zeros(3)
+    lib().orc_ab3_coeffs(dt, dt0, dt1, _p(c))
+    return c
+
+
+def check_timestep(p):
+    o = params_from(p)
+    dt, DT = ctypes.c_double(), ctypes.c_double()
+    lib().orc_check_timestep(ctypes.byref(o), ctypes.byref(dt), ctypes.byref(DT))
+    return dt.value, DT.value
+
+
+def threads(n=0):
+    return lib().orc_threads(n)

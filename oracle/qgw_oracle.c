@@ -1,0 +1,732 @@
+/*
+ * qgw_oracle.c -- TEST INFRASTRUCTURE ONLY: CPU FP64 restatement of the
+ * reference's timestep hot path.  See qgw_oracle.h for the function map and
+ * for what pins each function.  Loaded by tests, the smoke check and the
+ * CPU-baseline leg of the benchmark; never by the product.
+ *
+ * Compile with -ffp-contract=off so that a*b+c is rounded twice, as the
+ * reference's plain `gcc -O3` x86-64 build does.
+ */
+#include "qgw_oracle.h"
+
+#include <complex.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+typedef double _Complex cplx;
+
+static const double PI_REF = 3.141592653589793; /* src/qg.c:44 */
+static const double HUGE_REF = 1e30;            /* src/qg.c:43 */
+static const double TEPS = 1e-9;                /* src/timestep.h:16 */
+
+int orc_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+  return omp_get_max_threads();
+#else
+  (void)n;
+  return 1;
+#endif
+}
+
+/* ------------------------------------------------------------------------ */
+/* FFT: recursive mixed-radix decimation in time (Cooley & Tukey 1965)      */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+  int L;
+  cplx *W; /* W[m] = exp(-2 pi i m / L) */
+} fftplan;
+
+static int plan_init(fftplan *p, int L) {
+  p->L = L;
+  p->W = (cplx *)malloc(sizeof(cplx) * (size_t)L);
+  if (!p->W) return -1;
+  for (int m = 0; m < L; m++) {
+    double a = -2.0 * PI_REF * (double)m / (double)L;
+    p->W[m] = cos(a) + I * sin(a);
+  }
+  return 0;
+}
+
+static int smallest_factor(int N) {
+  for (int f = 2; f * f <= N; f++)
+    if (N % f == 0) return f;
+  return N;
+}
+
+/* out[0..N-1] = DFT of in[0], in[stride], ..., in[(N-1) stride];
+ * wstep = L/N indexes the size-L root table. */
+static void fft_rec(const fftplan *pl, cplx *out, const cplx *in, int N, int stride, int wstep) {
+  if (N == 1) {
+    out[0] = in[0];
+    return;
+  }
+  int p = smallest_factor(N);
+  int M = N / p;
+  for (int r = 0; r < p; r++) fft_rec(pl, out + r * M, in + r * stride, M, stride * p, wstep * p);
+  cplx v[p], res[p]; /* radix p may be a large prime */
+  for (int k = 0; k < M; k++) {
+    for (int r = 0; r < p; r++) v[r] = out[r * M + k] * pl->W[((long)r * k * wstep) % pl->L];
+    for (int t = 0; t < p; t++) {
+      cplx acc = 0;
+      for (int r = 0; r < p; r++) acc += v[r] * pl->W[((long)r * t * M * wstep) % pl->L];
+      res[t] = acc;
+    }
+    for (int t = 0; t < p; t++) out[k + M * t] = res[t];
+  }
+}
+
+int orc_fft(double *re, double *im, int L) {
+  if (L < 1) return -1;
+  fftplan pl;
+  if (plan_init(&pl, L)) return -1;
+  cplx *in = (cplx *)malloc(sizeof(cplx) * L), *out = (cplx *)malloc(sizeof(cplx) * L);
+  for (int j = 0; j < L; j++) in[j] = re[j] + I * im[j];
+  fft_rec(&pl, out, in, L, 1, 1);
+  for (int j = 0; j < L; j++) {
+    re[j] = creal(out[j]);
+    im[j] = cimag(out[j]);
+  }
+  free(in);
+  free(out);
+  free(pl.W);
+  return 0;
+}
+
+/* Two RODFT00 transforms at once: x and y (length n, strided) become the real
+ * and imaginary parts of a sequence odd-extended to length 2(n+1); the DFT of
+ * an odd real sequence is -i times its sine transform, hence
+ * DST(x) = -Im, DST(y) = Re. */
+static void dst_two(const fftplan *pl, cplx *bin, cplx *bout, double *x, double *y, int n, long stride) {
+  int L = pl->L;
+  bin[0] = 0;
+  bin[n + 1] = 0;
+  for (int j = 1; j <= n; j++) {
+    cplx z = x[(long)(j - 1) * stride] + I * (y ? y[(long)(j - 1) * stride] : 0.0);
+    bin[j] = z;
+    bin[L - j] = -z;
+  }
+  fft_rec(pl, bout, bin, L, 1, 1);
+  for (int k = 1; k <= n; k++) {
+    x[(long)(k - 1) * stride] = -cimag(bout[k]);
+    if (y) y[(long)(k - 1) * stride] = creal(bout[k]);
+  }
+}
+
+int orc_dst1d(double *x, int n) {
+  fftplan pl;
+  if (plan_init(&pl, 2 * (n + 1))) return -1;
+  cplx *a = (cplx *)malloc(sizeof(cplx) * 2 * (n + 1)), *b = (cplx *)malloc(sizeof(cplx) * 2 * (n + 1));
+  dst_two(&pl, a, b, x, NULL, n, 1);
+  free(a);
+  free(b);
+  free(pl.W);
+  return 0;
+}
+
+/* 2-D RODFT00 on an m x n row-major array: fftw_plan_r2r_2d(Ny, Nx, ...) of
+ * src/elliptic.h:63-66 (n0 = Ny slow, n1 = Nx fast). */
+static int dst2d_plans(double *a, int n, int m, const fftplan *px, const fftplan *py) {
+  int Lmax = 2 * ((n > m ? n : m) + 1);
+#pragma omp parallel
+  {
+    cplx *b1 = (cplx *)malloc(sizeof(cplx) * Lmax), *b2 = (cplx *)malloc(sizeof(cplx) * Lmax);
+#pragma omp for schedule(static)
+    for (int j = 0; j < m; j += 2)
+      dst_two(px, b1, b2, a + (long)j * n, (j + 1 < m) ? a + (long)(j + 1) * n : NULL, n, 1);
+#pragma omp for schedule(static)
+    for (int i = 0; i < n; i += 2)
+      dst_two(py, b1, b2, a + i, (i + 1 < n) ? a + i + 1 : NULL, m, n);
+    free(b1);
+    free(b2);
+  }
+  return 0;
+}
+
+int orc_dst2d(double *a, int n, int m) {
+  fftplan px, py;
+  if (plan_init(&px, 2 * (n + 1))) return -1;
+  if (plan_init(&py, 2 * (m + 1))) return -1;
+  dst2d_plans(a, n, m, &px, &py);
+  free(px.W);
+  free(py.W);
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Symmetric eigen-solver: cyclic Jacobi (Golub & Van Loan sec. 8.5)        */
+/* ------------------------------------------------------------------------ */
+
+int orc_eig_sym(int n, const double *A0, double *w, double *V) {
+  double *A = (double *)malloc(sizeof(double) * n * n);
+  memcpy(A, A0, sizeof(double) * n * n);
+  for (int i = 0; i < n * n; i++) V[i] = 0.0;
+  for (int i = 0; i < n; i++) V[i * n + i] = 1.0;
+  for (int sweep = 0; sweep < 100; sweep++) {
+    double off = 0.0, tot = 0.0;
+    for (int i = 0; i < n; i++)
+      for (int j = 0; j < n; j++) {
+        tot += A[i * n + j] * A[i * n + j];
+        if (i != j) off += A[i * n + j] * A[i * n + j];
+      }
+    if (off <= 1e-34 * tot || off == 0.0) break;
+    for (int p = 0; p < n - 1; p++)
+      for (int q = p + 1; q < n; q++) {
+        double apq = A[p * n + q];
+        if (apq == 0.0) continue;
+        double theta = (A[q * n + q] - A[p * n + p]) / (2.0 * apq);
+        double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+        double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+        for (int k = 0; k < n; k++) { /* A <- A J */
+          double akp = A[k * n + p], akq = A[k * n + q];
+          A[k * n + p] = c * akp - s * akq;
+          A[k * n + q] = s * akp + c * akq;
+        }
+        for (int k = 0; k < n; k++) { /* A <- J^T A */
+          double apk = A[p * n + k], aqk = A[q * n + k];
+          A[p * n + k] = c * apk - s * aqk;
+          A[q * n + k] = s * apk + c * aqk;
+        }
+        for (int k = 0; k < n; k++) {
+          double vkp = V[k * n + p], vkq = V[k * n + q];
+          V[k * n + p] = c * vkp - s * vkq;
+          V[k * n + q] = s * vkp + c * vkq;
+        }
+      }
+  }
+  for (int i = 0; i < n; i++) w[i] = A[i * n + i];
+  /* ascending selection sort of (w, columns of V) */
+  for (int i = 0; i < n; i++) {
+    int mi = i;
+    for (int j = i + 1; j < n; j++)
+      if (w[j] < w[mi]) mi = j;
+    if (mi != i) {
+      double t = w[i];
+      w[i] = w[mi];
+      w[mi] = t;
+      for (int k = 0; k < n; k++) {
+        double tv = V[k * n + i];
+        V[k * n + i] = V[k * n + mi];
+        V[k * n + mi] = tv;
+      }
+    }
+  }
+  free(A);
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* state                                                                      */
+/* ------------------------------------------------------------------------ */
+
+struct orc {
+  orc_params p;
+  int Nx, Ny, Nxp1, Nyp1, Nxp2, Nyp2;
+  double Delta, Lx, Ly, norm;
+  double *X, *Y, *K, *L;
+  double *psi, *q, *diff, *lapn, *topo, *f0, *f1, *f2, *wrk1;
+  double *cl2m, *cm2l, *iRd2;
+  double nu_hyper;
+  double t, dt, dt0, dt1, DT_max, t_out;
+  int it;
+  fftplan px, py;
+};
+
+#define IDX(s, i, j, k) ((long)(k) * (s)->Nxp2 * (s)->Nyp2 + (long)(j) * (s)->Nxp2 + (i))
+
+void orc_check_timestep(const orc_params *p, double *dt_out, double *DT_max_out) {
+  /* init_domain: Delta from the cell count, Ly from NY; Nx = NX-1 (bounded) */
+  double Delta = p->Lx / p->NX;
+  double Ly = p->NY * Delta;
+  int Nx = p->NX - 1;
+  double nu_hyper = pow(-1, p->n_hyper / 2 - 1) * fabs(p->nu_hyper); /* src/domain.h:210 */
+  double inv_t_nu_hyper = nu_hyper * 20. / pow(p->Lx / Nx, p->n_hyper);
+  double inv_t_nu = p->nu * 20. / pow(p->Lx / Nx, 2);
+  double inv_t_beta = 2. * p->beta * p->Lx;
+  /* max(max(a,b),c) with max(p,q) = p < q ? q : p unparenthesised */
+  double mx = inv_t_nu_hyper < inv_t_nu ? inv_t_nu
+              : inv_t_nu_hyper < inv_t_beta ? inv_t_beta
+              : inv_t_nu_hyper < inv_t_nu ? inv_t_nu : inv_t_nu_hyper;
+  double DT_max = pow(mx, -1.);
+  double dt = 0.0;
+  double cfl = p->cfl, sigma_f = p->sigma_f;
+  if (p->tau0 != 0 && sigma_f != 0) {
+    double A = sqrt(cfl * Delta / (p->tau0 / p->dh[0] * p->forc_mode * PI_REF / Ly * Ly));
+    double B = pow((Delta * cfl) * (Delta * cfl) / sigma_f, 1. / 3.);
+    /* 1e-3*min(A,B) with min(p,q) = p > q ? q : p */
+    dt = 1e-3 * A > B ? B : A;
+  } else if (p->tau0 != 0 && sigma_f == 0) {
+    dt = 1e-3 * sqrt(cfl * Delta / (p->tau0 / p->dh[0] * p->forc_mode * PI_REF / Ly * Ly));
+  } else if (p->tau0 == 0 && sigma_f != 0) {
+    dt = 1e-3 * pow((Delta * cfl) * (Delta * cfl) / sigma_f, 1. / 3.);
+  }
+  if (DT_max != 0 && dt > DT_max) dt = DT_max;
+  *dt_out = dt;
+  *DT_max_out = DT_max;
+}
+
+static int compute_modes(orc *s) {
+  const orc_params *p = &s->p;
+  int nl = p->nl;
+  double htotal = 0;
+  for (int k = 0; k < nl; k++) htotal += p->dh[k];
+  double *G = (double *)calloc((size_t)nl * nl, sizeof(double));
+  double dhc[ORC_NL_MAX];
+  for (int l = 0; l < nl - 1; l++) dhc[l] = 0.5 * (p->dh[l] + p->dh[l + 1]);
+  if (nl > 1) {
+    int l = 0;
+    G[nl * l + l + 1] = -(p->f0 * p->f0) / p->N2[l] / (dhc[l] * p->dh[l]);
+    G[nl * l + l] = -G[nl * l + l + 1];
+    for (l = 1; l < nl - 1; l++) {
+      G[nl * l + l - 1] = -(p->f0 * p->f0) / p->N2[l - 1] / (dhc[l - 1] * p->dh[l]);
+      G[nl * l + l + 1] = -(p->f0 * p->f0) / p->N2[l] / (dhc[l] * p->dh[l]);
+      G[nl * l + l] = -G[nl * l + l - 1] - G[nl * l + l + 1];
+    }
+    l = nl - 1;
+    G[nl * l + l - 1] = -(p->f0 * p->f0) / p->N2[l - 1] / (dhc[l - 1] * p->dh[l]);
+    G[nl * l + l] = -G[nl * l + l - 1];
+  } else {
+    G[0] = 1.;
+  }
+  /* Gamma = D^-1 S with S symmetric: eigen-decompose T = D^1/2 Gamma D^-1/2 */
+  double *T = (double *)malloc(sizeof(double) * nl * nl), *V = (double *)malloc(sizeof(double) * nl * nl);
+  double *w = (double *)malloc(sizeof(double) * nl);
+  double *vr = (double *)malloc(sizeof(double) * nl * nl), *vl = (double *)malloc(sizeof(double) * nl * nl);
+  for (int i = 0; i < nl; i++)
+    for (int j = 0; j < nl; j++) T[i * nl + j] = 0.0;
+  for (int i = 0; i < nl; i++) T[i * nl + i] = G[i * nl + i];
+  for (int i = 0; i + 1 < nl; i++) {
+    double a = G[i * nl + i + 1], b = G[(i + 1) * nl + i];
+    double v = (a < 0 ? -1.0 : 1.0) * sqrt(fabs(a * b));
+    T[i * nl + i + 1] = v;
+    T[(i + 1) * nl + i] = v;
+  }
+  orc_eig_sym(nl, T, w, V); /* ascending, as the qsort of src/eigmode.h:154-160 */
+  for (int k = 0; k < nl; k++)
+    for (int m = 0; m < nl; m++) {
+      vr[k * nl + m] = V[k * nl + m] / sqrt(p->dh[k]);
+      vl[k * nl + m] = V[k * nl + m] * sqrt(p->dh[k]);
+    }
+  /* Flierl normalisation, sign(x) = x > 0 ? 1 : -1 (src/qg.c:42) */
+  for (int m = 0; m < nl; m++) {
+    double dotp = 0.;
+    for (int k = 0; k < nl; k++) dotp += p->dh[k] * vr[k * nl + m] * vr[k * nl + m];
+    double flfac = (vr[m] > 0 ? 1 : -1) * sqrt(htotal / dotp);
+    for (int k = 0; k < nl; k++) vr[k * nl + m] = flfac * vr[k * nl + m];
+  }
+  for (int m = 0; m < nl; m++) {
+    double dotp = 0.;
+    for (int k = 0; k < nl; k++) dotp += vr[k * nl + m] * vl[k * nl + m];
+    for (int k = 0; k < nl; k++) vl[k * nl + m] = vl[k * nl + m] / dotp;
+  }
+  for (int m = 0; m < nl; m++)
+    for (int k = 0; k < nl; k++) {
+      s->cl2m[k * nl + m] = vl[m * nl + k];
+      s->cm2l[k * nl + m] = vr[k * nl + m];
+    }
+  for (int k = 1; k < nl; k++) s->iRd2[k] = w[k];
+  s->iRd2[0] = 0.;
+  if (nl == 1 && p->Ld != 0) s->iRd2[0] = 1 / (p->Ld * p->Ld);
+  free(G);
+  free(T);
+  free(V);
+  free(w);
+  free(vr);
+  free(vl);
+  return 0;
+}
+
+orc *orc_create(const orc_params *p, int *status) {
+  *status = 0;
+  if (p->bc_fac < 0 || p->NX < 3 || p->NY < 3 || p->nl < 1 || p->nl > ORC_NL_MAX) {
+    *status = -1;
+    return NULL;
+  }
+  orc *s = (orc *)calloc(1, sizeof(orc));
+  s->p = *p;
+  /* init_domain */
+  s->Delta = p->Lx / p->NX;
+  s->Lx = p->Lx;
+  s->Ly = p->NY * s->Delta;
+  s->Nx = p->NX - 1;
+  s->Ny = p->NY - 1;
+  s->Nxp1 = s->Nx + 1;
+  s->Nyp1 = s->Ny + 1;
+  s->Nxp2 = s->Nx + 2;
+  s->Nyp2 = s->Ny + 2;
+  s->X = (double *)calloc(s->Nxp2, sizeof(double));
+  s->Y = (double *)calloc(s->Nyp2, sizeof(double));
+  s->L = (double *)calloc(s->Nxp2, sizeof(double));
+  s->K = (double *)calloc(s->Nyp2, sizeof(double));
+  for (int i = 0; i < s->Nxp2; i++) s->X[i] = i * s->Delta;
+  for (int j = 0; j < s->Nyp2; j++) s->Y[j] = j * s->Delta;
+  for (int j = 1; j < s->Nyp1; j++) s->K[j] = PI_REF * j / p->Lx;
+  for (int i = 1; i < s->Nxp1; i++) s->L[i] = PI_REF * i / p->Lx;
+  /* init_elliptic (bounded) */
+  s->norm = 4.0 * s->Nxp1 * s->Nyp1;
+  s->wrk1 = (double *)calloc((size_t)s->Nx * s->Ny, sizeof(double));
+  plan_init(&s->px, 2 * (s->Nx + 1));
+  plan_init(&s->py, 2 * (s->Ny + 1));
+  s->cl2m = (double *)calloc((size_t)p->nl * p->nl, sizeof(double));
+  s->cm2l = (double *)calloc((size_t)p->nl * p->nl, sizeof(double));
+  s->iRd2 = (double *)calloc(p->nl, sizeof(double));
+  compute_modes(s);
+  /* init_vars */
+  size_t S = (size_t)s->Nxp2 * s->Nyp2 * p->nl;
+  s->psi = (double *)calloc(S, sizeof(double));
+  s->q = (double *)calloc(S, sizeof(double));
+  s->diff = (double *)calloc(S, sizeof(double));
+  s->lapn = (double *)calloc(S, sizeof(double));
+  s->topo = (double *)calloc((size_t)s->Nxp2 * s->Nyp2, sizeof(double));
+  for (int j = 0; j < s->Nyp2; j++)
+    for (int i = 0; i < s->Nxp2; i++) {
+#define SHELF(x, d) (1 - exp(-((x) * (x)) / (2 * ((d) * (d)))))
+      s->topo[IDX(s, i, j, 0)] =
+          p->h_topo * (1 - SHELF(s->X[i], p->w_topo) * SHELF(p->Lx - s->X[i], p->w_topo) *
+                               SHELF(s->Y[j], p->w_topo) * SHELF(s->Ly - s->Y[j], p->w_topo));
+#undef SHELF
+    }
+  s->nu_hyper = pow(-1, p->n_hyper / 2 - 1) * fabs(p->nu_hyper);
+  /* init_timestep */
+  s->f0 = (double *)calloc(S, sizeof(double));
+  s->f1 = (double *)calloc(S, sizeof(double));
+  s->f2 = (double *)calloc(S, sizeof(double));
+  orc_check_timestep(p, &s->dt, &s->DT_max);
+  if (p->dt_fixed > 0) s->dt = p->dt_fixed;
+  s->dt0 = s->dt;
+  s->dt1 = s->dt;
+  s->t = 0;
+  s->t_out = 0;
+  s->it = 0;
+  return s;
+}
+
+void orc_destroy(orc *s) {
+  if (!s) return;
+  free(s->X); free(s->Y); free(s->K); free(s->L);
+  free(s->psi); free(s->q); free(s->diff); free(s->lapn); free(s->topo);
+  free(s->f0); free(s->f1); free(s->f2); free(s->wrk1);
+  free(s->cl2m); free(s->cm2l); free(s->iRd2);
+  free(s->px.W); free(s->py.W);
+  free(s);
+}
+
+int orc_modes(orc *s, double *cl2m, double *cm2l, double *iRd2) {
+  int nl = s->p.nl;
+  memcpy(cl2m, s->cl2m, sizeof(double) * nl * nl);
+  memcpy(cm2l, s->cm2l, sizeof(double) * nl * nl);
+  memcpy(iRd2, s->iRd2, sizeof(double) * nl);
+  return 0;
+}
+
+int orc_set_modes(orc *s, const double *cl2m, const double *cm2l, const double *iRd2) {
+  int nl = s->p.nl;
+  memcpy(s->cl2m, cl2m, sizeof(double) * nl * nl);
+  memcpy(s->cm2l, cm2l, sizeof(double) * nl * nl);
+  memcpy(s->iRd2, iRd2, sizeof(double) * nl);
+  return 0;
+}
+
+double *orc_field(orc *s, int which) {
+  switch (which) {
+    case ORC_Q: return s->q;
+    case ORC_PSI: return s->psi;
+    case ORC_TOPO: return s->topo;
+    case ORC_F0: return s->f0;
+    case ORC_F1: return s->f1;
+    case ORC_F2: return s->f2;
+    case ORC_LAPN: return s->lapn;
+  }
+  return NULL;
+}
+
+void orc_sizes(orc *s, int *n, int *m, int *nl) {
+  *n = s->Nx;
+  *m = s->Ny;
+  *nl = s->p.nl;
+}
+
+double orc_nu_hyper(orc *s) { return s->nu_hyper; }
+
+/* adjust_bc, bounded walls only (src/domain.h:308-348): S, N, W, E in that
+ * order, so W/E own the corners. */
+void orc_adjust_bc(orc *s, double *field, int field_type) {
+  double psi_bc = 0.;
+  double *psi = s->psi;
+  double c = 2 * s->p.bc_fac / (s->Delta * s->Delta);
+  for (int k = 0; k < s->p.nl; k++) {
+    for (int i = 0; i < s->Nxp2; i++) {
+      int j = 0;
+      field[IDX(s, i, j, k)] = (field_type ? c * (psi[IDX(s, i, j + 1, k)] - psi_bc) : psi_bc);
+    }
+    for (int i = 0; i < s->Nxp2; i++) {
+      int j = s->Nyp1;
+      field[IDX(s, i, j, k)] = (field_type ? c * (psi[IDX(s, i, j - 1, k)] - psi_bc) : psi_bc);
+    }
+    for (int j = 0; j < s->Nyp2; j++) {
+      int i = 0;
+      field[IDX(s, i, j, k)] = (field_type ? c * (psi[IDX(s, i + 1, j, k)] - psi_bc) : psi_bc);
+    }
+    for (int j = 0; j < s->Nyp2; j++) {
+      int i = s->Nxp1;
+      field[IDX(s, i, j, k)] = (field_type ? c * (psi[IDX(s, i - 1, j, k)] - psi_bc) : psi_bc);
+    }
+  }
+}
+
+void orc_invert_pv(orc *s) {
+  int nl = s->p.nl, Nx = s->Nx, Ny = s->Ny;
+  double *psi = s->psi, *q = s->q, *wrk1 = s->wrk1;
+  size_t S = (size_t)s->Nxp2 * s->Nyp2 * nl;
+  for (size_t t = 0; t < S; t++) psi[t] = 0.;
+  for (int k = 0; k < nl; k++) {
+#pragma omp parallel for schedule(static)
+    for (int j = 1; j < s->Nyp1; j++)
+      for (int i = 1; i < s->Nxp1; i++) {
+        long f = (long)(j - 1) * Nx + (i - 1);
+        wrk1[f] = 0;
+        for (int l = 0; l < nl; l++) wrk1[f] += s->cl2m[k * nl + l] * q[IDX(s, i, j, l)];
+      }
+    dst2d_plans(wrk1, Nx, Ny, &s->px, &s->py);
+#pragma omp parallel for schedule(static)
+    for (int j = 1; j < s->Nyp1; j++)
+      for (int i = 1; i < s->Nxp1; i++) {
+        double fact = -(s->L[i] * s->L[i] + s->K[j] * s->K[j] + s->iRd2[k]);
+        long f = (long)(j - 1) * Nx + (i - 1);
+        wrk1[f] = wrk1[f] / fact;
+      }
+    dst2d_plans(wrk1, Nx, Ny, &s->px, &s->py);
+#pragma omp parallel for schedule(static)
+    for (int j = 1; j < s->Nyp1; j++)
+      for (int i = 1; i < s->Nxp1; i++) {
+        long f = (long)(j - 1) * Nx + (i - 1);
+        wrk1[f] = wrk1[f] / s->norm;
+      }
+#pragma omp parallel for schedule(static)
+    for (int j = 1; j < s->Nyp1; j++)
+      for (int i = 1; i < s->Nxp1; i++) {
+        long f = (long)(j - 1) * Nx + (i - 1);
+        for (int l = 0; l < nl; l++) psi[IDX(s, i, j, l)] += s->cm2l[l * nl + k] * wrk1[f];
+      }
+  }
+  orc_adjust_bc(s, psi, 0);
+  orc_adjust_bc(s, q, 1);
+}
+
+/* laplacian macro of src/domain.h:12 */
+static inline double lap(const orc *s, const double *p, int i, int j, int k) {
+  return (p[IDX(s, i + 1, j, k)] + p[IDX(s, i - 1, j, k)] + p[IDX(s, i, j + 1, k)] +
+          p[IDX(s, i, j - 1, k)] - 4 * p[IDX(s, i, j, k)]) /
+         (s->Delta * s->Delta);
+}
+
+/* jacobian_lev(p, q, k1, k2) of src/domain.h:17-27 (Arakawa 1966) */
+static inline double jac(const orc *s, const double *p, const double *q, int i, int j, int k1, int k2) {
+#define P(a, b) p[IDX(s, a, b, k1)]
+#define Q(a, b) q[IDX(s, a, b, k2)]
+  return (((P(i + 1, j) - P(i - 1, j)) * (Q(i, j + 1) - Q(i, j - 1)) +
+           (P(i, j - 1) - P(i, j + 1)) * (Q(i + 1, j) - Q(i - 1, j)) +
+           P(i + 1, j) * (Q(i + 1, j + 1) - Q(i + 1, j - 1)) -
+           P(i - 1, j) * (Q(i - 1, j + 1) - Q(i - 1, j - 1)) -
+           P(i, j + 1) * (Q(i + 1, j + 1) - Q(i - 1, j + 1)) +
+           P(i, j - 1) * (Q(i + 1, j - 1) - Q(i - 1, j - 1)) +
+           Q(i, j + 1) * (P(i + 1, j + 1) - P(i - 1, j + 1)) -
+           Q(i, j - 1) * (P(i + 1, j - 1) - P(i - 1, j - 1)) -
+           Q(i + 1, j) * (P(i + 1, j + 1) - P(i + 1, j - 1)) +
+           Q(i - 1, j) * (P(i - 1, j + 1) - P(i - 1, j - 1))) /
+          (12. * s->Delta * s->Delta));
+#undef P
+#undef Q
+}
+
+void orc_calc_diff(orc *s) {
+  int nl = s->p.nl;
+  size_t S = (size_t)s->Nxp2 * s->Nyp2 * nl;
+  memcpy(s->diff, s->psi, S * sizeof(double));
+  for (int n = 0; n <= s->p.n_hyper; n += 2) {
+    for (int k = 0; k < nl; k++) {
+#pragma omp parallel for schedule(static)
+      for (int j = 1; j < s->Nyp1; j++)
+        for (int i = 1; i < s->Nxp1; i++) s->lapn[IDX(s, i, j, k)] = lap(s, s->diff, i, j, k);
+    }
+    orc_adjust_bc(s, s->lapn, 1);
+    memcpy(s->diff, s->lapn, S * sizeof(double));
+  }
+}
+
+static double *field_ptr(orc *s, int which) { return orc_field(s, which); }
+
+void orc_rhs(orc *s, int which_out) {
+  double *f1 = field_ptr(s, which_out);
+  const orc_params *p = &s->p;
+  int nl = p->nl;
+  double *psi = s->psi, *q = s->q;
+  orc_invert_pv(s);
+  orc_calc_diff(s);
+  double Delta = s->Delta;
+  for (int k = 0; k < nl; k++) {
+#pragma omp parallel for schedule(static)
+    for (int j = 1; j < s->Nyp1; j++)
+      for (int i = 1; i < s->Nxp1; i++) {
+        f1[IDX(s, i, j, k)] = -jac(s, psi, q, i, j, k, k) -
+                              (p->beta * (psi[IDX(s, i + 1, j, k)] - psi[IDX(s, i - 1, j, k)]) / (2 * Delta)) +
+                              p->nu * (q[IDX(s, i + 1, j, k)] + q[IDX(s, i - 1, j, k)] + q[IDX(s, i, j + 1, k)] +
+                                       q[IDX(s, i, j - 1, k)] - 4 * q[IDX(s, i, j, k)]) /
+                                  (Delta * Delta) +
+                              s->nu_hyper * s->lapn[IDX(s, i, j, k)];
+      }
+  }
+  double dhb = p->dh[nl - 1];
+#pragma omp parallel for schedule(static)
+  for (int j = 1; j < s->Nyp1; j++)
+    for (int i = 1; i < s->Nxp1; i++) {
+      /* forcing_q (src/domain.h:33) */
+      f1[IDX(s, i, j, 0)] += (-p->tau0 / p->dh[0] * p->forc_mode * PI_REF / s->Ly *
+                              sin(p->forc_mode * PI_REF * (s->Y[j]) / s->Ly));
+      int k = nl - 1;
+      f1[IDX(s, i, j, nl - 1)] +=
+          -p->hEkb * p->f0 / (2 * dhb) *
+              (psi[IDX(s, i + 1, j, k)] + psi[IDX(s, i - 1, j, k)] + psi[IDX(s, i, j + 1, k)] +
+               psi[IDX(s, i, j - 1, k)] - 4 * psi[IDX(s, i, j, k)]) /
+              (Delta * Delta) -
+          jac(s, psi, s->topo, i, j, nl - 1, 0) * p->f0 / dhb;
+    }
+}
+
+double orc_adjust_timestep(orc *s) {
+  double dt_max = HUGE_REF;
+  double *psi = s->psi;
+  double Delta = s->Delta, cfl = s->p.cfl;
+  for (int k = 0; k < s->p.nl; k++)
+    for (int j = 1; j < s->Nyp1; j++)
+      for (int i = 1; i < s->Nxp1; i++) {
+        double u = -(psi[IDX(s, i + 1, j, k)] - psi[IDX(s, i, j, k)]) / Delta;
+        double v = (psi[IDX(s, i, j + 1, k)] - psi[IDX(s, i, j, k)]) / Delta;
+        u = fabs(u) < fabs(v) ? fabs(v) : fabs(u);
+        if (u != 0.) {
+          double dt_loc = cfl * Delta / u;
+          if (dt_loc < dt_max) dt_max = dt_loc;
+        }
+      }
+  double dt = s->dt;
+  if (dt < dt_max) dt = 2. * dt;
+  if (dt > s->DT_max) dt = s->DT_max;
+  if (dt > dt_max) dt = dt_max;
+  double t = s->t, t_out = s->t_out;
+  if (t_out > t + TEPS) {
+    unsigned int n_step = (t_out - t) / dt;
+    if (n_step == 0)
+      dt = t_out - t;
+    else {
+      double dt1 = (t_out - t) / n_step;
+      if (dt1 > dt * (1. + TEPS))
+        dt = (t_out - t) / (n_step + 1);
+      else if (dt1 < dt)
+        dt = dt1;
+    }
+  }
+  return dt;
+}
+
+void orc_ab3_coeffs(double dt, double dt0, double dt1, double *c) {
+  c[0] = dt * (2 * (dt * dt) + 3 * dt * (2 * dt0 + dt1) + 6 * dt0 * (dt0 + dt1)) / (6 * dt0 * (dt0 + dt1));
+  c[1] = (dt * dt) * (-dt / 3 - dt0 / 2 - dt1 / 2) / (dt0 * dt1);
+  c[2] = (dt * dt) * (dt / 3 + dt0 / 2) / (dt1 * (dt0 + dt1));
+}
+
+void orc_timestep(orc *s) {
+  double *swap = s->f2;
+  s->f2 = s->f1;
+  s->f1 = s->f0;
+  s->f0 = swap;
+  orc_rhs(s, ORC_F0);
+  s->dt1 = s->dt0;
+  s->dt0 = s->dt;
+  if (s->p.dt_fixed > 0)
+    s->dt = s->p.dt_fixed;
+  else
+    s->dt = orc_adjust_timestep(s);
+  double c[3];
+  orc_ab3_coeffs(s->dt, s->dt0, s->dt1, c);
+  double *q = s->q, *f0 = s->f0, *f1 = s->f1, *f2 = s->f2;
+  for (int k = 0; k < s->p.nl; k++) {
+#pragma omp parallel for schedule(static)
+    for (int j = 1; j < s->Nyp1; j++)
+      for (int i = 1; i < s->Nxp1; i++) {
+        long o = IDX(s, i, j, k);
+        q[o] = q[o] + c[0] * f0[o] + c[1] * f1[o] + c[2] * f2[o];
+      }
+  }
+  s->t = s->t + s->dt;
+  s->it++;
+}
+
+void orc_run(orc *s, int nsteps) {
+  for (int i = 0; i < nsteps; i++) orc_timestep(s);
+}
+
+void orc_get_time(orc *s, double *t, double *dt, double *dt0, double *dt1, int *it, double *DT_max) {
+  if (t) *t = s->t;
+  if (dt) *dt = s->dt;
+  if (dt0) *dt0 = s->dt0;
+  if (dt1) *dt1 = s->dt1;
+  if (it) *it = s->it;
+  if (DT_max) *DT_max = s->DT_max;
+}
+
+void orc_set_time(orc *s, double t, double dt, double dt0, double dt1, int it, double t_out) {
+  s->t = t;
+  s->dt = dt;
+  s->dt0 = dt0;
+  s->dt1 = dt1;
+  s->it = it;
+  s->t_out = t_out;
+}
+
+void orc_diagnostics(orc *s, double *E_layer, double *Z_layer, double *E_tot, double *Z_tot) {
+  int nl = s->p.nl;
+  size_t S = (size_t)s->Nxp2 * s->Nyp2;
+  double w = s->Delta * s->Delta / (s->Lx * s->Lx);
+  double H = 0, Et = 0, Zt = 0;
+  for (int l = 0; l < nl; l++) H += s->p.dh[l];
+  for (int l = 0; l < nl; l++) {
+    double e = 0, z = 0;
+    for (size_t t = 0; t < S; t++) {
+      e += s->psi[l * S + t] * s->q[l * S + t];
+      z += s->q[l * S + t] * s->q[l * S + t];
+    }
+    e = -e * w;
+    z = z * w;
+    if (E_layer) E_layer[l] = e;
+    if (Z_layer) Z_layer[l] = z;
+    Et += s->p.dh[l] * e;
+    Zt += s->p.dh[l] * z;
+  }
+  if (E_tot) *E_tot = Et / (2 * H);
+  if (Z_tot) *Z_tot = Zt / H;
+}
+
+/* The stencil macros on caller-supplied fields (storage layout of this
+ * state, nl layers), for the identity checks of the tests: out = J(p,q) or
+ * laplacian(p) at interior points, zero elsewhere. */
+void orc_jacobian_field(orc *s, const double *p, const double *q, double *out) {
+  size_t S = (size_t)s->Nxp2 * s->Nyp2 * s->p.nl;
+  for (size_t t = 0; t < S; t++) out[t] = 0.0;
+  for (int k = 0; k < s->p.nl; k++)
+    for (int j = 1; j < s->Nyp1; j++)
+      for (int i = 1; i < s->Nxp1; i++) out[IDX(s, i, j, k)] = jac(s, p, q, i, j, k, k);
+}
+
+void orc_laplacian_field(orc *s, const double *p, double *out) {
+  size_t S = (size_t)s->Nxp2 * s->Nyp2 * s->p.nl;
+  for (size_t t = 0; t < S; t++) out[t] = 0.0;
+  for (int k = 0; k < s->p.nl; k++)
+    for (int j = 1; j < s->Nyp1; j++)
+      for (int i = 1; i < s->Nxp1; i++) out[IDX(s, i, j, k)] = lap(s, p, i, j, k);
+}

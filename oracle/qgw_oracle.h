@@ -1,0 +1,101 @@
+/*
+ * qgw_oracle.h -- TEST INFRASTRUCTURE ONLY.
+ *
+ * The oracle is a CPU FP64 restatement of the reference's timestep hot path
+ * (bounded branch), written from the reference sources in their loop order,
+ * with its own FFT-based sine transform and its own (Jacobi) eigen-solver in
+ * place of FFTW and LAPACKE, which are absent here.  Only the tests, the
+ * smoke check and the benchmark's CPU-baseline leg may load it; the product
+ * (libqgw_hip.so, libqgw_host.so) never does.
+ *
+ * Function -> reference map (file:line):
+ *   orc_create ............. init_domain src/domain.h:64-176, init_elliptic
+ *                            src/elliptic.h:29-74, init_vars src/domain.h:178-212,
+ *                            init_timestep/check_timestep src/timestep.h:56-67,18-54
+ *   orc_modes .............. compute_eigmode src/eigmode.h:87-271
+ *   orc_invert_pv .......... invert_pv src/elliptic.h:76-155 (bounded branch)
+ *   orc_adjust_bc .......... adjust_bc src/domain.h:221-349 (bounded branch)
+ *   orc_calc_diff .......... calc_diff src/dynamics.h:13-44
+ *   orc_rhs ................ rhs src/dynamics.h:47-73 with the stencil macros
+ *                            src/domain.h:12-36
+ *   orc_adjust_timestep .... adjust_timestep src/timestep.h:69-111
+ *   orc_timestep ........... timestep src/timestep.h:114-167 (no stochastic or
+ *                            4-D forcing), AB3 coefficients doc/timestep.org:95-97
+ *   orc_dst2d / orc_dst1d .. FFTW RODFT00 definition (src/elliptic.h:63-66)
+ *   orc_diagnostics ........ test/test_energy_injection.py:24-25
+ *
+ * Pins (each checked in tests/): DST against scipy.fft.dst(type=1) fixtures,
+ * the O(n^2) direct sum and DST o DST = 2(n+1) I; eigenmodes against
+ * scipy.linalg.eigh fixtures, the nl=2 closed form and the invariants;
+ * invert_pv against single-sine-mode closed forms; the Jacobian against its
+ * antisymmetry and exactness identities and a brute-force 12-term sum; AB3
+ * coefficients against the sympy result of doc/timestep.org; adjust_timestep
+ * (source restatement only) against hand-computed cases of the macro
+ * expansions.
+ */
+#ifndef QGW_ORACLE_H
+#define QGW_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORC_NL_MAX 1000
+
+typedef struct orc_params {
+  int NX, NY, nl, n_hyper;
+  double Lx;
+  double dh[ORC_NL_MAX];
+  double N2[ORC_NL_MAX];
+  double tau0, forc_mode, f0, beta, nu, nu_hyper, hEkb, Ld, h_topo, w_topo, bc_fac;
+  double cfl, sigma_f, tend, dt_out;
+  double dt_fixed; /* > 0: fixed step extension */
+} orc_params;
+
+typedef struct orc orc;
+
+/* fields for orc_field */
+#define ORC_Q 1
+#define ORC_PSI 2
+#define ORC_TOPO 3
+#define ORC_F0 4
+#define ORC_F1 5
+#define ORC_F2 6
+#define ORC_LAPN 7 /* lap_n_diff */
+
+orc *orc_create(const orc_params *p, int *status);
+void orc_destroy(orc *s);
+int orc_modes(orc *s, double *cl2m, double *cm2l, double *iRd2);
+int orc_set_modes(orc *s, const double *cl2m, const double *cm2l, const double *iRd2);
+double *orc_field(orc *s, int which);
+void orc_sizes(orc *s, int *n, int *m, int *nl);
+void orc_invert_pv(orc *s);
+void orc_adjust_bc(orc *s, double *field, int type);
+void orc_calc_diff(orc *s);
+void orc_rhs(orc *s, int which_out);
+double orc_adjust_timestep(orc *s);
+void orc_timestep(orc *s);
+void orc_run(orc *s, int nsteps);
+void orc_get_time(orc *s, double *t, double *dt, double *dt0, double *dt1, int *it, double *DT_max);
+void orc_set_time(orc *s, double t, double dt, double dt0, double dt1, int it, double t_out);
+void orc_diagnostics(orc *s, double *E_layer, double *Z_layer, double *E_tot, double *Z_tot);
+void orc_ab3_coeffs(double dt, double dt0, double dt1, double *c);
+double orc_nu_hyper(orc *s);
+
+/* stand-alone transforms and solvers */
+int orc_dst1d(double *x, int n);           /* unnormalised RODFT00, in place */
+int orc_dst2d(double *a, int n, int m);    /* m rows of n (x fastest), in place */
+int orc_fft(double *re, double *im, int L);/* forward complex DFT, in place */
+int orc_eig_sym(int n, const double *A, double *w, double *V); /* Jacobi, ascending */
+void orc_check_timestep(const orc_params *p, double *dt, double *DT_max);
+int orc_threads(int n);                    /* set OpenMP threads (0: default); returns count */
+void orc_jacobian_field(orc *s, const double *p, const double *q, double *out);
+void orc_laplacian_field(orc *s, const double *p, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

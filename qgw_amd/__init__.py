@@ -1,0 +1,388 @@
+"""qgw_amd -- the MI355X timestep hot path of the multi-layer QG model qgw.
+
+Python view of the two native libraries built in this directory:
+
+  libqgw_host.so  host C: parameter reader, grid, vertical modes, seeded
+                  initial condition, FP64 diagnostics (include/qgw_host.h)
+  libqgw_hip.so   HIP for gfx950: the per-step hot path behind a C-ABI
+                  (include/qgw_hip.h)
+
+The names follow the reference's own call boundaries (src/qg.c:153-200):
+``Model.invert_pv`` (src/elliptic.h:76), ``Model.rhs`` (src/dynamics.h:47),
+``Model.adjust_timestep`` (src/timestep.h:69) and ``Model.timestep``
+(src/timestep.h:114).  Arrays are numpy float64 in the reference storage
+layout ``idx(i,j,k) = k*Nxp2*Nyp2 + j*Nxp2 + i`` (src/domain.h:62), i.e. shape
+``(nl, m+2, n+2)``.
+
+Every device call goes through libqgw_hip.so; there is no CPU fallback.  If
+the library is missing or no GPU is present the calls raise.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+NL_MAX = 1000
+
+FP64, FP32, FP32_PLAIN = 0, 1, 2
+FIELD_Q, FIELD_PSI, FIELD_TOPO, FIELD_F0, FIELD_F1, FIELD_F2 = 1, 2, 3, 4, 5, 6
+PASS_NAMES = ["P1", "T1", "P2", "T2", "P3", "P4", "CFL", "DIAG"]
+
+QGW_OK = 0
+
+
+class QGWError(RuntimeError):
+    pass
+
+
+class Params(ctypes.Structure):
+    """Mirror of ``qgw_params`` (include/qgw_params.h)."""
+
+    _fields_ = [
+        ("NX", ctypes.c_int), ("NY", ctypes.c_int), ("nl", ctypes.c_int), ("n_hyper", ctypes.c_int),
+        ("Lx", ctypes.c_double), ("dh", ctypes.c_double * NL_MAX),
+        ("tau0", ctypes.c_double), ("forc_mode", ctypes.c_double), ("f0", ctypes.c_double),
+        ("beta", ctypes.c_double), ("nu", ctypes.c_double), ("nu_hyper", ctypes.c_double),
+        ("hEkb", ctypes.c_double), ("N2", ctypes.c_double * NL_MAX),
+        ("Ld", ctypes.c_double), ("h_topo", ctypes.c_double), ("w_topo", ctypes.c_double),
+        ("bc_fac", ctypes.c_double), ("tend", ctypes.c_double), ("dt_out", ctypes.c_double),
+        ("dt_forc", ctypes.c_double), ("dt_forc_period", ctypes.c_double), ("sigma_f", ctypes.c_double),
+        ("k_f", ctypes.c_double), ("cfl", ctypes.c_double),
+        ("dt_fixed", ctypes.c_double), ("nsteps", ctypes.c_int), ("ic_kind", ctypes.c_int),
+        ("seed", ctypes.c_uint64), ("ic_urms", ctypes.c_double), ("ic_lambda0", ctypes.c_double),
+        ("precision", ctypes.c_int), ("verbose", ctypes.c_int),
+    ]
+
+    def set(self, **kw):
+        for k, v in kw.items():
+            if k in ("dh", "N2"):
+                arr = getattr(self, k)
+                for i, x in enumerate(v):
+                    arr[i] = float(x)
+            else:
+                setattr(self, k, v)
+        return self
+
+    def as_dict(self):
+        d = {}
+        for name, _ in self._fields_:
+            v = getattr(self, name)
+            if name in ("dh", "N2"):
+                v = [v[i] for i in range(max(self.nl, 1))]
+            d[name] = v
+        return d
+
+
+class Grid(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int), ("m", ctypes.c_int), ("nxp2", ctypes.c_int), ("nyp2", ctypes.c_int),
+                ("Delta", ctypes.c_double), ("Lx", ctypes.c_double), ("Ly", ctypes.c_double)]
+
+
+_host = None
+_hip = None
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_dp)
+
+
+def host_lib():
+    """libqgw_host.so (pure C, usable without a GPU)."""
+    global _host
+    if _host is None:
+        path = os.path.join(HERE, "libqgw_host.so")
+        if not os.path.exists(path):
+            raise QGWError("libqgw_host.so not built (run __graft_entry__.build())")
+        lib = ctypes.CDLL(path, mode=ctypes.RTLD_GLOBAL)
+        P = ctypes.POINTER(Params)
+        lib.qgwh_default_params.argtypes = [P]
+        lib.qgwh_read_params.argtypes = [ctypes.c_char_p, P]
+        lib.qgwh_grid.argtypes = [P, ctypes.POINTER(Grid), _dp, _dp, _dp, _dp]
+        lib.qgwh_topo.argtypes = [P, _dp]
+        lib.qgwh_eigmode.argtypes = [P, _dp, _dp, _dp]
+        lib.qgwh_gamma.argtypes = [P, _dp]
+        lib.qgwh_wind_row.argtypes = [P, _dp]
+        lib.qgwh_make_ic.argtypes = [P, _dp, _dp, _dp, ctypes.c_uint64, _dp, _dp]
+        lib.qgwh_dst2d.argtypes = [_dp, ctypes.c_int, ctypes.c_int]
+        lib.qgwh_diagnostics.argtypes = [P, _dp, _dp, _dp, _dp, _dp, _dp]
+        lib.qgwh_check_timestep.argtypes = [P, ctypes.c_double, _dp, _dp]
+        lib.qgwh_nu_hyper_signed.argtypes = [P]
+        lib.qgwh_nu_hyper_signed.restype = ctypes.c_double
+        lib.qgwh_strerror.argtypes = [ctypes.c_int]
+        lib.qgwh_strerror.restype = ctypes.c_char_p
+        lib.qgwh_params_size.restype = ctypes.c_size_t
+        if lib.qgwh_params_size() != ctypes.sizeof(Params):
+            raise QGWError("qgw_params layout mismatch between C and Python")
+        _host = lib
+    return _host
+
+
+def hip_lib():
+    """libqgw_hip.so (needs the HIP runtime; compute calls need a GPU)."""
+    global _hip
+    if _hip is None:
+        host_lib()
+        path = os.path.join(HERE, "libqgw_hip.so")
+        if not os.path.exists(path):
+            raise QGWError("libqgw_hip.so not built (run __graft_entry__.build())")
+        lib = ctypes.CDLL(path, mode=ctypes.RTLD_GLOBAL)
+        vp = ctypes.c_void_p
+        P = ctypes.POINTER(Params)
+        ip = ctypes.POINTER(ctypes.c_int)
+        lib.qgw_create.argtypes = [P, ctypes.c_int, ip]
+        lib.qgw_create.restype = vp
+        for name, args in [
+            ("qgw_destroy", [vp]), ("qgw_set_modes", [vp, _dp, _dp, _dp]),
+            ("qgw_upload", [vp, ctypes.c_int, _dp]), ("qgw_download", [vp, ctypes.c_int, _dp]),
+            ("qgw_invert_pv", [vp]), ("qgw_rhs", [vp]), ("qgw_adjust_timestep", [vp, _dp]),
+            ("qgw_timestep", [vp]), ("qgw_run", [vp, ctypes.c_int]),
+            ("qgw_diagnostics", [vp, _dp, _dp, _dp, _dp]),
+            ("qgw_get_time", [vp, _dp, _dp, _dp, _dp, ip]),
+            ("qgw_set_time", [vp, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                              ctypes.c_int]),
+            ("qgw_set_next_output", [vp, ctypes.c_double]),
+            ("qgw_synchronize", [vp]), ("qgw_time_run", [vp, ctypes.c_int, _dp]),
+            ("qgw_profile_enable", [vp, ctypes.c_int]), ("qgw_profile_read", [vp, _dp, ip, ctypes.c_int]),
+            ("qgw_device_bytes", [vp, ctypes.POINTER(ctypes.c_size_t)]),
+        ]:
+            f = getattr(lib, name)
+            f.argtypes = args
+            f.restype = ctypes.c_int
+        lib.qgw_strerror.argtypes = [ctypes.c_int]
+        lib.qgw_strerror.restype = ctypes.c_char_p
+        lib.qgw_last_error.argtypes = [vp]
+        lib.qgw_last_error.restype = ctypes.c_char_p
+        _hip = lib
+    return _hip
+
+
+# ---------------------------------------------------------------------------
+# host helpers
+# ---------------------------------------------------------------------------
+
+def default_params(**kw):
+    p = Params()
+    host_lib().qgwh_default_params(ctypes.byref(p))
+    return p.set(**kw)
+
+
+def read_params(path, p=None):
+    """Parse a reference-style params.in (src/extra.h:98-153 semantics)."""
+    if p is None:
+        p = default_params()
+    st = host_lib().qgwh_read_params(os.fsencode(path), ctypes.byref(p))
+    if st != QGW_OK:
+        raise QGWError("read_params(%s): %s" % (path, host_lib().qgwh_strerror(st).decode()))
+    return p
+
+
+def grid(p):
+    g = Grid()
+    n1, m1 = p.NX + 1, p.NY + 1
+    X, Y, K, L = np.zeros(n1), np.zeros(m1), np.zeros(m1), np.zeros(n1)
+    st = host_lib().qgwh_grid(ctypes.byref(p), ctypes.byref(g), _ptr(X), _ptr(Y), _ptr(K), _ptr(L))
+    if st != QGW_OK:
+        raise QGWError("grid: unsupported parameters")
+    return g, X, Y, K, L
+
+
+def eigmode(p):
+    """(cl2m, cm2l, iRd2) as flat float64 arrays (src/eigmode.h:87-271)."""
+    nl = p.nl
+    cl2m, cm2l, iRd2 = np.zeros(nl * nl), np.zeros(nl * nl), np.zeros(nl)
+    st = host_lib().qgwh_eigmode(ctypes.byref(p), _ptr(cl2m), _ptr(cm2l), _ptr(iRd2))
+    if st != QGW_OK:
+        raise QGWError("eigmode failed: %s" % host_lib().qgwh_strerror(st).decode())
+    return cl2m, cm2l, iRd2
+
+
+def gamma(p):
+    G = np.zeros(p.nl * p.nl)
+    host_lib().qgwh_gamma(ctypes.byref(p), _ptr(G))
+    return G.reshape(p.nl, p.nl)
+
+
+def topo(p):
+    g = grid(p)[0]
+    t = np.zeros((g.nyp2, g.nxp2))
+    host_lib().qgwh_topo(ctypes.byref(p), _ptr(t))
+    return t
+
+
+def wind_row(p):
+    g = grid(p)[0]
+    w = np.zeros(g.nyp2)
+    host_lib().qgwh_wind_row(ctypes.byref(p), _ptr(w))
+    return w
+
+
+def make_ic(p, modes=None, seed=None):
+    """Seeded initial condition (psi, q), each shape (nl, m+2, n+2)."""
+    g = grid(p)[0]
+    if modes is None:
+        modes = eigmode(p)
+    cl2m, cm2l, iRd2 = [np.ascontiguousarray(a, dtype=np.float64) for a in modes]
+    psi = np.zeros((p.nl, g.nyp2, g.nxp2))
+    q = np.zeros_like(psi)
+    s = p.seed if seed is None else seed
+    st = host_lib().qgwh_make_ic(ctypes.byref(p), _ptr(cl2m), _ptr(cm2l), _ptr(iRd2), ctypes.c_uint64(s),
+                                 _ptr(psi), _ptr(q))
+    if st != QGW_OK:
+        raise QGWError("make_ic failed: %s" % host_lib().qgwh_strerror(st).decode())
+    return psi, q
+
+
+def dst2d(a):
+    """Unnormalised 2-D RODFT00 of a (m, n) array (host implementation)."""
+    b = np.ascontiguousarray(a, dtype=np.float64).copy()
+    m, n = b.shape
+    host_lib().qgwh_dst2d(_ptr(b), n, m)
+    return b
+
+
+def diagnostics_host(p, psi, q):
+    nl = p.nl
+    El, Zl = np.zeros(nl), np.zeros(nl)
+    Et, Zt = ctypes.c_double(), ctypes.c_double()
+    psi = np.ascontiguousarray(psi, dtype=np.float64)
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    host_lib().qgwh_diagnostics(ctypes.byref(p), _ptr(psi), _ptr(q), _ptr(El), _ptr(Zl),
+                                ctypes.byref(Et), ctypes.byref(Zt))
+    return El, Zl, Et.value, Zt.value
+
+
+def check_timestep(p):
+    dt, DT = ctypes.c_double(), ctypes.c_double()
+    h = host_lib()
+    h.qgwh_check_timestep(ctypes.byref(p), h.qgwh_nu_hyper_signed(ctypes.byref(p)), ctypes.byref(dt),
+                          ctypes.byref(DT))
+    return dt.value, DT.value
+
+
+# ---------------------------------------------------------------------------
+# device model
+# ---------------------------------------------------------------------------
+
+class Model:
+    """One run on the GPU: a ``qgw_ctx`` of libqgw_hip.so."""
+
+    def __init__(self, p, precision=FP64, modes=None):
+        self.lib = hip_lib()
+        self.p = p
+        self.precision = precision
+        st = ctypes.c_int(0)
+        self.ctx = self.lib.qgw_create(ctypes.byref(p), precision, ctypes.byref(st))
+        if not self.ctx:
+            raise QGWError("qgw_create failed: %s" % self.lib.qgw_strerror(st.value).decode())
+        g = grid(p)[0]
+        self.n, self.m, self.nl = g.n, g.m, p.nl
+        self.shape = (p.nl, g.nyp2, g.nxp2)
+        if modes is None:
+            modes = eigmode(p)
+        self.set_modes(*modes)
+
+    def _chk(self, st, what):
+        if st != QGW_OK:
+            raise QGWError("%s: %s (%s)" % (what, self.lib.qgw_strerror(st).decode(),
+                                            self.lib.qgw_last_error(self.ctx).decode()))
+
+    def close(self):
+        if self.ctx:
+            self.lib.qgw_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_modes(self, cl2m, cm2l, iRd2):
+        a = [np.ascontiguousarray(x, dtype=np.float64) for x in (cl2m, cm2l, iRd2)]
+        self._chk(self.lib.qgw_set_modes(self.ctx, _ptr(a[0]), _ptr(a[1]), _ptr(a[2])), "set_modes")
+
+    def upload(self, field, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        need = self.shape[1] * self.shape[2] * (1 if field == FIELD_TOPO else self.nl)
+        if a.size != need:
+            raise QGWError("upload: expected %d values, got %d" % (need, a.size))
+        self._chk(self.lib.qgw_upload(self.ctx, field, _ptr(a)), "upload")
+
+    def download(self, field):
+        shape = (self.shape[1], self.shape[2]) if field == FIELD_TOPO else self.shape
+        a = np.zeros(shape)
+        self._chk(self.lib.qgw_download(self.ctx, field, _ptr(a)), "download")
+        return a
+
+    def set_q(self, q):
+        self.upload(FIELD_Q, q)
+
+    @property
+    def q(self):
+        return self.download(FIELD_Q)
+
+    @property
+    def psi(self):
+        return self.download(FIELD_PSI)
+
+    def invert_pv(self):
+        self._chk(self.lib.qgw_invert_pv(self.ctx), "invert_pv")
+
+    def rhs(self):
+        self._chk(self.lib.qgw_rhs(self.ctx), "rhs")
+        return self.download(FIELD_F0)
+
+    def adjust_timestep(self):
+        dt = ctypes.c_double()
+        self._chk(self.lib.qgw_adjust_timestep(self.ctx, ctypes.byref(dt)), "adjust_timestep")
+        return dt.value
+
+    def timestep(self):
+        self._chk(self.lib.qgw_timestep(self.ctx), "timestep")
+
+    def run(self, nsteps):
+        self._chk(self.lib.qgw_run(self.ctx, int(nsteps)), "run")
+
+    def time_run(self, nsteps):
+        ms = ctypes.c_double()
+        self._chk(self.lib.qgw_time_run(self.ctx, int(nsteps), ctypes.byref(ms)), "time_run")
+        return ms.value
+
+    def synchronize(self):
+        self._chk(self.lib.qgw_synchronize(self.ctx), "synchronize")
+
+    def diagnostics(self):
+        El, Zl = np.zeros(self.nl), np.zeros(self.nl)
+        Et, Zt = ctypes.c_double(), ctypes.c_double()
+        self._chk(self.lib.qgw_diagnostics(self.ctx, _ptr(El), _ptr(Zl), ctypes.byref(Et), ctypes.byref(Zt)),
+                  "diagnostics")
+        return El, Zl, Et.value, Zt.value
+
+    def get_time(self):
+        t, dt, dt0, dt1 = (ctypes.c_double() for _ in range(4))
+        it = ctypes.c_int()
+        self._chk(self.lib.qgw_get_time(self.ctx, ctypes.byref(t), ctypes.byref(dt), ctypes.byref(dt0),
+                                        ctypes.byref(dt1), ctypes.byref(it)), "get_time")
+        return dict(t=t.value, dt=dt.value, dt0=dt0.value, dt1=dt1.value, it=it.value)
+
+    def set_time(self, t, dt, dt0, dt1, it):
+        self._chk(self.lib.qgw_set_time(self.ctx, t, dt, dt0, dt1, it), "set_time")
+
+    def set_next_output(self, t_out):
+        self._chk(self.lib.qgw_set_next_output(self.ctx, t_out), "set_next_output")
+
+    def profile(self, on=True):
+        self._chk(self.lib.qgw_profile_enable(self.ctx, 1 if on else 0), "profile_enable")
+
+    def profile_read(self):
+        ms = np.zeros(len(PASS_NAMES))
+        cnt = (ctypes.c_int * len(PASS_NAMES))()
+        self._chk(self.lib.qgw_profile_read(self.ctx, _ptr(ms), cnt, len(PASS_NAMES)), "profile_read")
+        return {PASS_NAMES[i]: (ms[i], cnt[i]) for i in range(len(PASS_NAMES))}
+
+    def device_bytes(self):
+        b = ctypes.c_size_t()
+        self._chk(self.lib.qgw_device_bytes(self.ctx, ctypes.byref(b)), "device_bytes")
+        return b.value

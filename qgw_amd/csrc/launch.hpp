@@ -1,0 +1,81 @@
+// launch.hpp -- launch descriptors shared by the context code (qgw_ctx.hip)
+// and the per-precision kernel instantiation units (qgw_launch_f32.hip,
+// qgw_launch_f64.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "qgw_hip.h"
+#include "kernels.hpp"
+
+namespace qgw {
+
+// Layers handled by the register-resident projections of P1/P3.
+constexpr int kMaxLayers = 16;
+
+struct LaunchArgs {
+  Geo g{};
+  int prec = 0;
+  int Lx = 0, Ly = 0;
+  hipStream_t stream = nullptr;
+  void *q = nullptr, *qn = nullptr, *psi = nullptr, *what = nullptr, *whatT = nullptr;
+  void *cl2m = nullptr, *cm2l = nullptr, *twx = nullptr, *twy = nullptr;
+  void *fnew = nullptr, *f1 = nullptr, *f2 = nullptr, *comp = nullptr, *topo = nullptr, *wind = nullptr;
+  const double *La2 = nullptr, *Kb2 = nullptr, *iRd2 = nullptr;
+  double norm = 1.0, ring_c = 0.0;
+  StepCoef cf{};
+  int nlap = 0;
+  unsigned long long *cfl_max = nullptr;
+};
+
+// Per-precision entry points (defined in qgw_launch_f32.hip / qgw_launch_f64.hip).
+template <typename T> int launch_p1_t(const LaunchArgs &a);
+template <typename T> int launch_t1_t(const LaunchArgs &a);
+template <typename T> int launch_p2_t(const LaunchArgs &a);
+template <typename T> int launch_t2_t(const LaunchArgs &a);
+template <typename T> int launch_p3_t(const LaunchArgs &a);
+template <typename T> int launch_p4_t(const LaunchArgs &a);
+template <typename T> int launch_cfl_t(const LaunchArgs &a);
+template <typename T> int launch_diag_t(const LaunchArgs &a, double *part, int nblk, double *out);
+template <typename T> int launch_from_f64_t(const LaunchArgs &a, int layers, const double *src, void *dst, void *resid);
+template <typename T> int launch_to_f64_t(const LaunchArgs &a, int layers, const void *src, const void *resid, double *dst);
+bool launch_has_length(int L);
+
+// The specialisations live in the two instantiation units.
+#define QGW_DECLARE_SPECS(T)                                                                          \
+  template <> int launch_p1_t<T>(const LaunchArgs &a);                                                \
+  template <> int launch_t1_t<T>(const LaunchArgs &a);                                                \
+  template <> int launch_p2_t<T>(const LaunchArgs &a);                                                \
+  template <> int launch_t2_t<T>(const LaunchArgs &a);                                                \
+  template <> int launch_p3_t<T>(const LaunchArgs &a);                                                \
+  template <> int launch_p4_t<T>(const LaunchArgs &a);                                                \
+  template <> int launch_cfl_t<T>(const LaunchArgs &a);                                               \
+  template <> int launch_diag_t<T>(const LaunchArgs &a, double *part, int nblk, double *out);         \
+  template <> int launch_from_f64_t<T>(const LaunchArgs &a, int layers, const double *src, void *dst, \
+                                       void *resid);                                                  \
+  template <> int launch_to_f64_t<T>(const LaunchArgs &a, int layers, const void *src, const void *resid, \
+                                     double *dst);
+QGW_DECLARE_SPECS(float)
+QGW_DECLARE_SPECS(double)
+#undef QGW_DECLARE_SPECS
+
+#define QGW_DISPATCH(fn, a, ...) \
+  ((a).prec == QGW_FP64 ? fn##_t<double>((a), ##__VA_ARGS__) : fn##_t<float>((a), ##__VA_ARGS__))
+
+inline int launch_p1(const LaunchArgs &a) { return QGW_DISPATCH(launch_p1, a); }
+inline int launch_t1(const LaunchArgs &a) { return QGW_DISPATCH(launch_t1, a); }
+inline int launch_p2(const LaunchArgs &a) { return QGW_DISPATCH(launch_p2, a); }
+inline int launch_t2(const LaunchArgs &a) { return QGW_DISPATCH(launch_t2, a); }
+inline int launch_p3(const LaunchArgs &a) { return QGW_DISPATCH(launch_p3, a); }
+inline int launch_p4(const LaunchArgs &a) { return QGW_DISPATCH(launch_p4, a); }
+inline int launch_cfl(const LaunchArgs &a) { return QGW_DISPATCH(launch_cfl, a); }
+inline int launch_diag(const LaunchArgs &a, double *part, int nblk, double *out) {
+  return QGW_DISPATCH(launch_diag, a, part, nblk, out);
+}
+inline int launch_from_f64(const LaunchArgs &a, int layers, const double *src, void *dst, void *resid) {
+  return QGW_DISPATCH(launch_from_f64, a, layers, src, dst, resid);
+}
+inline int launch_to_f64(const LaunchArgs &a, int layers, const void *src, const void *resid, double *dst) {
+  return QGW_DISPATCH(launch_to_f64, a, layers, src, resid, dst);
+}
+
+}  // namespace qgw

@@ -1,0 +1,151 @@
+// launch_impl.hpp -- kernel instantiation and launch for one arithmetic type.
+// Included exactly once by qgw_launch_f32.hip (QGW_T = float) and by
+// qgw_launch_f64.hip (QGW_T = double).
+#pragma once
+#ifndef QGW_T
+#error "define QGW_T before including launch_impl.hpp"
+#endif
+#include "launch.hpp"
+
+namespace qgw {
+
+using T = QGW_T;
+
+// Supported transform lengths L = 2(n+1): 2^a (8..8192) and 3*2^a (24..6144).
+#define QGW_LENGTHS(X) \
+  X(8) X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(2048) X(4096) X(8192) \
+  X(24) X(48) X(96) X(192) X(384) X(768) X(1536) X(3072) X(6144)
+
+template <> int launch_p1_t<T>(const LaunchArgs &a) {
+  const dim3 grid(a.g.m);
+  switch (a.Lx) {
+#define QGW_CASE(L)                                                                                  \
+  case L:                                                                                            \
+    if (a.g.nl <= 4)                                                                                 \
+      hipLaunchKernelGGL((k_p1<T, L, 4>), grid, dim3(fft_threads(L)), 0, a.stream, a.g,             \
+                         (const T *)a.q, (T *)a.what, (const T *)a.cl2m, (const cx<T> *)a.twx);    \
+    else                                                                                             \
+      hipLaunchKernelGGL((k_p1<T, L, kMaxLayers>), grid, dim3(fft_threads(L)), 0, a.stream, a.g,    \
+                         (const T *)a.q, (T *)a.what, (const T *)a.cl2m, (const cx<T> *)a.twx);    \
+    return 0;
+    QGW_LENGTHS(QGW_CASE)
+#undef QGW_CASE
+  }
+  return QGW_ERR_ARG;
+}
+
+template <> int launch_p3_t<T>(const LaunchArgs &a) {
+  const dim3 grid(a.g.m);
+  switch (a.Lx) {
+#define QGW_CASE(L)                                                                                   \
+  case L:                                                                                             \
+    if (a.g.nl <= 4)                                                                                  \
+      hipLaunchKernelGGL((k_p3<T, L, 4>), grid, dim3(fft_threads(L)), 0, a.stream, a.g,              \
+                         (const T *)a.what, (T *)a.psi, (T *)a.q, (const T *)a.cm2l, (T)a.ring_c,    \
+                         (const cx<T> *)a.twx);                                                       \
+    else                                                                                              \
+      hipLaunchKernelGGL((k_p3<T, L, kMaxLayers>), grid, dim3(fft_threads(L)), 0, a.stream, a.g,     \
+                         (const T *)a.what, (T *)a.psi, (T *)a.q, (const T *)a.cm2l, (T)a.ring_c,    \
+                         (const cx<T> *)a.twx);                                                       \
+    return 0;
+    QGW_LENGTHS(QGW_CASE)
+#undef QGW_CASE
+  }
+  return QGW_ERR_ARG;
+}
+
+template <> int launch_p2_t<T>(const LaunchArgs &a) {
+  const long rows = (long)a.g.nl * a.g.n;
+  const dim3 grid((unsigned)((rows + 1) / 2));
+  switch (a.Ly) {
+#define QGW_CASE(L)                                                                               \
+  case L:                                                                                         \
+    hipLaunchKernelGGL((k_p2<T, L>), grid, dim3(fft_threads(L)), 0, a.stream, a.g, (T *)a.whatT, \
+                       a.La2, a.Kb2, a.iRd2, a.norm, (const cx<T> *)a.twy);                       \
+    return 0;
+    QGW_LENGTHS(QGW_CASE)
+#undef QGW_CASE
+  }
+  return QGW_ERR_ARG;
+}
+
+constexpr int kTile = (sizeof(T) == 8) ? 32 : 64;
+
+template <> int launch_t1_t<T>(const LaunchArgs &a) {
+  // what [nl][m][Pn] -> whatT [nl][n][Pm]
+  const dim3 grid((a.g.n + kTile - 1) / kTile, (a.g.m + kTile - 1) / kTile, a.g.nl);
+  hipLaunchKernelGGL((k_transpose<T, kTile>), grid, dim3(256), 0, a.stream, (const T *)a.what, (T *)a.whatT,
+                     a.g.m, a.g.n, a.g.Pn, a.g.WS, a.g.Pm, a.g.WTS);
+  return 0;
+}
+
+template <> int launch_t2_t<T>(const LaunchArgs &a) {
+  const dim3 grid((a.g.m + kTile - 1) / kTile, (a.g.n + kTile - 1) / kTile, a.g.nl);
+  hipLaunchKernelGGL((k_transpose<T, kTile>), grid, dim3(256), 0, a.stream, (const T *)a.whatT, (T *)a.what,
+                     a.g.n, a.g.m, a.g.Pm, a.g.WTS, a.g.Pn, a.g.WS);
+  return 0;
+}
+
+constexpr int kTX = 64, kTY = 16;
+
+template <int NLAP, int KAHAN> static void p4_go(const LaunchArgs &a) {
+  const dim3 grid((a.g.n + kTX - 1) / kTX, (a.g.m + kTY - 1) / kTY, a.g.nl);
+  hipLaunchKernelGGL((k_p4<T, NLAP, KAHAN, kTX, kTY>), grid, dim3(kTX * 4), 0, a.stream, a.g, (const T *)a.psi,
+                     (const T *)a.q, (T *)a.qn, (T *)a.fnew, (const T *)a.f1, (const T *)a.f2, (T *)a.comp,
+                     (const T *)a.topo, (const T *)a.wind, a.cf);
+}
+
+template <int KAHAN> static int p4_nlap(const LaunchArgs &a) {
+  switch (a.nlap) {
+    case 0: p4_go<0, KAHAN>(a); return 0;
+    case 1: p4_go<1, KAHAN>(a); return 0;
+    case 2: p4_go<2, KAHAN>(a); return 0;
+    case 3: p4_go<3, KAHAN>(a); return 0;
+    case 4: p4_go<4, KAHAN>(a); return 0;
+    case 5: p4_go<5, KAHAN>(a); return 0;
+  }
+  return QGW_ERR_ARG;
+}
+
+template <> int launch_p4_t<T>(const LaunchArgs &a) {
+  if (sizeof(T) == 4 && a.comp) return p4_nlap<1>(a);
+  return p4_nlap<0>(a);
+}
+
+template <> int launch_cfl_t<T>(const LaunchArgs &a) {
+  long I = (long)a.g.n * a.g.m * a.g.nl;
+  long blocks = (I + 255) / 256;
+  if (blocks > 2048) blocks = 2048;
+  hipLaunchKernelGGL((k_cfl<T>), dim3((unsigned)blocks), dim3(256), 0, a.stream, a.g, (const T *)a.psi,
+                     a.cfl_max);
+  return 0;
+}
+
+template <> int launch_diag_t<T>(const LaunchArgs &a, double *part, int nblk, double *out) {
+  hipLaunchKernelGGL((k_diag_partial<T>), dim3(nblk, a.g.nl), dim3(256), 0, a.stream, a.g, (const T *)a.psi,
+                     (const T *)a.q, (const T *)a.comp, part, nblk);
+  hipLaunchKernelGGL((k_diag_final<256>), dim3(a.g.nl), dim3(256), 0, a.stream, (const double *)part, nblk, out);
+  return 0;
+}
+
+template <>
+int launch_from_f64_t<T>(const LaunchArgs &a, int layers, const double *src, void *dst, void *resid) {
+  long tot = (long)(a.g.n + 2) * (a.g.m + 2) * layers;
+  long blocks = (tot + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL((k_from_f64<T>), dim3((unsigned)blocks), dim3(256), 0, a.stream, a.g, layers, src, (T *)dst,
+                     (T *)resid);
+  return 0;
+}
+
+template <>
+int launch_to_f64_t<T>(const LaunchArgs &a, int layers, const void *src, const void *resid, double *dst) {
+  long tot = (long)(a.g.n + 2) * (a.g.m + 2) * layers;
+  long blocks = (tot + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL((k_to_f64<T>), dim3((unsigned)blocks), dim3(256), 0, a.stream, a.g, layers, (const T *)src,
+                     (const T *)resid, dst);
+  return 0;
+}
+
+}  // namespace qgw

@@ -1,0 +1,749 @@
+// qgw_ctx.hip -- the C-ABI of the MI355X timestep hot path (libqgw_hip.so).
+//
+// Host side of every entry point declared in include/qgw_hip.h: context set-up
+// (init_domain/init_vars/init_timestep/init_elliptic, src/domain.h:64,178,
+// src/timestep.h:56, src/elliptic.h:29), upload/download in the reference idx
+// layout, the per-call hot-path boundaries (invert_pv, rhs, adjust_timestep,
+// timestep: src/elliptic.h:76, src/dynamics.h:47, src/timestep.h:69,114) and
+// the host logic of the variable step (src/timestep.h:69-111, 126-133).
+// The kernels are in kernels.hpp / fft.hpp and are instantiated per precision
+// in qgw_launch_f32.hip and qgw_launch_f64.hip.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <vector>
+
+#include "qgw_hip.h"
+#include "launch.hpp"
+
+namespace {
+
+constexpr double kPi = 3.141592653589793;  // src/qg.c:44
+constexpr double kHuge = 1e30;             // src/qg.c:43
+constexpr double kTeps = 1e-9;             // src/timestep.h:16
+
+bool supported_length(int L) {
+  int r = L;
+  while (r % 2 == 0) r /= 2;
+  while (r % 3 == 0) r /= 3;
+  if (r != 1) return false;
+  return qgw::launch_has_length(L);
+}
+
+}  // namespace
+
+struct qgw_ctx {
+  qgw_params p;
+  int prec;
+  size_t esz;
+  qgw::Geo g;
+  qgw_grid grid;
+  int Lx, Ly;
+  hipStream_t stream;
+  void *q[2];
+  int qcur;
+  void *psi, *fa[3], *comp, *topo, *wind, *what, *whatT, *cl2m, *cm2l, *twx, *twy;
+  double *iRd2_d, *La2, *Kb2, *diag_part, *diag_out, *stage;
+  unsigned long long *cfl_max;
+  int nblk_diag;
+  bool modes_set;
+  double t, dt, dt0, dt1, DT_max, t_out;
+  int it;
+  double nu_hyper;
+  size_t bytes;
+  // profiling
+  bool prof;
+  struct Rec {
+    int pass;
+    hipEvent_t a, b;
+  };
+  std::vector<Rec> recs;
+  std::vector<hipEvent_t> evpool;
+  double prof_ms[QGW_NPASS];
+  int prof_cnt[QGW_NPASS];
+  char err[256];
+};
+
+namespace {
+
+int fail_hip(qgw_ctx *c, hipError_t e, const char *what) {
+  if (c) snprintf(c->err, sizeof c->err, "%s: %s", what, hipGetErrorString(e));
+  return QGW_ERR_HIP;
+}
+
+#define HIPCHK(c, call)                                  \
+  do {                                                   \
+    hipError_t _e = (call);                              \
+    if (_e != hipSuccess) return fail_hip((c), _e, #call); \
+  } while (0)
+
+int dalloc(qgw_ctx *c, void **ptr, size_t bytes) {
+  *ptr = nullptr;
+  if (bytes == 0) bytes = 16;
+  hipError_t e = hipMalloc(ptr, bytes);
+  if (e != hipSuccess) {
+    fail_hip(c, e, "hipMalloc");
+    return QGW_ERR_NOMEM;
+  }
+  e = hipMemset(*ptr, 0, bytes);
+  if (e != hipSuccess) return fail_hip(c, e, "hipMemset");
+  c->bytes += bytes;
+  return QGW_OK;
+}
+
+hipEvent_t ev_get(qgw_ctx *c) {
+  if (!c->evpool.empty()) {
+    hipEvent_t e = c->evpool.back();
+    c->evpool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+int prof_flush(qgw_ctx *c) {
+  if (c->recs.empty()) return QGW_OK;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (auto &r : c->recs) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+      c->prof_ms[r.pass] += ms;
+      c->prof_cnt[r.pass] += 1;
+    }
+    c->evpool.push_back(r.a);
+    c->evpool.push_back(r.b);
+  }
+  c->recs.clear();
+  return QGW_OK;
+}
+
+// Bracket one launch with events when profiling.
+struct PassScope {
+  qgw_ctx *c;
+  int pass;
+  hipEvent_t a = nullptr;
+  PassScope(qgw_ctx *cc, int ps) : c(cc), pass(ps) {
+    if (c->prof) {
+      if (c->recs.size() > 4000) prof_flush(c);
+      a = ev_get(c);
+      if (a) hipEventRecord(a, c->stream);
+    }
+  }
+  ~PassScope() {
+    if (c->prof && a) {
+      hipEvent_t b = ev_get(c);
+      if (b) {
+        hipEventRecord(b, c->stream);
+        c->recs.push_back({pass, a, b});
+      }
+    }
+  }
+};
+
+int check_launch(qgw_ctx *c, const char *what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail_hip(c, e, what);
+  return QGW_OK;
+}
+
+template <typename T>
+int make_twiddles(qgw_ctx *c, int L, void **out) {
+  std::vector<T> h(2 * (size_t)L);
+  for (int k = 0; k < L; k++) {
+    double a = -2.0 * kPi * (double)k / (double)L;
+    h[2 * k] = (T)cos(a);
+    h[2 * k + 1] = (T)sin(a);
+  }
+  int st = dalloc(c, out, sizeof(T) * 2 * L);
+  if (st) return st;
+  HIPCHK(c, hipMemcpy(*out, h.data(), sizeof(T) * 2 * L, hipMemcpyHostToDevice));
+  return QGW_OK;
+}
+
+template <typename T>
+int upload_vec(qgw_ctx *c, void *dst, const double *src, size_t cnt) {
+  std::vector<T> h(cnt);
+  for (size_t i = 0; i < cnt; i++) h[i] = (T)src[i];
+  HIPCHK(c, hipMemcpy(dst, h.data(), sizeof(T) * cnt, hipMemcpyHostToDevice));
+  return QGW_OK;
+}
+
+int upload_typed(qgw_ctx *c, void *dst, const double *src, size_t cnt) {
+  return c->prec == QGW_FP64 ? upload_vec<double>(c, dst, src, cnt) : upload_vec<float>(c, dst, src, cnt);
+}
+
+// Storage pitch: n+2 rounded up to 64 elements.
+int pitch_of(int w) { return (w + 63) / 64 * 64; }
+
+void release(qgw_ctx *c) {
+  if (!c) return;
+  void *ptrs[] = {c->q[0], c->q[1], c->psi, c->fa[0], c->fa[1], c->fa[2], c->comp, c->topo, c->wind,
+                  c->what, c->whatT, c->cl2m, c->cm2l, c->twx, c->twy, c->iRd2_d, c->La2, c->Kb2,
+                  c->diag_part, c->diag_out, c->stage, c->cfl_max};
+  for (void *p : ptrs)
+    if (p) hipFree(p);
+  for (auto &r : c->recs) {
+    hipEventDestroy(r.a);
+    hipEventDestroy(r.b);
+  }
+  for (auto e : c->evpool) hipEventDestroy(e);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+}
+
+// ---- hot-path sequences ---------------------------------------------------
+
+int run_invert(qgw_ctx *c) {
+  if (!c->modes_set) {
+    snprintf(c->err, sizeof c->err, "vertical modes not set (qgw_set_modes)");
+    return QGW_ERR_STATE;
+  }
+  qgw::LaunchArgs a;
+  a.g = c->g;
+  a.prec = c->prec;
+  a.Lx = c->Lx;
+  a.Ly = c->Ly;
+  a.stream = c->stream;
+  a.q = c->q[c->qcur];
+  a.psi = c->psi;
+  a.what = c->what;
+  a.whatT = c->whatT;
+  a.cl2m = c->cl2m;
+  a.cm2l = c->cm2l;
+  a.twx = c->twx;
+  a.twy = c->twy;
+  a.La2 = c->La2;
+  a.Kb2 = c->Kb2;
+  a.iRd2 = c->iRd2_d;
+  a.norm = 4.0 * (double)(c->g.n + 1) * (double)(c->g.m + 1);
+  a.ring_c = 2.0 * c->p.bc_fac / (c->grid.Delta * c->grid.Delta);
+  int st;
+  {
+    PassScope ps(c, QGW_PASS_P1);
+    st = qgw::launch_p1(a);
+  }
+  if (st || (st = check_launch(c, "p1"))) return st;
+  {
+    PassScope ps(c, QGW_PASS_T1);
+    st = qgw::launch_t1(a);
+  }
+  if (st || (st = check_launch(c, "t1"))) return st;
+  {
+    PassScope ps(c, QGW_PASS_P2);
+    st = qgw::launch_p2(a);
+  }
+  if (st || (st = check_launch(c, "p2"))) return st;
+  {
+    PassScope ps(c, QGW_PASS_T2);
+    st = qgw::launch_t2(a);
+  }
+  if (st || (st = check_launch(c, "t2"))) return st;
+  {
+    PassScope ps(c, QGW_PASS_P3);
+    st = qgw::launch_p3(a);
+  }
+  if (st || (st = check_launch(c, "p3"))) return st;
+  return QGW_OK;
+}
+
+int run_p4(qgw_ctx *c, const double *cab3, int update) {
+  qgw::LaunchArgs a;
+  a.g = c->g;
+  a.prec = c->prec;
+  a.stream = c->stream;
+  a.q = c->q[c->qcur];
+  a.qn = c->q[c->qcur ^ 1];
+  a.psi = c->psi;
+  a.fnew = c->fa[0];
+  a.f1 = c->fa[1];
+  a.f2 = c->fa[2];
+  a.comp = (c->prec == QGW_FP32) ? c->comp : nullptr;
+  a.topo = c->topo;
+  a.wind = c->wind;
+  const double D = c->grid.Delta;
+  qgw::StepCoef &cf = a.cf;
+  cf.c0 = cab3[0];
+  cf.c1 = cab3[1];
+  cf.c2 = cab3[2];
+  cf.beta = c->p.beta;
+  cf.nu = c->p.nu;
+  cf.nu_hyper = c->nu_hyper;
+  cf.inv_d2 = 1.0 / (D * D);
+  cf.inv_12d2 = 1.0 / (12. * D * D);
+  cf.inv_2d = 1.0 / (2 * D);
+  const double dhb = c->p.dh[c->p.nl - 1];
+  cf.drag = -c->p.hEkb * c->p.f0 / (2 * dhb);
+  cf.topo_fac = c->p.f0 / dhb;
+  cf.ring_c = 2.0 * c->p.bc_fac / (D * D);
+  cf.update = update;
+  a.nlap = (c->nu_hyper != 0.0) ? (c->p.n_hyper / 2 + 1) : 0;
+  int st;
+  {
+    PassScope ps(c, QGW_PASS_P4);
+    st = qgw::launch_p4(a);
+  }
+  if (st || (st = check_launch(c, "p4"))) return st;
+  if (update) c->qcur ^= 1;
+  return QGW_OK;
+}
+
+void ab3_coeffs(double dt, double dt0, double dt1, double *c) {
+  // variable-step AB3 (src/timestep.h:131-133, doc/timestep.org:95-97)
+  c[0] = dt * (2 * (dt * dt) + 3 * dt * (2 * dt0 + dt1) + 6 * dt0 * (dt0 + dt1)) / (6 * dt0 * (dt0 + dt1));
+  c[1] = (dt * dt) * (-dt / 3 - dt0 / 2 - dt1 / 2) / (dt0 * dt1);
+  c[2] = (dt * dt) * (dt / 3 + dt0 / 2) / (dt1 * (dt0 + dt1));
+}
+
+// adjust_timestep (src/timestep.h:69-111): CFL reduction on the device, the
+// doubling / cap / snap-to-output logic on the host.
+int adaptive_dt(qgw_ctx *c, double *dt_new) {
+  HIPCHK(c, hipMemsetAsync(c->cfl_max, 0, sizeof(unsigned long long), c->stream));
+  qgw::LaunchArgs a;
+  a.g = c->g;
+  a.prec = c->prec;
+  a.stream = c->stream;
+  a.psi = c->psi;
+  a.cfl_max = c->cfl_max;
+  int st;
+  {
+    PassScope ps(c, QGW_PASS_CFL);
+    st = qgw::launch_cfl(a);
+  }
+  if (st || (st = check_launch(c, "cfl"))) return st;
+  unsigned long long bits = 0;
+  HIPCHK(c, hipMemcpyAsync(&bits, c->cfl_max, sizeof bits, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  double dmax;
+  memcpy(&dmax, &bits, sizeof dmax);
+  const double Delta = c->grid.Delta, cfl = c->p.cfl;
+  // min over points of cfl*Delta/u equals cfl*Delta/max(u): division is monotone
+  double dt_max = kHuge;
+  const double umax = dmax / Delta;
+  if (umax != 0.) dt_max = cfl * Delta / umax;
+  if (!isfinite(dmax)) {
+    snprintf(c->err, sizeof c->err, "non-finite psi in the CFL reduction");
+    return QGW_ERR_NUMERIC;
+  }
+  double dt = c->dt;
+  if (dt < dt_max) dt = 2. * dt;
+  if (dt > c->DT_max) dt = c->DT_max;
+  if (dt > dt_max) dt = dt_max;
+  const double t = c->t, t_out = c->t_out;
+  if (t_out > t + kTeps) {
+    unsigned int n_step = (t_out - t) / dt;
+    if (n_step == 0)
+      dt = t_out - t;
+    else {
+      double dt1 = (t_out - t) / n_step;
+      if (dt1 > dt * (1. + kTeps))
+        dt = (t_out - t) / (n_step + 1);
+      else if (dt1 < dt)
+        dt = dt1;
+    }
+  }
+  *dt_new = dt;
+  return QGW_OK;
+}
+
+int one_step(qgw_ctx *c) {
+  // rotate the tendency history (src/timestep.h:117-120)
+  void *swap = c->fa[2];
+  c->fa[2] = c->fa[1];
+  c->fa[1] = c->fa[0];
+  c->fa[0] = swap;
+  int st = run_invert(c);
+  if (st) return st;
+  c->dt1 = c->dt0;
+  c->dt0 = c->dt;
+  if (c->p.dt_fixed > 0) {
+    c->dt = c->p.dt_fixed;
+  } else {
+    double nd;
+    st = adaptive_dt(c, &nd);
+    if (st) return st;
+    c->dt = nd;
+  }
+  double cab[3];
+  ab3_coeffs(c->dt, c->dt0, c->dt1, cab);
+  st = run_p4(c, cab, 1);
+  if (st) return st;
+  c->t = c->t + c->dt;
+  c->it++;
+  return QGW_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+qgw_ctx *qgw_create(const qgw_params *p, int precision, int *status) {
+  int dummy;
+  if (!status) status = &dummy;
+  *status = QGW_ERR_ARG;
+  if (!p) return nullptr;
+  if (precision != QGW_FP64 && precision != QGW_FP32 && precision != QGW_FP32_PLAIN) return nullptr;
+  if (p->nl < 1 || p->nl > qgw::kMaxLayers) return nullptr;
+  if (p->n_hyper < 0 || (p->n_hyper % 2) != 0 || p->n_hyper > 8) return nullptr;
+  if (p->k_f != 0 || p->dt_forc_period != 0) return nullptr;  // stochastic / file forcing not provided
+  qgw_grid grid;
+  if (qgwh_grid(p, &grid, nullptr, nullptr, nullptr, nullptr) != QGW_OK) return nullptr;
+  const int Lx = 2 * (grid.n + 1), Ly = 2 * (grid.m + 1);
+  if (!supported_length(Lx) || !supported_length(Ly)) return nullptr;
+
+  qgw_ctx *c = new qgw_ctx();
+  memset(c->err, 0, sizeof c->err);
+  c->p = *p;
+  c->prec = precision;
+  c->esz = (precision == QGW_FP64) ? 8 : 4;
+  c->grid = grid;
+  c->Lx = Lx;
+  c->Ly = Ly;
+  const int n = grid.n, m = grid.m, nl = p->nl;
+  qgw::Geo &g = c->g;
+  g.n = n;
+  g.m = m;
+  g.nl = nl;
+  g.P = pitch_of(n + 2);
+  g.LS = (long)(m + 2) * g.P;
+  g.Pn = pitch_of(n);
+  g.WS = (long)m * g.Pn;
+  g.Pm = pitch_of(m);
+  g.WTS = (long)n * g.Pm;
+  *status = QGW_ERR_HIP;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    release(c);
+    return nullptr;
+  }
+  const size_t field = c->esz * (size_t)g.LS * nl;
+  int st = QGW_OK;
+  st = st ? st : dalloc(c, &c->q[0], field);
+  st = st ? st : dalloc(c, &c->q[1], field);
+  st = st ? st : dalloc(c, &c->psi, field);
+  for (int i = 0; i < 3; i++) st = st ? st : dalloc(c, &c->fa[i], field);
+  if (precision == QGW_FP32) st = st ? st : dalloc(c, &c->comp, field);
+  st = st ? st : dalloc(c, &c->topo, c->esz * (size_t)g.LS);
+  st = st ? st : dalloc(c, &c->wind, c->esz * (size_t)(m + 2));
+  st = st ? st : dalloc(c, &c->what, c->esz * (size_t)g.WS * nl);
+  st = st ? st : dalloc(c, &c->whatT, c->esz * (size_t)g.WTS * nl);
+  st = st ? st : dalloc(c, &c->cl2m, c->esz * (size_t)nl * nl);
+  st = st ? st : dalloc(c, &c->cm2l, c->esz * (size_t)nl * nl);
+  st = st ? st : dalloc(c, (void **)&c->iRd2_d, sizeof(double) * nl);
+  st = st ? st : dalloc(c, (void **)&c->La2, sizeof(double) * (n + 2));
+  st = st ? st : dalloc(c, (void **)&c->Kb2, sizeof(double) * (m + 2));
+  c->nblk_diag = 256;
+  st = st ? st : dalloc(c, (void **)&c->diag_part, sizeof(double) * 2 * c->nblk_diag * nl);
+  st = st ? st : dalloc(c, (void **)&c->diag_out, sizeof(double) * 2 * nl);
+  st = st ? st : dalloc(c, (void **)&c->cfl_max, sizeof(unsigned long long));
+  if (!st) {
+    if (precision == QGW_FP64) {
+      st = make_twiddles<double>(c, Lx, &c->twx);
+      if (!st) st = make_twiddles<double>(c, Ly, &c->twy);
+    } else {
+      st = make_twiddles<float>(c, Lx, &c->twx);
+      if (!st) st = make_twiddles<float>(c, Ly, &c->twy);
+    }
+  }
+  if (st) {
+    *status = st;
+    release(c);
+    return nullptr;
+  }
+  // wavenumbers squared (src/domain.h:169-174; K uses Lx as the reference does)
+  {
+    std::vector<double> Lw(n + 2), Kw(m + 2), La2(n + 2, 0.0), Kb2(m + 2, 0.0);
+    qgwh_grid(p, &grid, nullptr, nullptr, Kw.data(), Lw.data());
+    for (int i = 0; i < n + 2; i++) La2[i] = Lw[i] * Lw[i];
+    for (int j = 0; j < m + 2; j++) Kb2[j] = Kw[j] * Kw[j];
+    if (hipMemcpy(c->La2, La2.data(), sizeof(double) * (n + 2), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->Kb2, Kb2.data(), sizeof(double) * (m + 2), hipMemcpyHostToDevice) != hipSuccess) {
+      release(c);
+      return nullptr;
+    }
+    std::vector<double> wind(m + 2);
+    qgwh_wind_row(p, wind.data());
+    if (upload_typed(c, c->wind, wind.data(), m + 2)) {
+      release(c);
+      return nullptr;
+    }
+  }
+  // topography (src/domain.h:200-207)
+  {
+    std::vector<double> topo((size_t)(n + 2) * (m + 2));
+    qgwh_topo(p, topo.data());
+    if (qgw_upload(c, QGW_FIELD_TOPO, topo.data()) != QGW_OK) {
+      release(c);
+      return nullptr;
+    }
+  }
+  // time state (init_vars sign rule, check_timestep, init_timestep)
+  c->nu_hyper = qgwh_nu_hyper_signed(p);
+  qgwh_check_timestep(p, c->nu_hyper, &c->dt, &c->DT_max);
+  if (p->dt_fixed > 0) c->dt = p->dt_fixed;
+  c->dt0 = c->dt;
+  c->dt1 = c->dt;
+  c->t = 0;
+  c->t_out = 0;
+  c->it = 0;
+  c->qcur = 0;
+  c->modes_set = false;
+  // the zero fills above ran on the null stream; the step runs on c->stream
+  if (hipDeviceSynchronize() != hipSuccess) {
+    release(c);
+    return nullptr;
+  }
+  *status = QGW_OK;
+  return c;
+}
+
+int qgw_destroy(qgw_ctx *c) {
+  if (!c) return QGW_ERR_ARG;
+  if (c->stream) hipStreamSynchronize(c->stream);
+  release(c);
+  return QGW_OK;
+}
+
+int qgw_set_modes(qgw_ctx *c, const double *cl2m, const double *cm2l, const double *iRd2) {
+  if (!c || !cl2m || !cm2l || !iRd2) return QGW_ERR_ARG;
+  const int nl = c->p.nl;
+  for (int i = 0; i < nl * nl; i++)
+    if (!isfinite(cl2m[i]) || !isfinite(cm2l[i])) return QGW_ERR_ARG;
+  for (int i = 0; i < nl; i++)
+    if (!isfinite(iRd2[i])) return QGW_ERR_ARG;
+  int st = upload_typed(c, c->cl2m, cl2m, (size_t)nl * nl);
+  if (st) return st;
+  st = upload_typed(c, c->cm2l, cm2l, (size_t)nl * nl);
+  if (st) return st;
+  HIPCHK(c, hipMemcpy(c->iRd2_d, iRd2, sizeof(double) * nl, hipMemcpyHostToDevice));
+  c->modes_set = true;
+  return QGW_OK;
+}
+
+static void *field_ptr(qgw_ctx *c, int field, int *layers, void **resid) {
+  *resid = nullptr;
+  *layers = c->p.nl;
+  switch (field) {
+    case QGW_FIELD_Q:
+      if (c->prec == QGW_FP32) *resid = c->comp;
+      return c->q[c->qcur];
+    case QGW_FIELD_PSI: return c->psi;
+    case QGW_FIELD_TOPO: *layers = 1; return c->topo;
+    case QGW_FIELD_F0: return c->fa[0];
+    case QGW_FIELD_F1: return c->fa[1];
+    case QGW_FIELD_F2: return c->fa[2];
+  }
+  return nullptr;
+}
+
+static int ensure_stage(qgw_ctx *c) {
+  if (c->stage) return QGW_OK;
+  size_t b = sizeof(double) * (size_t)(c->g.n + 2) * (c->g.m + 2) * c->p.nl;
+  hipError_t e = hipMalloc((void **)&c->stage, b);
+  if (e != hipSuccess) return fail_hip(c, e, "hipMalloc(stage)");
+  c->bytes += b;
+  return QGW_OK;
+}
+
+int qgw_upload(qgw_ctx *c, int field, const double *host) {
+  if (!c || !host) return QGW_ERR_ARG;
+  int layers;
+  void *resid;
+  void *dst = field_ptr(c, field, &layers, &resid);
+  if (!dst) return QGW_ERR_ARG;
+  int st = ensure_stage(c);
+  if (st) return st;
+  size_t cnt = (size_t)(c->g.n + 2) * (c->g.m + 2) * layers;
+  HIPCHK(c, hipMemcpyAsync(c->stage, host, cnt * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  qgw::LaunchArgs a;
+  a.g = c->g;
+  a.prec = c->prec;
+  a.stream = c->stream;
+  st = qgw::launch_from_f64(a, layers, c->stage, dst, resid);
+  if (st || (st = check_launch(c, "from_f64"))) return st;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return QGW_OK;
+}
+
+int qgw_download(qgw_ctx *c, int field, double *host) {
+  if (!c || !host) return QGW_ERR_ARG;
+  int layers;
+  void *resid;
+  void *src = field_ptr(c, field, &layers, &resid);
+  if (!src) return QGW_ERR_ARG;
+  int st = ensure_stage(c);
+  if (st) return st;
+  qgw::LaunchArgs a;
+  a.g = c->g;
+  a.prec = c->prec;
+  a.stream = c->stream;
+  st = qgw::launch_to_f64(a, layers, src, resid, c->stage);
+  if (st || (st = check_launch(c, "to_f64"))) return st;
+  size_t cnt = (size_t)(c->g.n + 2) * (c->g.m + 2) * layers;
+  HIPCHK(c, hipMemcpyAsync(host, c->stage, cnt * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return QGW_OK;
+}
+
+int qgw_invert_pv(qgw_ctx *c) {
+  if (!c) return QGW_ERR_ARG;
+  return run_invert(c);
+}
+
+int qgw_rhs(qgw_ctx *c) {
+  if (!c) return QGW_ERR_ARG;
+  int st = run_invert(c);
+  if (st) return st;
+  const double z[3] = {0, 0, 0};
+  return run_p4(c, z, 0);
+}
+
+int qgw_adjust_timestep(qgw_ctx *c, double *dt_out) {
+  if (!c) return QGW_ERR_ARG;
+  if (c->p.dt_fixed > 0) {
+    if (dt_out) *dt_out = c->p.dt_fixed;
+    return QGW_OK;
+  }
+  double nd;
+  int st = adaptive_dt(c, &nd);
+  if (st) return st;
+  c->dt = nd;
+  if (dt_out) *dt_out = nd;
+  return QGW_OK;
+}
+
+int qgw_timestep(qgw_ctx *c) {
+  if (!c) return QGW_ERR_ARG;
+  return one_step(c);
+}
+
+int qgw_run(qgw_ctx *c, int nsteps) {
+  if (!c || nsteps < 0) return QGW_ERR_ARG;
+  for (int i = 0; i < nsteps; i++) {
+    int st = one_step(c);
+    if (st) return st;
+  }
+  return QGW_OK;
+}
+
+int qgw_diagnostics(qgw_ctx *c, double *E_layer, double *Z_layer, double *E_tot, double *Z_tot) {
+  if (!c) return QGW_ERR_ARG;
+  const int nl = c->p.nl;
+  qgw::LaunchArgs a;
+  a.g = c->g;
+  a.prec = c->prec;
+  a.stream = c->stream;
+  a.psi = c->psi;
+  a.q = c->q[c->qcur];
+  a.comp = (c->prec == QGW_FP32) ? c->comp : nullptr;
+  int st;
+  {
+    PassScope ps(c, QGW_PASS_DIAG);
+    st = qgw::launch_diag(a, c->diag_part, c->nblk_diag, c->diag_out);
+  }
+  if (st || (st = check_launch(c, "diag"))) return st;
+  std::vector<double> h(2 * nl);
+  HIPCHK(c, hipMemcpyAsync(h.data(), c->diag_out, sizeof(double) * 2 * nl, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const double w = c->grid.Delta * c->grid.Delta / (c->p.Lx * c->p.Lx);
+  double H = 0, Et = 0, Zt = 0;
+  for (int l = 0; l < nl; l++) H += c->p.dh[l];
+  for (int l = 0; l < nl; l++) {
+    const double e = -h[2 * l] * w, z = h[2 * l + 1] * w;
+    if (E_layer) E_layer[l] = e;
+    if (Z_layer) Z_layer[l] = z;
+    Et += c->p.dh[l] * e;
+    Zt += c->p.dh[l] * z;
+  }
+  if (E_tot) *E_tot = Et / (2 * H);
+  if (Z_tot) *Z_tot = Zt / H;
+  return QGW_OK;
+}
+
+int qgw_get_time(qgw_ctx *c, double *t, double *dt, double *dt0, double *dt1, int *it) {
+  if (!c) return QGW_ERR_ARG;
+  if (t) *t = c->t;
+  if (dt) *dt = c->dt;
+  if (dt0) *dt0 = c->dt0;
+  if (dt1) *dt1 = c->dt1;
+  if (it) *it = c->it;
+  return QGW_OK;
+}
+
+int qgw_set_time(qgw_ctx *c, double t, double dt, double dt0, double dt1, int it) {
+  if (!c) return QGW_ERR_ARG;
+  c->t = t;
+  c->dt = dt;
+  c->dt0 = dt0;
+  c->dt1 = dt1;
+  c->it = it;
+  return QGW_OK;
+}
+
+int qgw_set_next_output(qgw_ctx *c, double t_out) {
+  if (!c) return QGW_ERR_ARG;
+  c->t_out = t_out;
+  return QGW_OK;
+}
+
+int qgw_synchronize(qgw_ctx *c) {
+  if (!c) return QGW_ERR_ARG;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return QGW_OK;
+}
+
+int qgw_time_run(qgw_ctx *c, int nsteps, double *ms) {
+  if (!c || !ms || nsteps < 0) return QGW_ERR_ARG;
+  hipEvent_t a, b;
+  HIPCHK(c, hipEventCreate(&a));
+  HIPCHK(c, hipEventCreate(&b));
+  HIPCHK(c, hipEventRecord(a, c->stream));
+  int st = qgw_run(c, nsteps);
+  if (!st) {
+    HIPCHK(c, hipEventRecord(b, c->stream));
+    HIPCHK(c, hipEventSynchronize(b));
+    float f = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&f, a, b));
+    *ms = f;
+  }
+  hipEventDestroy(a);
+  hipEventDestroy(b);
+  return st;
+}
+
+int qgw_profile_enable(qgw_ctx *c, int on) {
+  if (!c) return QGW_ERR_ARG;
+  if (!on) {
+    int st = prof_flush(c);
+    if (st) return st;
+  }
+  c->prof = on != 0;
+  return QGW_OK;
+}
+
+int qgw_profile_read(qgw_ctx *c, double *ms, int *count, int npass) {
+  if (!c) return QGW_ERR_ARG;
+  int st = prof_flush(c);
+  if (st) return st;
+  for (int i = 0; i < npass && i < QGW_NPASS; i++) {
+    if (ms) ms[i] = c->prof_ms[i];
+    if (count) count[i] = c->prof_cnt[i];
+    c->prof_ms[i] = 0;
+    c->prof_cnt[i] = 0;
+  }
+  return QGW_OK;
+}
+
+int qgw_device_bytes(qgw_ctx *c, size_t *bytes) {
+  if (!c || !bytes) return QGW_ERR_ARG;
+  *bytes = c->bytes;
+  return QGW_OK;
+}
+
+const char *qgw_strerror(int status) { return qgwh_strerror(status); }
+
+const char *qgw_last_error(qgw_ctx *c) { return c ? c->err : ""; }
+
+}  // extern "C"

@@ -1,0 +1,753 @@
+/*
+ * qgw_host.c -- host-side set-up of a qgw run, in C (FP64, once per run).
+ *
+ *   parameter file reader ........ behaviour of src/extra.h:98-153
+ *   grid / wavenumbers / topo .... src/domain.h:64-176, 200-207
+ *   stretching matrix and modes .. src/eigmode.h:87-271 (own symmetric QL solver)
+ *   wind row ..................... src/domain.h:33
+ *   initial-dt quirks ............ src/timestep.h:18-54 with src/qg.c:40-41
+ *   seeded initial condition ..... no reference counterpart (the reference reads restart.nc)
+ *   diagnostics .................. test/test_energy_injection.py:24-25
+ */
+#include <ctype.h>
+#include <math.h>
+#include <stddef.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <complex.h>
+
+#include "qgw_host.h"
+
+/* The reference's pi literal (src/qg.c:44); equal to M_PI in binary64. */
+static const double QGW_PI = 3.141592653589793;
+
+/* ------------------------------------------------------------------------ */
+/* defaults                                                                   */
+/* ------------------------------------------------------------------------ */
+
+void qgwh_default_params(qgw_params *p) {
+  memset(p, 0, sizeof(*p));
+  /* src/qg.c:58-83 */
+  p->nl = 1;
+  for (int i = 0; i < QGW_NL_MAX; i++) { p->dh[i] = 0.0; p->N2[i] = 0.0; }
+  p->dh[0] = 1.0;
+  p->N2[0] = 1.0;
+  p->tend = 1.0;
+  p->dt_out = 0.1;
+  p->cfl = 0.2;
+  p->beta = 0.0;
+  p->nu = 0.0;
+  p->n_hyper = 0;
+  p->nu_hyper = 0.0;
+  p->hEkb = 0.0;
+  p->tau0 = 0.0;
+  p->forc_mode = 1.0;
+  p->f0 = 1.e-5;
+  p->bc_fac = 0.0;
+  p->Ld = 0.0;
+  p->h_topo = 0.0;
+  p->w_topo = 1.0;
+  /* src/forcing.h:13-14, 40-41 */
+  p->sigma_f = 1.0;
+  p->k_f = 0.0;
+  p->dt_forc = 0.0;
+  p->dt_forc_period = 0.0;
+  /* extensions */
+  p->dt_fixed = 0.0;
+  p->nsteps = 0;
+  p->ic_kind = 0;
+  p->seed = 20261203ULL;
+  p->ic_urms = 0.1;
+  p->ic_lambda0 = 200.0e3;
+  p->precision = 0;
+  p->verbose = 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* parameter file reader                                                      */
+/* ------------------------------------------------------------------------ */
+
+enum { PK_INT, PK_DOUBLE, PK_ARRAY, PK_U64 };
+
+typedef struct {
+  const char *name;
+  int kind;
+  size_t offset;
+} param_key;
+
+#define PKEY(nm, kd) {#nm, kd, offsetof(qgw_params, nm)}
+
+/* Registered keys: the reference's 25 (src/qg.c:113-137) and the extensions. */
+static const param_key k_keys[] = {
+    PKEY(NX, PK_INT),          PKEY(NY, PK_INT),         PKEY(nl, PK_INT),
+    PKEY(n_hyper, PK_INT),     PKEY(Lx, PK_DOUBLE),      PKEY(dh, PK_ARRAY),
+    PKEY(tau0, PK_DOUBLE),     PKEY(forc_mode, PK_DOUBLE), PKEY(f0, PK_DOUBLE),
+    PKEY(beta, PK_DOUBLE),     PKEY(nu, PK_DOUBLE),      PKEY(nu_hyper, PK_DOUBLE),
+    PKEY(hEkb, PK_DOUBLE),     PKEY(N2, PK_ARRAY),       PKEY(Ld, PK_DOUBLE),
+    PKEY(h_topo, PK_DOUBLE),   PKEY(w_topo, PK_DOUBLE),  PKEY(bc_fac, PK_DOUBLE),
+    PKEY(tend, PK_DOUBLE),     PKEY(dt_out, PK_DOUBLE),  PKEY(dt_forc, PK_DOUBLE),
+    PKEY(dt_forc_period, PK_DOUBLE), PKEY(sigma_f, PK_DOUBLE), PKEY(k_f, PK_DOUBLE),
+    PKEY(cfl, PK_DOUBLE),
+    /* extensions */
+    PKEY(dt_fixed, PK_DOUBLE), PKEY(nsteps, PK_INT),     PKEY(ic_kind, PK_INT),
+    PKEY(seed, PK_U64),        PKEY(ic_urms, PK_DOUBLE), PKEY(ic_lambda0, PK_DOUBLE),
+    PKEY(precision, PK_INT),   PKEY(verbose, PK_INT),
+};
+
+/* Array values: the text after '=' with the newline cut; the first field is
+ * delimited by any of "[,]" and the following ones by "," only, and each
+ * field is converted with atof (so "2900.]" reads as 2900).  Same field rule
+ * as src/extra.h:98-115. */
+static void parse_array(char *text, double *arr) {
+  char *nl = strchr(text, '\n');
+  if (nl) *nl = '\0';
+  int n = 0;
+  char *save = NULL;
+  char *tok = strtok_r(text, "[,]", &save);
+  while (tok != NULL && n < QGW_NL_MAX) {
+    arr[n++] = atof(tok);
+    tok = strtok_r(NULL, ",", &save);
+  }
+}
+
+int qgwh_read_params(const char *path, qgw_params *p) {
+  FILE *fp = fopen(path, "rt");
+  if (!fp) return QGW_ERR_IO;
+  char line[300];
+  while (fgets(line, sizeof line, fp)) {
+    /* drop every space (not tabs), as src/extra.h:26-33 */
+    char *w = line;
+    for (char *r = line; *r; r++)
+      if (*r != ' ') *w++ = *r;
+    *w = '\0';
+    char *save = NULL;
+    char *key = strtok_r(line, "=", &save);
+    char *val = strtok_r(NULL, "=", &save);
+    if (!key) continue;
+    for (size_t i = 0; i < sizeof k_keys / sizeof k_keys[0]; i++) {
+      if (strcmp(k_keys[i].name, key) != 0) continue;
+      char *dst = (char *)p + k_keys[i].offset;
+      const char *v = val ? val : "";
+      switch (k_keys[i].kind) {
+        case PK_INT: *(int *)dst = atoi(v); break;
+        case PK_DOUBLE: *(double *)dst = atof(v); break;
+        case PK_U64: *(uint64_t *)dst = strtoull(v, NULL, 10); break;
+        case PK_ARRAY:
+          if (val) parse_array(val, (double *)dst);
+          break;
+      }
+      if (p->verbose) fprintf(stdout, "scan param %s\n", k_keys[i].name);
+    }
+  }
+  fclose(fp);
+  return QGW_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* grid                                                                       */
+/* ------------------------------------------------------------------------ */
+
+int qgwh_grid(const qgw_params *p, qgw_grid *g, double *X, double *Y, double *Kw, double *Lw) {
+  if (!p || !g) return QGW_ERR_ARG;
+  if (p->bc_fac < 0.0) return QGW_ERR_ARG; /* periodic branch not provided */
+  if (p->NX < 3 || p->NY < 3 || !(p->Lx > 0.0)) return QGW_ERR_ARG;
+  g->Delta = p->Lx / p->NX;
+  g->Lx = p->Lx;
+  g->Ly = p->NY * g->Delta;
+  g->n = p->NX - 1;
+  g->m = p->NY - 1;
+  g->nxp2 = g->n + 2;
+  g->nyp2 = g->m + 2;
+  if (X)
+    for (int i = 0; i < g->nxp2; i++) X[i] = i * g->Delta;
+  if (Y)
+    for (int j = 0; j < g->nyp2; j++) Y[j] = j * g->Delta;
+  if (Kw) {
+    for (int j = 0; j < g->nyp2; j++) Kw[j] = 0.0;
+    for (int j = 1; j < g->m + 1; j++) Kw[j] = QGW_PI * j / p->Lx;
+  }
+  if (Lw) {
+    for (int i = 0; i < g->nxp2; i++) Lw[i] = 0.0;
+    for (int i = 1; i < g->n + 1; i++) Lw[i] = QGW_PI * i / p->Lx;
+  }
+  return QGW_OK;
+}
+
+static double shelf(double x, double d) { return 1.0 - exp(-(x * x) / (2.0 * (d * d))); }
+
+int qgwh_topo(const qgw_params *p, double *topo) {
+  qgw_grid g;
+  int st = qgwh_grid(p, &g, NULL, NULL, NULL, NULL);
+  if (st) return st;
+  for (int j = 0; j < g.nyp2; j++) {
+    double y = j * g.Delta;
+    for (int i = 0; i < g.nxp2; i++) {
+      double x = i * g.Delta;
+      double s = shelf(x, p->w_topo) * shelf(p->Lx - x, p->w_topo) * shelf(y, p->w_topo) *
+                 shelf(g.Ly - y, p->w_topo);
+      topo[(size_t)j * g.nxp2 + i] = p->h_topo * (1.0 - s);
+    }
+  }
+  return QGW_OK;
+}
+
+int qgwh_wind_row(const qgw_params *p, double *forc_row) {
+  qgw_grid g;
+  int st = qgwh_grid(p, &g, NULL, NULL, NULL, NULL);
+  if (st) return st;
+  for (int j = 0; j < g.nyp2; j++) {
+    double y = j * g.Delta;
+    forc_row[j] = -p->tau0 / p->dh[0] * p->forc_mode * QGW_PI / g.Ly *
+                  sin(p->forc_mode * QGW_PI * y / g.Ly);
+  }
+  return QGW_OK;
+}
+
+double qgwh_nu_hyper_signed(const qgw_params *p) {
+  return pow(-1.0, p->n_hyper / 2 - 1) * fabs(p->nu_hyper);
+}
+
+void qgwh_check_timestep(const qgw_params *p, double nu_hyper_signed, double *dt_out,
+                         double *DT_max_out) {
+  qgw_grid g;
+  if (qgwh_grid(p, &g, NULL, NULL, NULL, NULL)) {
+    *dt_out = 0.0;
+    *DT_max_out = 0.0;
+    return;
+  }
+  /* Lx/Nx uses the interior count (Nx = NX-1 after init_domain). */
+  double h = p->Lx / g.n;
+  double a = nu_hyper_signed * 20. / pow(h, p->n_hyper);
+  double b = p->nu * 20. / pow(h, 2);
+  double c = 2. * p->beta * p->Lx;
+  /* the unparenthesised max(max(a,b),c) of src/timestep.h:27 evaluates as */
+  double mx = (a < b) ? b : ((a < c) ? c : a);
+  double DT_max = pow(mx, -1.);
+  double dt = 0.0;
+  double Delta = g.Delta, Ly = g.Ly;
+  if (p->tau0 != 0 && p->sigma_f != 0) {
+    double s1 = sqrt(p->cfl * Delta / (p->tau0 / p->dh[0] * p->forc_mode * QGW_PI / Ly * Ly));
+    double s2 = pow((Delta * p->cfl) * (Delta * p->cfl) / p->sigma_f, 1. / 3.);
+    /* 1e-3*min(s1,s2) expands to (1e-3*s1 > s2) ? s2 : s1 */
+    dt = (1e-3 * s1 > s2) ? s2 : s1;
+  } else if (p->tau0 != 0 && p->sigma_f == 0) {
+    dt = 1e-3 * sqrt(p->cfl * Delta / (p->tau0 / p->dh[0] * p->forc_mode * QGW_PI / Ly * Ly));
+  } else if (p->tau0 == 0 && p->sigma_f != 0) {
+    dt = 1e-3 * pow((Delta * p->cfl) * (Delta * p->cfl) / p->sigma_f, 1. / 3.);
+  }
+  if (DT_max != 0 && dt > DT_max) dt = DT_max;
+  *dt_out = dt;
+  *DT_max_out = DT_max;
+}
+
+/* ------------------------------------------------------------------------ */
+/* vertical modes                                                             */
+/* ------------------------------------------------------------------------ */
+
+int qgwh_gamma(const qgw_params *p, double *G) {
+  int nl = p->nl;
+  if (nl < 1 || nl > QGW_NL_MAX) return QGW_ERR_ARG;
+  double dhc[QGW_NL_MAX];
+  for (int l = 0; l < nl - 1; l++) dhc[l] = 0.5 * (p->dh[l] + p->dh[l + 1]);
+  for (int i = 0; i < nl * nl; i++) G[i] = 0.0;
+  if (nl == 1) {
+    G[0] = 1.0;
+    return QGW_OK;
+  }
+  double f02 = p->f0 * p->f0;
+  for (int l = 0; l < nl; l++) {
+    double lo = 0.0, up = 0.0;
+    if (l > 0) {
+      lo = -f02 / p->N2[l - 1] / (dhc[l - 1] * p->dh[l]);
+      G[nl * l + l - 1] = lo;
+    }
+    if (l < nl - 1) {
+      up = -f02 / p->N2[l] / (dhc[l] * p->dh[l]);
+      G[nl * l + l + 1] = up;
+    }
+    G[nl * l + l] = -lo - up;
+  }
+  /* the first and last rows carry a single off-diagonal term */
+  G[0] = -G[1];
+  G[nl * (nl - 1) + nl - 1] = -G[nl * (nl - 1) + nl - 2];
+  return QGW_OK;
+}
+
+/* Symmetric tridiagonal eigen-solver: implicit QL with Wilkinson shifts
+ * (Golub & Van Loan, Matrix Computations, sec. 8.3; the classic tql2 scheme).
+ * d[0..n-1] diagonal, e[0..n-2] off-diagonal; on return d holds the
+ * eigenvalues and column k of Z (row-major n*n, Z[i*n+k]) the eigenvector. */
+static int tridiag_ql(int n, double *d, double *e_in, double *Z) {
+  double *e = (double *)calloc((size_t)n + 1, sizeof(double));
+  if (!e) return QGW_ERR_NOMEM;
+  for (int i = 0; i < n - 1; i++) e[i] = e_in[i];
+  e[n - 1] = 0.0;
+  for (int i = 0; i < n * n; i++) Z[i] = 0.0;
+  for (int i = 0; i < n; i++) Z[i * n + i] = 1.0;
+  for (int l = 0; l < n; l++) {
+    int iter = 0;
+    for (;;) {
+      int m;
+      for (m = l; m < n - 1; m++) {
+        double dd = fabs(d[m]) + fabs(d[m + 1]);
+        if (fabs(e[m]) <= 1e-300 + 2.2204460492503131e-16 * dd) break;
+      }
+      if (m == l) break;
+      if (++iter > 200) {
+        free(e);
+        return QGW_ERR_NUMERIC;
+      }
+      double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+      double r = hypot(g, 1.0);
+      g = d[m] - d[l] + e[l] / (g + (g >= 0 ? fabs(r) : -fabs(r)));
+      double s = 1.0, c = 1.0, pp = 0.0;
+      int i;
+      for (i = m - 1; i >= l; i--) {
+        double f = s * e[i];
+        double b = c * e[i];
+        r = hypot(f, g);
+        e[i + 1] = r;
+        if (r == 0.0) {
+          d[i + 1] -= pp;
+          e[m] = 0.0;
+          break;
+        }
+        s = f / r;
+        c = g / r;
+        g = d[i + 1] - pp;
+        r = (d[i] - g) * s + 2.0 * c * b;
+        pp = s * r;
+        d[i + 1] = g + pp;
+        g = c * r - b;
+        for (int k = 0; k < n; k++) {
+          double zf = Z[k * n + i + 1];
+          Z[k * n + i + 1] = s * Z[k * n + i] + c * zf;
+          Z[k * n + i] = c * Z[k * n + i] - s * zf;
+        }
+      }
+      if (r == 0.0 && i >= l) continue;
+      d[l] -= pp;
+      e[l] = g;
+      e[m] = 0.0;
+    }
+  }
+  free(e);
+  return QGW_OK;
+}
+
+int qgwh_eigmode(const qgw_params *p, double *cl2m, double *cm2l, double *iRd2) {
+  int nl = p->nl;
+  if (nl < 1 || nl > QGW_NL_MAX) return QGW_ERR_ARG;
+  double H = 0.0;
+  for (int k = 0; k < nl; k++) H += p->dh[k];
+  if (nl == 1) {
+    /* Gamma = [1]: both vectors are 1 after the normalisations */
+    cl2m[0] = 1.0;
+    cm2l[0] = 1.0;
+    iRd2[0] = (p->Ld != 0) ? 1.0 / (p->Ld * p->Ld) : 0.0;
+    return QGW_OK;
+  }
+  for (int l = 0; l < nl; l++) {
+    if (!(p->dh[l] > 0.0)) return QGW_ERR_ARG;
+    if (l < nl - 1 && !(p->N2[l] != 0.0)) return QGW_ERR_ARG;
+  }
+  size_t nn = (size_t)nl * nl;
+  double *G = (double *)malloc(nn * sizeof(double));
+  double *Z = (double *)malloc(nn * sizeof(double));
+  double *vr = (double *)malloc(nn * sizeof(double));
+  double *vl = (double *)malloc(nn * sizeof(double));
+  double *d = (double *)malloc(nl * sizeof(double));
+  double *e = (double *)malloc(nl * sizeof(double));
+  int *ord = (int *)malloc(nl * sizeof(int));
+  int st = QGW_OK;
+  if (!G || !Z || !vr || !vl || !d || !e || !ord) {
+    st = QGW_ERR_NOMEM;
+    goto done;
+  }
+  qgwh_gamma(p, G);
+  /* D^(1/2) Gamma D^(-1/2) is symmetric (dh_l G_{l,l+1} = dh_{l+1} G_{l+1,l});
+   * its off-diagonal is -sqrt(G_{l,l+1} G_{l+1,l}) (both factors are negative
+   * for positive N2) */
+  for (int l = 0; l < nl; l++) d[l] = G[nl * l + l];
+  for (int l = 0; l < nl - 1; l++) {
+    double prod = G[nl * l + l + 1] * G[nl * (l + 1) + l];
+    double s = (G[nl * l + l + 1] < 0) ? -1.0 : 1.0;
+    e[l] = s * sqrt(fabs(prod));
+  }
+  st = tridiag_ql(nl, d, e, Z);
+  if (st) goto done;
+  /* ascending order (src/eigmode.h:154-185); a stable insertion sort */
+  for (int i = 0; i < nl; i++) ord[i] = i;
+  for (int i = 1; i < nl; i++) {
+    int v = ord[i], j = i - 1;
+    while (j >= 0 && d[ord[j]] > d[v]) {
+      ord[j + 1] = ord[j];
+      j--;
+    }
+    ord[j + 1] = v;
+  }
+  /* right vectors D^(-1/2) u, left vectors D^(1/2) u, columns sorted */
+  for (int m = 0; m < nl; m++) {
+    int src = ord[m];
+    for (int k = 0; k < nl; k++) {
+      double sq = sqrt(p->dh[k]);
+      vr[k * nl + m] = Z[k * nl + src] / sq;
+      vl[k * nl + m] = Z[k * nl + src] * sq;
+    }
+  }
+  /* Flierl (1978) normalisation, surface-positive (src/eigmode.h:192-214) */
+  for (int m = 0; m < nl; m++) {
+    double dot = 0.0;
+    for (int k = 0; k < nl; k++) dot += p->dh[k] * vr[k * nl + m] * vr[k * nl + m];
+    double fl = ((vr[m] > 0) ? 1.0 : -1.0) * sqrt(H / dot);
+    for (int k = 0; k < nl; k++) vr[k * nl + m] *= fl;
+  }
+  /* left vectors scaled to vl^T vr = 1 (src/eigmode.h:215-223) */
+  for (int m = 0; m < nl; m++) {
+    double dot = 0.0;
+    for (int k = 0; k < nl; k++) dot += vr[k * nl + m] * vl[k * nl + m];
+    for (int k = 0; k < nl; k++) vl[k * nl + m] /= dot;
+  }
+  for (int m = 0; m < nl; m++)
+    for (int k = 0; k < nl; k++) {
+      cl2m[k * nl + m] = vl[m * nl + k];
+      cm2l[k * nl + m] = vr[k * nl + m];
+    }
+  for (int k = 1; k < nl; k++) iRd2[k] = d[ord[k]];
+  iRd2[0] = 0.0;
+  if (p->verbose) {
+    fprintf(stdout, "Deformation radii:\n");
+    for (int k = 0; k < nl; k++)
+      fprintf(stdout, "iRd2: %g , def radius: %g\n", iRd2[k], iRd2[k] > 0 ? sqrt(1 / iRd2[k]) : 0);
+  }
+done:
+  free(G);
+  free(Z);
+  free(vr);
+  free(vl);
+  free(d);
+  free(e);
+  free(ord);
+  return st;
+}
+
+/* ------------------------------------------------------------------------ */
+/* host FFT and 2-D sine transform (initial condition only)                  */
+/* ------------------------------------------------------------------------ */
+
+typedef double _Complex cplx;
+
+typedef struct {
+  int L;
+  int nr;
+  int radix[32];
+  cplx *tw; /* tw[m] = exp(-2 pi i m / L) */
+} hplan;
+
+static int hplan_make(hplan *pl, int L) {
+  pl->L = L;
+  pl->nr = 0;
+  int r = L;
+  while (r % 4 == 0) { pl->radix[pl->nr++] = 4; r /= 4; }
+  while (r % 2 == 0) { pl->radix[pl->nr++] = 2; r /= 2; }
+  while (r % 3 == 0) { pl->radix[pl->nr++] = 3; r /= 3; }
+  while (r % 5 == 0) { pl->radix[pl->nr++] = 5; r /= 5; }
+  if (r != 1) return QGW_ERR_ARG;
+  pl->tw = (cplx *)malloc(sizeof(cplx) * L);
+  if (!pl->tw) return QGW_ERR_NOMEM;
+  for (int m = 0; m < L; m++) {
+    double a = -2.0 * QGW_PI * (double)m / (double)L;
+    pl->tw[m] = cos(a) + I * sin(a);
+  }
+  return QGW_OK;
+}
+
+/* Stockham autosort FFT (forward, e^{-2 pi i jk/L}); result is returned in
+ * whichever of x/y holds it. */
+static cplx *hfft(const hplan *pl, cplx *x, cplx *y) {
+  int L = pl->L, Ns = 1;
+  cplx *src = x, *dst = y;
+  for (int s = 0; s < pl->nr; s++) {
+    int R = pl->radix[s];
+    int stride = L / R;
+    int twstep = L / (Ns * R);
+    for (int j = 0; j < stride; j++) {
+      int k = j % Ns;
+      cplx v[5], o[5];
+      for (int r = 0; r < R; r++) v[r] = src[j + r * stride] * pl->tw[(r * k * twstep) % L];
+      if (R == 4) {
+        cplx a0 = v[0] + v[2], a1 = v[0] - v[2], a2 = v[1] + v[3], a3 = (v[1] - v[3]) * (-I);
+        o[0] = a0 + a2;
+        o[2] = a0 - a2;
+        o[1] = a1 + a3;
+        o[3] = a1 - a3;
+      } else if (R == 2) {
+        o[0] = v[0] + v[1];
+        o[1] = v[0] - v[1];
+      } else {
+        for (int q = 0; q < R; q++) {
+          cplx acc = 0;
+          for (int r = 0; r < R; r++) acc += v[r] * pl->tw[((r * q) % R) * (L / R)];
+          o[q] = acc;
+        }
+      }
+      int base = (j / Ns) * Ns * R + k;
+      for (int r = 0; r < R; r++) dst[base + r * Ns] = o[r];
+    }
+    Ns *= R;
+    cplx *t = src;
+    src = dst;
+    dst = t;
+  }
+  return src;
+}
+
+/* Unnormalised DST-I of two real sequences a, b of length n (strided) at once:
+ * the complex sequence a + i b, odd-extended to length 2(n+1), is transformed
+ * and DST(a) = -Im, DST(b) = Re of the result. */
+static void dst_pair(const hplan *pl, cplx *w1, cplx *w2, double *a, double *b, int n,
+                     ptrdiff_t stride) {
+  int L = pl->L;
+  w1[0] = 0;
+  w1[n + 1] = 0;
+  for (int j = 1; j <= n; j++) {
+    cplx z = a[(ptrdiff_t)(j - 1) * stride] + I * (b ? b[(ptrdiff_t)(j - 1) * stride] : 0.0);
+    w1[j] = z;
+    w1[L - j] = -z;
+  }
+  cplx *r = hfft(pl, w1, w2);
+  for (int k = 1; k <= n; k++) {
+    a[(ptrdiff_t)(k - 1) * stride] = -cimag(r[k]);
+    if (b) b[(ptrdiff_t)(k - 1) * stride] = creal(r[k]);
+  }
+}
+
+int qgwh_dst2d(double *a, int n, int m) {
+  if (n < 1 || m < 1) return QGW_ERR_ARG;
+  hplan px, py;
+  int st = hplan_make(&px, 2 * (n + 1));
+  if (st) return st;
+  st = hplan_make(&py, 2 * (m + 1));
+  if (st) {
+    free(px.tw);
+    return st;
+  }
+  int err = 0;
+#pragma omp parallel
+  {
+    cplx *w1 = (cplx *)malloc(sizeof(cplx) * (size_t)(2 * (n > m ? n : m) + 4));
+    cplx *w2 = (cplx *)malloc(sizeof(cplx) * (size_t)(2 * (n > m ? n : m) + 4));
+    if (!w1 || !w2) {
+#pragma omp atomic write
+      err = 1;
+    } else {
+      /* rows (x direction, contiguous), two at a time */
+#pragma omp for schedule(static)
+      for (int j = 0; j < m; j += 2) {
+        double *r0 = a + (size_t)j * n;
+        double *r1 = (j + 1 < m) ? a + (size_t)(j + 1) * n : NULL;
+        dst_pair(&px, w1, w2, r0, r1, n, 1);
+      }
+      /* columns (y direction, stride n), two at a time */
+#pragma omp for schedule(static)
+      for (int i = 0; i < n; i += 2) {
+        double *c0 = a + i;
+        double *c1 = (i + 1 < n) ? a + i + 1 : NULL;
+        dst_pair(&py, w1, w2, c0, c1, m, n);
+      }
+    }
+    free(w1);
+    free(w2);
+  }
+  free(px.tw);
+  free(py.tw);
+  return err ? QGW_ERR_NOMEM : QGW_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* seeded initial condition                                                   */
+/* ------------------------------------------------------------------------ */
+
+/* SplitMix64 (Steele, Lea & Flood 2014). */
+static uint64_t splitmix64(uint64_t *s) {
+  uint64_t z = (*s += 0x9e3779b97f4a7c15ULL);
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+  return z ^ (z >> 31);
+}
+
+/* uniform in (0, 1] */
+static double unif(uint64_t *s) { return ((double)(splitmix64(s) >> 11) + 1.0) * 0x1.0p-53; }
+
+int qgwh_make_ic(const qgw_params *p, const double *cl2m, const double *cm2l, const double *iRd2,
+                 uint64_t seed, double *psi, double *q) {
+  qgw_grid g;
+  int st = qgwh_grid(p, &g, NULL, NULL, NULL, NULL);
+  if (st) return st;
+  int n = g.n, m = g.m, nl = p->nl;
+  size_t NI = (size_t)n * m;
+  size_t S = (size_t)g.nxp2 * g.nyp2;
+  double *X = (double *)malloc(sizeof(double) * NI * nl);
+  double *W = (double *)malloc(sizeof(double) * NI * nl);
+  if (!X || !W) {
+    free(X);
+    free(W);
+    return QGW_ERR_NOMEM;
+  }
+  double kappa0 = 2.0 * QGW_PI / p->ic_lambda0;
+  double k06 = pow(kappa0, 6);
+  uint64_t s = seed;
+  int have = 0;
+  double spare = 0.0;
+  for (int l = 0; l < nl; l++)
+    for (int b = 1; b <= m; b++)
+      for (int a = 1; a <= n; a++) {
+        double gauss;
+        if (have) {
+          gauss = spare;
+          have = 0;
+        } else {
+          double u1 = unif(&s), u2 = unif(&s);
+          double r = sqrt(-2.0 * log(u1));
+          gauss = r * cos(2.0 * QGW_PI * u2);
+          spare = r * sin(2.0 * QGW_PI * u2);
+          have = 1;
+        }
+        double ka = QGW_PI * a / p->Lx, kb = QGW_PI * b / p->Lx;
+        double kap = sqrt(ka * ka + kb * kb);
+        double amp = 1.0 / sqrt(k06 + pow(kap, 6));
+        X[(size_t)l * NI + (size_t)(b - 1) * n + (a - 1)] = gauss * amp;
+      }
+  double norm = 4.0 * (n + 1) * (double)(m + 1);
+  /* psi per layer */
+  memcpy(W, X, sizeof(double) * NI * nl);
+  for (int l = 0; l < nl && !st; l++) st = qgwh_dst2d(W + (size_t)l * NI, n, m);
+  if (st) goto out;
+  for (size_t t = 0; t < NI * nl; t++) W[t] /= norm;
+  /* RMS of the forward-difference velocity over the interior (zero ring) */
+  double sum = 0.0;
+  for (int l = 0; l < nl; l++)
+    for (int j = 1; j <= m; j++)
+      for (int i = 1; i <= n; i++) {
+        double c = W[(size_t)l * NI + (size_t)(j - 1) * n + (i - 1)];
+        double e = (i < n) ? W[(size_t)l * NI + (size_t)(j - 1) * n + i] : 0.0;
+        double nn = (j < m) ? W[(size_t)l * NI + (size_t)j * n + (i - 1)] : 0.0;
+        double u = -(e - c) / g.Delta, v = (nn - c) / g.Delta;
+        sum += u * u + v * v;
+      }
+  double rms = sqrt(sum / ((double)NI * nl));
+  double scale = (rms > 0.0 && p->ic_urms > 0.0) ? p->ic_urms / rms : 1.0;
+  for (size_t t = 0; t < NI * nl; t++) {
+    X[t] *= scale;
+    W[t] *= scale;
+  }
+  for (size_t t = 0; t < S * nl; t++) psi[t] = 0.0;
+  for (int l = 0; l < nl; l++)
+    for (int j = 1; j <= m; j++)
+      for (int i = 1; i <= n; i++)
+        psi[(size_t)l * S + (size_t)j * g.nxp2 + i] = W[(size_t)l * NI + (size_t)(j - 1) * n + (i - 1)];
+  /* q: spectral modal Helmholtz operator applied to the coefficients */
+  {
+    double *Xm = (double *)malloc(sizeof(double) * nl);
+    double *Qm = (double *)malloc(sizeof(double) * nl);
+    if (!Xm || !Qm) {
+      free(Xm);
+      free(Qm);
+      st = QGW_ERR_NOMEM;
+      goto out;
+    }
+    for (int b = 1; b <= m; b++)
+      for (int a = 1; a <= n; a++) {
+        size_t o = (size_t)(b - 1) * n + (a - 1);
+        double La = QGW_PI * a / p->Lx, Kb = QGW_PI * b / p->Lx;
+        for (int k = 0; k < nl; k++) {
+          double acc = 0.0;
+          for (int l = 0; l < nl; l++) acc += cl2m[k * nl + l] * X[(size_t)l * NI + o];
+          Xm[k] = acc;
+          Qm[k] = -(La * La + Kb * Kb + iRd2[k]) * acc;
+        }
+        for (int l = 0; l < nl; l++) {
+          double acc = 0.0;
+          for (int k = 0; k < nl; k++) acc += cm2l[l * nl + k] * Qm[k];
+          W[(size_t)l * NI + o] = acc;
+        }
+      }
+    free(Xm);
+    free(Qm);
+  }
+  for (int l = 0; l < nl && !st; l++) st = qgwh_dst2d(W + (size_t)l * NI, n, m);
+  if (st) goto out;
+  for (size_t t = 0; t < S * nl; t++) q[t] = 0.0;
+  for (int l = 0; l < nl; l++)
+    for (int j = 1; j <= m; j++)
+      for (int i = 1; i <= n; i++)
+        q[(size_t)l * S + (size_t)j * g.nxp2 + i] = W[(size_t)l * NI + (size_t)(j - 1) * n + (i - 1)] / norm;
+  /* wall ring of q (src/domain.h:308-348 with a zero psi ring) */
+  {
+    double c = 2.0 * p->bc_fac / (g.Delta * g.Delta);
+    for (int l = 0; l < nl; l++) {
+      double *qq = q + (size_t)l * S, *pp = psi + (size_t)l * S;
+      for (int i = 0; i < g.nxp2; i++) {
+        qq[i] = c * pp[g.nxp2 + i];
+        qq[(size_t)(m + 1) * g.nxp2 + i] = c * pp[(size_t)m * g.nxp2 + i];
+      }
+      for (int j = 0; j < g.nyp2; j++) {
+        qq[(size_t)j * g.nxp2] = c * pp[(size_t)j * g.nxp2 + 1];
+        qq[(size_t)j * g.nxp2 + n + 1] = c * pp[(size_t)j * g.nxp2 + n];
+      }
+    }
+  }
+out:
+  free(X);
+  free(W);
+  return st;
+}
+
+/* ------------------------------------------------------------------------ */
+/* diagnostics                                                                */
+/* ------------------------------------------------------------------------ */
+
+int qgwh_diagnostics(const qgw_params *p, const double *psi, const double *q, double *E_layer,
+                     double *Z_layer, double *E_tot, double *Z_tot) {
+  qgw_grid g;
+  int st = qgwh_grid(p, &g, NULL, NULL, NULL, NULL);
+  if (st) return st;
+  size_t S = (size_t)g.nxp2 * g.nyp2;
+  double w = g.Delta * g.Delta / (p->Lx * p->Lx);
+  double H = 0.0, Et = 0.0, Zt = 0.0;
+  for (int l = 0; l < p->nl; l++) H += p->dh[l];
+  for (int l = 0; l < p->nl; l++) {
+    double e = 0.0, z = 0.0;
+    for (size_t t = 0; t < S; t++) {
+      e += psi[(size_t)l * S + t] * q[(size_t)l * S + t];
+      z += q[(size_t)l * S + t] * q[(size_t)l * S + t];
+    }
+    e = -e * w;
+    z = z * w;
+    if (E_layer) E_layer[l] = e;
+    if (Z_layer) Z_layer[l] = z;
+    Et += p->dh[l] * e;
+    Zt += p->dh[l] * z;
+  }
+  if (E_tot) *E_tot = Et / (2.0 * H);
+  if (Z_tot) *Z_tot = Zt / H;
+  return QGW_OK;
+}
+
+const char *qgwh_strerror(int s) {
+  switch (s) {
+    case QGW_OK: return "ok";
+    case QGW_ERR_ARG: return "invalid argument or unsupported configuration";
+    case QGW_ERR_IO: return "file not found or unreadable";
+    case QGW_ERR_NOMEM: return "allocation failed";
+    case QGW_ERR_HIP: return "HIP runtime error";
+    case QGW_ERR_NUMERIC: return "numerical failure (eigen-solver or non-finite state)";
+    case QGW_ERR_STATE: return "call out of order";
+    default: return "unknown status";
+  }
+}
+
+/* Size of the parameter struct, so that bindings can check their layout. */
+size_t qgwh_params_size(void) { return sizeof(qgw_params); }

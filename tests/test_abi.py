@@ -1,0 +1,47 @@
+"""The C-ABI libraries load and export every entry point the headers declare
+(no device compute here)."""
+import ctypes
+import os
+import re
+
+import qgw_amd as Q
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared(header):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(qgwh?_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_host_library_exports_header():
+    lib = Q.host_lib()
+    names = _declared("qgw_host.h")
+    assert names
+    for n in names:
+        assert hasattr(lib, n), n
+
+
+def test_hip_library_exports_header():
+    lib = Q.hip_lib()
+    names = _declared("qgw_hip.h")
+    assert "qgw_timestep" in names and "qgw_invert_pv" in names
+    for n in names:
+        assert hasattr(lib, n), n
+
+
+def test_params_struct_layout():
+    assert Q.host_lib().qgwh_params_size() == ctypes.sizeof(Q.Params)
+
+
+def test_strerror_codes():
+    lib = Q.hip_lib()
+    for code in (0, -1, -2, -3, -4, -5, -6, -99):
+        assert lib.qgw_strerror(code)
+
+
+def test_hip_library_is_gfx950():
+    so = os.path.join(ROOT, "qgw_amd", "libqgw_hip.so")
+    data = open(so, "rb").read()
+    assert b"gfx950" in data

@@ -1,0 +1,300 @@
+"""Pins of the oracle (CPU FP64 restatement of the reference step).
+
+Each check is chosen so that a plausible mistake in the oracle (a dropped
+term, a wrong sign or index, a transposed operand) fails it: library golden
+vectors (scipy), closed forms, invariants and brute force on small inputs.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+import qgw_amd as Q
+from qgw_amd import configs
+from conftest import rel
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+# ---------------------------------------------------------------- transforms
+
+def test_fft_matches_numpy():
+    rng = np.random.default_rng(1)
+    for L in (1, 2, 3, 8, 12, 48, 96, 97, 256, 3072):
+        z = rng.standard_normal(L) + 1j * rng.standard_normal(L)
+        assert np.abs(O.fft(z) - np.fft.fft(z)).max() < 1e-12 * max(1, np.abs(z).sum())
+
+
+def test_dst_golden_scipy():
+    g = np.load(os.path.join(GOLD, "dst_golden.npz"))
+    for n in (3, 7, 63, 255, 1535):
+        y = O.dst1d(g["x_%d" % n])
+        assert np.abs(y - g["y_%d" % n]).max() < 1e-12 * np.abs(g["y_%d" % n]).max()
+    assert np.allclose(O.dst1d(g["x_123"]), [9.65685425, -4.0, 1.65685425], atol=1e-8)
+    assert rel(O.dst2d(g["a2"]), g["A2"]) < 1e-14
+
+
+def test_dst_brute_force_and_involution():
+    rng = np.random.default_rng(2)
+    for n in (1, 5, 16, 47):
+        x = rng.standard_normal(n)
+        j = np.arange(1, n + 1)
+        S = 2 * np.sin(np.pi * np.outer(j, j) / (n + 1))
+        assert np.abs(O.dst1d(x) - S @ x).max() < 1e-12 * np.abs(S @ x).max()
+        # RODFT00 applied twice is 2(n+1) times the identity
+        assert np.abs(O.dst1d(O.dst1d(x)) - 2 * (n + 1) * x).max() < 1e-12 * (n + 1)
+
+
+def test_dst_sine_mode_is_delta():
+    n, a = 31, 5
+    j = np.arange(1, n + 1)
+    y = O.dst1d(np.sin(np.pi * a * j / (n + 1)))
+    expect = np.zeros(n)
+    expect[a - 1] = n + 1
+    assert np.abs(y - expect).max() < 1e-12
+
+
+# ---------------------------------------------------------------- eigenmodes
+
+def _normalised_from(w, U, dh):
+    """Reference normalisation applied to an orthonormal eigenbasis of T."""
+    dh = np.asarray(dh)
+    H = dh.sum()
+    vr = U / np.sqrt(dh)[:, None]
+    vl = U * np.sqrt(dh)[:, None]
+    for mm in range(len(dh)):
+        s = 1.0 if vr[0, mm] > 0 else -1.0
+        f = s * np.sqrt(H / np.sum(dh * vr[:, mm] ** 2))
+        vr[:, mm] *= f
+        vl[:, mm] /= np.dot(vr[:, mm], vl[:, mm])
+    return vl.T, vr  # cl2m[k, l], cm2l[l, k]
+
+
+@pytest.mark.parametrize("name,mk", [("c1", configs.c1), ("c2", configs.c2), ("c3", configs.c3)])
+def test_oracle_modes_golden(name, mk):
+    g = np.load(os.path.join(GOLD, "eig_golden.npz"))
+    p = mk()
+    o = O.Oracle(Q.default_params(NX=8, NY=8, nl=p.nl, Lx=p.Lx, dh=list(p.dh)[:p.nl],
+                                  N2=list(p.N2)[:p.nl - 1], f0=p.f0))
+    cl2m, cm2l, iRd2 = o.modes()
+    nl = p.nl
+    w = g[name + "_w"]
+    assert np.allclose(iRd2[1:], w[1:], rtol=1e-10)
+    assert iRd2[0] == 0.0
+    gl, gm = _normalised_from(w, g[name + "_U"], g[name + "_dh"])
+    assert np.abs(cl2m.reshape(nl, nl) - gl).max() < 1e-9 * np.abs(gl).max()
+    assert np.abs(cm2l.reshape(nl, nl) - gm).max() < 1e-9 * np.abs(gm).max()
+
+
+def test_oracle_modes_two_layer_closed_form():
+    # nl = 2: eigenvalues {0, f0^2/g' (1/H1 + 1/H2)}; 256^2 x 2 set: 1/Rd^2 = 1.142857e-9
+    p = configs.c1()
+    o = O.Oracle(p)
+    _, _, iRd2 = o.modes()
+    gp = p.N2[0] * 0.5 * (p.dh[0] + p.dh[1])
+    assert iRd2[1] == pytest.approx(p.f0 ** 2 / gp * (1 / p.dh[0] + 1 / p.dh[1]), rel=1e-13)
+    assert iRd2[1] == pytest.approx(1.142857142857e-9, rel=1e-10)
+
+
+def test_oracle_modes_invariants_and_known_values():
+    p = configs.c2()
+    o = O.Oracle(Q.default_params(NX=8, NY=8, nl=3, Lx=p.Lx, dh=list(p.dh)[:3], N2=list(p.N2)[:2], f0=p.f0))
+    cl2m, cm2l, iRd2 = [np.asarray(a) for a in o.modes()]
+    nl = 3
+    A, B = cl2m.reshape(nl, nl), cm2l.reshape(nl, nl)
+    assert np.abs(A @ B - np.eye(nl)).max() < 1e-12
+    G = Q.gamma(p)
+    lam = np.array([0.0] + list(iRd2[1:]))
+    # Gamma * cm2l = cm2l * diag(lambda) (barotropic eigenvalue forced to 0)
+    assert np.abs(G @ B - B * lam[None, :]).max() < 1e-10 * np.abs(G).max() * np.abs(B).max()
+    dh = np.array(p.dh[:3])
+    assert np.allclose((dh[:, None] * B ** 2).sum(0), dh.sum(), rtol=1e-12)
+    assert (B[0] > 0).all()
+    assert np.allclose(iRd2[1:], [7.3688e-10, 2.2818e-09], rtol=2e-4)
+
+
+def test_eig_sym_jacobi_vs_numpy():
+    rng = np.random.default_rng(3)
+    for n in (1, 2, 5, 16, 32):
+        M = rng.standard_normal((n, n))
+        M = M + M.T
+        w, V = O.eig_sym(M)
+        assert np.allclose(w, np.linalg.eigvalsh(M), atol=1e-11 * np.abs(M).max())
+        assert np.abs(M @ V - V * w[None, :]).max() < 1e-10 * np.abs(M).max()
+
+
+# ---------------------------------------------------------------- stencils
+
+def _state(NX=12, nl=1, **kw):
+    base = dict(NX=NX, NY=NX, nl=nl, Lx=1.2e6, dh=[1000.0] * nl, N2=[1e-5] * max(nl - 1, 1), f0=1e-4)
+    base.update(kw)
+    return O.Oracle(Q.default_params(**base))
+
+
+def test_jacobian_identities():
+    o = _state()
+    rng = np.random.default_rng(4)
+    a, b = rng.standard_normal(o.shape), rng.standard_normal(o.shape)
+    Jab, Jba = o.jacobian(a, b), o.jacobian(b, a)
+    assert np.abs(Jab + Jba).max() < 1e-12 * np.abs(Jab).max()
+    assert np.abs(o.jacobian(a, a)).max() < 1e-12 * np.abs(Jab).max()
+    assert np.abs(o.jacobian(a, np.full(o.shape, 3.7))).max() < 1e-12 * np.abs(Jab).max()
+    # J(x, y) = 1 for the coordinate fields
+    D = 1.2e6 / 12
+    jj, ii = np.meshgrid(np.arange(o.shape[1]), np.arange(o.shape[2]), indexing="ij")
+    X, Y = (ii * D)[None], (jj * D)[None]
+    J = o.jacobian(X, Y)
+    assert np.allclose(J[:, 1:-1, 1:-1], 1.0, rtol=1e-13)
+
+
+def test_jacobian_brute_force_three_jacobian_average():
+    # Arakawa (1966): J = (J++ + J+x + Jx+)/3, written here in its three-term
+    # form; the oracle uses the expanded 12-term form of src/domain.h:17-27
+    o = _state(NX=6)
+    rng = np.random.default_rng(5)
+    p, q = rng.standard_normal(o.shape), rng.standard_normal(o.shape)
+    D = 1.2e6 / 6
+    P, Qf = p[0], q[0]
+    ref = np.zeros_like(P)
+    for j in range(1, P.shape[0] - 1):
+        for i in range(1, P.shape[1] - 1):
+            jpp = ((P[j, i + 1] - P[j, i - 1]) * (Qf[j + 1, i] - Qf[j - 1, i]) -
+                   (P[j + 1, i] - P[j - 1, i]) * (Qf[j, i + 1] - Qf[j, i - 1]))
+            jpx = (P[j, i + 1] * (Qf[j + 1, i + 1] - Qf[j - 1, i + 1]) -
+                   P[j, i - 1] * (Qf[j + 1, i - 1] - Qf[j - 1, i - 1]) -
+                   P[j + 1, i] * (Qf[j + 1, i + 1] - Qf[j + 1, i - 1]) +
+                   P[j - 1, i] * (Qf[j - 1, i + 1] - Qf[j - 1, i - 1]))
+            jxp = (Qf[j + 1, i] * (P[j + 1, i + 1] - P[j + 1, i - 1]) -
+                   Qf[j - 1, i] * (P[j - 1, i + 1] - P[j - 1, i - 1]) -
+                   Qf[j, i + 1] * (P[j + 1, i + 1] - P[j - 1, i + 1]) +
+                   Qf[j, i - 1] * (P[j + 1, i - 1] - P[j - 1, i - 1]))
+            ref[j, i] = (jpp + jpx + jxp) / (12 * D * D)
+    J = o.jacobian(p, q)[0]
+    assert np.abs(J - ref).max() < 1e-12 * np.abs(ref).max()
+
+
+def test_laplacian_exact_on_quadratics_and_sine_eigenvalue():
+    o = _state(NX=16)
+    D = 1.2e6 / 16
+    jj, ii = np.meshgrid(np.arange(o.shape[1]), np.arange(o.shape[2]), indexing="ij")
+    x, y = ii * D, jj * D
+    L = o.laplacian((3 * x ** 2 - 2 * x * y + 5 * y ** 2)[None])
+    assert np.allclose(L[0, 1:-1, 1:-1], 6 - 0 + 10, rtol=1e-9)
+    n = 15
+    a, b = 3, 5
+    s = np.sin(np.pi * a * ii / (n + 1)) * np.sin(np.pi * b * jj / (n + 1))
+    mu = -(4 / D ** 2) * (np.sin(a * np.pi / (2 * (n + 1))) ** 2 + np.sin(b * np.pi / (2 * (n + 1))) ** 2)
+    L = o.laplacian(s[None])
+    assert np.abs(L[0, 1:-1, 1:-1] - mu * s[1:-1, 1:-1]).max() < 1e-12 * abs(mu)
+
+
+# ---------------------------------------------------------------- inversion
+
+@pytest.mark.parametrize("nl", [1, 3])
+def test_invert_pv_single_sine_mode(nl):
+    NX = 32
+    p = Q.default_params(NX=NX, NY=NX, nl=nl, Lx=1.0e6, dh=[500.0, 1000.0, 2500.0][:nl],
+                         N2=[2e-5, 1e-5][:max(nl - 1, 1)], f0=1e-4)
+    o = O.Oracle(p)
+    cl2m, cm2l, iRd2 = o.modes()
+    n = NX - 1
+    a, b, k = 4, 7, nl - 1
+    jj, ii = np.meshgrid(np.arange(n + 2), np.arange(n + 2), indexing="ij")
+    s = np.sin(np.pi * a * ii / (n + 1)) * np.sin(np.pi * b * jj / (n + 1))
+    v = np.asarray(cm2l).reshape(nl, nl)[:, k]
+    q = v[:, None, None] * s[None]
+    o.set_q(q)
+    o.invert_pv()
+    kap2 = (np.pi * a / 1.0e6) ** 2 + (np.pi * b / 1.0e6) ** 2 + iRd2[k]
+    assert np.abs(o.field(O.PSI) + q / kap2).max() < 1e-12 * np.abs(q / kap2).max()
+
+
+# ---------------------------------------------------------------- time stepping
+
+def test_ab3_coefficients_golden():
+    g = np.load(os.path.join(GOLD, "ab3_golden.npz"))
+    for (dt, dt0, dt1), c in zip(g["cases"], g["coeffs"]):
+        assert np.allclose(O.ab3_coeffs(dt, dt0, dt1), c, rtol=1e-13)
+        assert sum(O.ab3_coeffs(dt, dt0, dt1)) == pytest.approx(dt, rel=1e-13)
+    assert np.allclose(O.ab3_coeffs(1.0, 1.0, 1.0), [23 / 12, -16 / 12, 5 / 12], rtol=1e-14)
+
+
+def test_check_timestep_macro_quirks():
+    # DT_max: max(max(a,b),c) ignores c whenever a < b
+    p = Q.default_params(NX=64, NY=64, nl=1, Lx=1e6, nu=1.0, beta=1e-11, n_hyper=4, nu_hyper=1e9)
+    dt, DT = O.check_timestep(p)
+    a = -1e9 * 20 / (1e6 / 63) ** 4
+    b = 1.0 * 20 / (1e6 / 63) ** 2
+    assert DT == pytest.approx(1 / b, rel=1e-14)  # a < b, c = 2*beta*Lx is ignored
+    D = 1e6 / 64
+    # tau0 == 0 and sigma_f = 1 (default): dt = 1e-3 ((Delta cfl)^2/sigma_f)^(1/3)
+    assert dt == pytest.approx(1e-3 * ((D * 0.2) ** 2) ** (1 / 3), rel=1e-14)
+    # 1e-3*min(A,B) expands to (1e-3*A > B) ? B : A (no 1e-3 factor on the result)
+    p = Q.default_params(NX=64, NY=64, nl=1, Lx=1e6, tau0=1e-5, dh=[1000.0], sigma_f=1.0, beta=1e-11)
+    dt, DT = O.check_timestep(p)
+    D = 1e6 / 64
+    A = np.sqrt(0.2 * D / (1e-5 / 1000 * 1.0 * np.pi / 1e6 * 1e6))
+    B = ((D * 0.2) ** 2 / 1.0) ** (1 / 3)
+    assert 1e-3 * A > B
+    assert dt == pytest.approx(B, rel=1e-14)
+    assert DT == pytest.approx(1 / (2e-11 * 1e6), rel=1e-14)
+    # with nu = nu_hyper = beta = 0 the signed zero of nu_hyper (pow(-1,-1)*0 = -0)
+    # wins the quirky max, so DT_max = pow(-0., -1) = -inf and dt is capped to it
+    p = Q.default_params(NX=64, NY=64, nl=1, Lx=1e6, tau0=1e-5, dh=[1000.0])
+    dt, DT = O.check_timestep(p)
+    assert DT == -np.inf and dt == -np.inf
+
+
+def test_sine_mode_decay_under_hyperviscosity_matches_ab3_recursion():
+    # single layer, single sine mode, beta = J-terms = forcing = 0: q_k obeys
+    # q' = lam q with lam = -nu_h mu^3 / kappa^2 (mu: 5-point eigenvalue), and
+    # AB3 from zero history gives a closed recursion for the amplitude.
+    NX, a, b = 32, 3, 2
+    Lx = 1.0e6
+    p = Q.default_params(NX=NX, NY=NX, nl=1, Lx=Lx, dh=[1000.0], f0=1e-4, n_hyper=4, nu_hyper=1e12,
+                         dt_fixed=3600.0)
+    o = O.Oracle(p)
+    n = NX - 1
+    D = Lx / NX
+    jj, ii = np.meshgrid(np.arange(n + 2), np.arange(n + 2), indexing="ij")
+    s = np.sin(np.pi * a * ii / (n + 1)) * np.sin(np.pi * b * jj / (n + 1))
+    o.set_q(1e-6 * s[None])
+    kap2 = (np.pi * a / Lx) ** 2 + (np.pi * b / Lx) ** 2
+    mu = -(4 / D ** 2) * (np.sin(a * np.pi / (2 * (n + 1))) ** 2 + np.sin(b * np.pi / (2 * (n + 1))) ** 2)
+    # f = nu_h * lap^3 psi = nu_h mu^3 psi = nu_h mu^3 (-q/kap2)
+    lam = o.nu_hyper() * mu ** 3 * (-1.0 / kap2)
+    h = 3600.0
+    amp = [1.0]
+    f_hist = [0.0, 0.0]
+    for _ in range(20):
+        f0 = lam * amp[-1]
+        amp.append(amp[-1] + h * (23 * f0 - 16 * f_hist[0] + 5 * f_hist[1]) / 12)
+        f_hist = [f0, f_hist[0]]
+    o.run(20)
+    sel = np.abs(s) > 0.1
+    got = o.field(O.Q)[0][sel] / (1e-6 * s[sel])
+    assert lam < 0 and amp[-1] < 0.9999
+    assert np.abs(got - amp[-1]).max() < 1e-12
+
+
+def test_energy_enstrophy_conserved_by_jacobian_alone():
+    # no dissipation, beta, forcing or drag: only the Arakawa Jacobian acts;
+    # E and Z then change only at the time-truncation order of AB3
+    p = Q.default_params(NX=64, NY=64, nl=2, Lx=1e6, dh=[1000.0, 3000.0], N2=[1e-5], f0=1e-4,
+                         dt_fixed=600.0, ic_urms=0.1, ic_lambda0=200e3, seed=7)
+    modes = Q.eigmode(p)
+    psi0, q0 = Q.make_ic(p, modes)
+    q0[:, :3, :] = 0
+    q0[:, -3:, :] = 0
+    q0[:, :, :3] = 0
+    q0[:, :, -3:] = 0
+    o = O.Oracle(p)
+    o.set_q(q0)
+    o.invert_pv()
+    _, _, E0, Z0 = o.diagnostics()
+    o.run(40)
+    o.invert_pv()
+    _, _, E1, Z1 = o.diagnostics()
+    assert abs(E1 - E0) / abs(E0) < 1e-4
+    assert abs(Z1 - Z0) / abs(Z0) < 1e-3
