@@ -1,0 +1,218 @@
+// fft_reg.hpp -- register-blocked workgroup FFT for gfx950.
+//
+// Each of NT = L/E threads holds E elements of the length-L sequence in
+// registers, always at its "home" positions p = t + m*NT (m = 0..E-1), both
+// before the first stage and after the last one.  Stages are Stockham radix-R
+// butterflies (R divides E, so a thread runs E/R butterflies per stage); only
+// the exchanges between stages go through LDS, with one pad slot per 16
+// entries, which keeps the strided writes of the early stages on distinct
+// banks.  Because input and output live at the home positions, callers load
+// their sequence straight from global memory and store the result straight
+// back, with no extra LDS pass at either end.
+//
+// Lengths 2^a use E = 16 (L >= 1024) or E = 8 (radices 16/8/4/2); lengths
+// 3*2^a use E = 8 with the LDS Stockham of fft.hpp between a write and a read
+// of the home positions.
+#pragma once
+#include "fft.hpp"
+
+namespace qgw {
+
+constexpr bool is_pow2(int L) { return L > 0 && (L & (L - 1)) == 0; }
+
+template <int L> struct RegPlan {
+  static constexpr int E = is_pow2(L) ? ((L >= 1024) ? 16 : 8) : 8;
+  static constexpr int NT = L / E;
+  // radix of stage s (powers of two): E while the remaining length allows, then the rest
+  static constexpr int radix(int s) {
+    int rem = L, r = 0;
+    for (int i = 0; i <= s; i++) {
+      r = (rem >= E) ? E : rem;
+      rem /= r;
+    }
+    return r;
+  }
+  static constexpr int count() {
+    int rem = L, c = 0;
+    while (rem > 1) {
+      rem /= (rem >= E) ? E : rem;
+      c++;
+    }
+    return c;
+  }
+  static constexpr int ns(int s) {
+    int p = 1;
+    for (int i = 0; i < s; i++) p *= radix(i);
+    return p;
+  }
+  // LDS slots: one pad per 16 entries
+  static constexpr int slots = L + L / 16 + 1;
+};
+
+__device__ __forceinline__ int rslot(int i) { return i + (i >> 4); }
+
+// ---- radix-16 butterfly: 4 x 4 with internal twiddles W16^(n2 k1) --------
+template <typename T> __device__ __forceinline__ void bfly16(cx<T> *v) {
+  const T c1 = T(0.92387953251128675612818318939678829);  // cos(pi/8)
+  const T s1 = T(0.38268343236508977172845998403039887);  // sin(pi/8)
+  const T r2 = T(0.70710678118654752440084436210484904);
+  cx<T> y[4][4];
+#pragma unroll
+  for (int n2 = 0; n2 < 4; n2++) {
+    cx<T> a[4] = {v[n2], v[n2 + 4], v[n2 + 8], v[n2 + 12]};
+    Bfly<T, 4>::run(a);
+#pragma unroll
+    for (int k1 = 0; k1 < 4; k1++) y[n2][k1] = a[k1];
+  }
+  // y[n2][k1] *= W16^(n2*k1), W16 = exp(-2 pi i/16)
+  // n2*k1 = 1: (c1, -s1); 2: (r2, -r2); 3: (s1, -c1); 4: -i; 6: (-r2, -r2); 9: (-c1, s1)
+  auto mulw = [&](cx<T> a, T wr, T wi) { return mkc<T>(a.x * wr - a.y * wi, a.x * wi + a.y * wr); };
+  y[1][1] = mulw(y[1][1], c1, -s1);
+  y[1][2] = mulw(y[1][2], r2, -r2);
+  y[1][3] = mulw(y[1][3], s1, -c1);
+  y[2][1] = mulw(y[2][1], r2, -r2);
+  y[2][2] = cnegi<T>(y[2][2]);
+  y[2][3] = mulw(y[2][3], -r2, -r2);
+  y[3][1] = mulw(y[3][1], s1, -c1);
+  y[3][2] = mulw(y[3][2], -r2, -r2);
+  y[3][3] = mulw(y[3][3], -c1, s1);
+#pragma unroll
+  for (int k1 = 0; k1 < 4; k1++) {
+    cx<T> a[4] = {y[0][k1], y[1][k1], y[2][k1], y[3][k1]};
+    Bfly<T, 4>::run(a);
+#pragma unroll
+    for (int k2 = 0; k2 < 4; k2++) v[k1 + 4 * k2] = a[k2];
+  }
+}
+
+template <typename T, int R> __device__ __forceinline__ void bfly(cx<T> *v) {
+  if constexpr (R == 16)
+    bfly16<T>(v);
+  else
+    Bfly<T, R>::run(v);
+}
+
+// Twiddle the R inputs of one butterfly: v[r] *= W^(r k), W = tw[step] (index
+// into the size-L table).  Powers are built from at most log2(R) table loads.
+template <typename T, int R, int L>
+__device__ __forceinline__ void twiddle(cx<T> *v, const cx<T> *__restrict__ tw, int base) {
+  // base = k * L/(Ns*R): the table index of W^1
+  if constexpr (R == 2) {
+    v[1] = cmul<T>(v[1], tw[base]);
+  } else if constexpr (R == 4) {
+    const cx<T> w1 = tw[base], w2 = tw[2 * base];
+    v[1] = cmul<T>(v[1], w1);
+    v[2] = cmul<T>(v[2], w2);
+    v[3] = cmul<T>(v[3], cmul<T>(w1, w2));
+  } else if constexpr (R == 8) {
+    const cx<T> w1 = tw[base], w2 = tw[2 * base], w4 = tw[4 * base];
+    const cx<T> w3 = cmul<T>(w1, w2);
+    v[1] = cmul<T>(v[1], w1);
+    v[2] = cmul<T>(v[2], w2);
+    v[3] = cmul<T>(v[3], w3);
+    v[4] = cmul<T>(v[4], w4);
+    v[5] = cmul<T>(v[5], cmul<T>(w1, w4));
+    v[6] = cmul<T>(v[6], cmul<T>(w2, w4));
+    v[7] = cmul<T>(v[7], cmul<T>(w3, w4));
+  } else if constexpr (R == 16) {
+    const cx<T> w1 = tw[base], w2 = tw[2 * base], w4 = tw[4 * base], w8 = tw[8 * base];
+    const cx<T> w3 = cmul<T>(w1, w2), w5 = cmul<T>(w1, w4), w6 = cmul<T>(w2, w4);
+    const cx<T> w7 = cmul<T>(w3, w4);
+    v[1] = cmul<T>(v[1], w1);
+    v[2] = cmul<T>(v[2], w2);
+    v[3] = cmul<T>(v[3], w3);
+    v[4] = cmul<T>(v[4], w4);
+    v[5] = cmul<T>(v[5], w5);
+    v[6] = cmul<T>(v[6], w6);
+    v[7] = cmul<T>(v[7], w7);
+    v[8] = cmul<T>(v[8], w8);
+    v[9] = cmul<T>(v[9], cmul<T>(w1, w8));
+    v[10] = cmul<T>(v[10], cmul<T>(w2, w8));
+    v[11] = cmul<T>(v[11], cmul<T>(w3, w8));
+    v[12] = cmul<T>(v[12], cmul<T>(w4, w8));
+    v[13] = cmul<T>(v[13], cmul<T>(w5, w8));
+    v[14] = cmul<T>(v[14], cmul<T>(w6, w8));
+    v[15] = cmul<T>(v[15], cmul<T>(w7, w8));
+  }
+}
+
+// Stage S of the power-of-two plan.  On entry v[] holds this stage's inputs:
+// v[w*R + r] = X[j_w + r*L/R], j_w = t + w*NT.  On exit (S not last) v[] holds
+// the next stage's inputs; for the last stage v[w*R + r] is the output at
+// position t + (w + r*E/R)*NT.
+template <typename T, int L, int S>
+__device__ __forceinline__ void reg_stage(cx<T> *v, cx<T> *buf, const cx<T> *__restrict__ tw, int t) {
+  using Pl = RegPlan<L>;
+  constexpr int E = Pl::E, NT = Pl::NT;
+  constexpr int R = Pl::radix(S);
+  constexpr int Ns = Pl::ns(S);
+  constexpr int NB = E / R;  // butterflies per thread
+#pragma unroll
+  for (int w = 0; w < NB; w++) {
+    const int j = t + w * NT;
+    const int k = j % Ns;
+    if constexpr (Ns > 1) twiddle<T, R, L>(v + w * R, tw, k * (L / (Ns * R)));
+    bfly<T, R>(v + w * R);
+  }
+  if constexpr (S + 1 < Pl::count()) {
+    constexpr int R2 = Pl::radix(S + 1);
+    constexpr int NB2 = E / R2;
+    __syncthreads();  // previous readers of buf are done
+#pragma unroll
+    for (int w = 0; w < NB; w++) {
+      const int j = t + w * NT;
+      const int base = (j / Ns) * Ns * R + (j % Ns);
+#pragma unroll
+      for (int r = 0; r < R; r++) buf[rslot(base + r * Ns)] = v[w * R + r];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int w = 0; w < NB2; w++) {
+      const int j = t + w * NT;
+#pragma unroll
+      for (int r = 0; r < R2; r++) v[w * R2 + r] = buf[rslot(j + r * (L / R2))];
+    }
+    reg_stage<T, L, S + 1>(v, buf, tw, t);
+  }
+}
+
+// Home-position index helpers: element m of a thread sits at position
+// t + m*NT.  Stage inputs of the first stage and outputs of the last stage are
+// permutations of the home set; these map register slots to m.
+template <int L> constexpr int in_m(int slot) {
+  // first stage radix R0: v[w*R0 + r] = X[t + w*NT + r*(E/R0)*NT]
+  constexpr int E = RegPlan<L>::E, R0 = RegPlan<L>::radix(0);
+  return (slot / R0) + (slot % R0) * (E / R0);
+}
+template <int L> constexpr int out_m(int slot) {
+  constexpr int E = RegPlan<L>::E, RL = RegPlan<L>::radix(RegPlan<L>::count() - 1);
+  return (slot / RL) + (slot % RL) * (E / RL);
+}
+
+// Forward FFT of the sequence held at the home positions.  h[m] is the value
+// at position t + m*NT, in and out.  buf: RegPlan<L>::slots entries (powers of
+// two) or lds_slots(L) (other lengths).
+template <typename T, int L>
+__device__ __forceinline__ void fft_home(cx<T> *h, cx<T> *buf, const cx<T> *__restrict__ tw, int t) {
+  constexpr int E = RegPlan<L>::E, NT = RegPlan<L>::NT;
+  if constexpr (is_pow2(L)) {
+    cx<T> v[E];
+#pragma unroll
+    for (int s = 0; s < E; s++) v[s] = h[in_m<L>(s)];
+    reg_stage<T, L, 0>(v, buf, tw, t);
+#pragma unroll
+    for (int s = 0; s < E; s++) h[out_m<L>(s)] = v[s];
+  } else {
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < E; m++) buf[lds_slot(t + m * NT)] = h[m];
+    __syncthreads();
+    Stages<T, L, NT, 0>::run(buf, tw, t);
+#pragma unroll
+    for (int m = 0; m < E; m++) h[m] = buf[lds_slot(t + m * NT)];
+  }
+}
+
+template <int L> constexpr int home_slots() { return is_pow2(L) ? RegPlan<L>::slots : lds_slots(L); }
+
+}  // namespace qgw

@@ -4,18 +4,18 @@
 //   q[2], psi, f[3], comp   nl layers of (m+2) rows, row pitch P >= n+2; the
 //                           one-point wall ring is stored (reference idx
 //                           layout, src/domain.h:62, with a padded pitch)
-//   what                    nl modes of m rows (y) x n columns (x frequency), pitch Pn
+//   what                    nl layers of m rows (y) x n columns (x frequency), pitch Pn
 //   whatT                   nl modes of n rows (x frequency) x m columns (y), pitch Pm
 //
 // Passes of one step (bounded branch of invert_pv, src/elliptic.h:76-155, and
 // rhs/timestep, src/dynamics.h:47-73, src/timestep.h:114-167):
-//   P1  per grid row: layer->mode projection, forward sine transform along x
-//   T1  what -> whatT (tile transpose)
-//   P2  per x-frequency: forward sine transform along y, division by
+//   P1  per grid row and layer pair: forward sine transform along x
+//   T1  what (layers) -> whatT (modes): tile transpose + layer->mode projection
+//   P2  per x-frequency and mode: forward sine transform along y, division by
 //       -(L^2 + K^2 + 1/Rd^2) * 4(n+1)(m+1), inverse sine transform along y
-//   T2  whatT -> what
-//   P3  per grid row: inverse sine transform along x, mode->layer projection,
-//       psi row and the wall ring of psi and q
+//   T2  whatT (modes) -> what (layers): tile transpose + mode->layer projection
+//   P3  per grid row and layer pair: inverse sine transform along x, psi row
+//       and the wall ring of psi and q
 //   P4  per 2-D tile and layer: Arakawa Jacobian, beta, nu lap q,
 //       nu_hyper lap^(n_hyper+2) psi, wind, drag, topography, AB3 update
 //
@@ -28,8 +28,17 @@
 #include <stdint.h>
 
 #include "fft.hpp"
+#include "fft_reg.hpp"
 
 namespace qgw {
+
+// Workgroup id -> work item such that the ids the hardware deals round-robin
+// to one of the 8 XCDs cover a contiguous range of work items.
+__device__ __forceinline__ int xcd_remap(int b, int total) {
+  const int x = b & 7, i = b >> 3;
+  const int q = total >> 3, r = total & 7;
+  return x * q + (x < r ? x : r) + i;
+}
 
 // Sizes shared by every kernel.
 struct Geo {
@@ -43,254 +52,244 @@ struct Geo {
 };
 
 // ===========================================================================
-// P1: projection + forward transform along x.  One workgroup per grid row j,
-// all modes; q of the row is held in registers (NLC layers, PPT points per
-// thread) and the modes are transformed two at a time.
+// P1: forward sine transform along x of two layers of one grid row.  The
+// pair (layer l0, layer l1) is packed as a + i b and odd-extended; each thread
+// loads its home positions straight from q.
 // ===========================================================================
-template <typename T, int L, int NLC>
-__global__ void __launch_bounds__(fft_threads(L))
-k_p1(Geo g, const T *__restrict__ q, T *__restrict__ what, const T *__restrict__ cl2m,
-     const cx<T> *__restrict__ tw) {
-  constexpr int NT = fft_threads(L);
-  constexpr int PPT = (L / 2 - 1 + NT - 1) / NT;
-  __shared__ cx<T> buf[lds_slots(L)];
+template <typename T, int L>
+__global__ void __launch_bounds__(RegPlan<L>::NT)
+k_p1(Geo g, const T *__restrict__ q, T *__restrict__ what, const cx<T> *__restrict__ tw) {
+  constexpr int E = RegPlan<L>::E, NT = RegPlan<L>::NT;
+  __shared__ cx<T> buf[home_slots<L>()];
   const int t = threadIdx.x;
-  const int j = blockIdx.x + 1;  // grid row 1..m
-  const int n = g.n, nl = g.nl;
-  T qv[NLC][PPT];
+  const int npair = (g.nl + 1) / 2;
+  const int lp = blockIdx.x % npair;
+  const int j = blockIdx.x / npair + 1;
+  const int l0 = 2 * lp, l1 = l0 + 1;
+  const bool has1 = l1 < g.nl;
+  const T *r0 = q + l0 * g.LS + (long)j * g.P + 1;
+  const T *r1 = q + (has1 ? l1 : l0) * g.LS + (long)j * g.P + 1;
+  cx<T> h[E];
 #pragma unroll
-  for (int l = 0; l < NLC; l++)
-#pragma unroll
-    for (int s = 0; s < PPT; s++) {
-      const int x = t + s * NT;
-      qv[l][s] = (l < nl && x < n) ? q[l * g.LS + (long)j * g.P + x + 1] : T(0);
-    }
-  for (int k0 = 0; k0 < nl; k0 += 2) {
-    const int k1 = k0 + 1;
-    const bool has1 = k1 < nl;
-#pragma unroll
-    for (int s = 0; s < PPT; s++) {
-      const int x = t + s * NT;
-      if (x < n) {
-        T w0 = T(0), w1 = T(0);
-#pragma unroll
-        for (int l = 0; l < NLC; l++) {
-          if (l < nl) {
-            w0 += cl2m[k0 * nl + l] * qv[l][s];
-            if (has1) w1 += cl2m[k1 * nl + l] * qv[l][s];
-          }
-        }
-        buf[lds_slot(x + 1)] = mkc<T>(w0, w1);
-        buf[lds_slot(L - x - 1)] = mkc<T>(-w0, -w1);
+  for (int m = 0; m < E; m++) {
+    const int p = t + m * NT;
+    T a = T(0), b = T(0);
+    if (p != 0 && p != L / 2) {
+      const int x = (p < L / 2) ? p - 1 : L - p - 1;
+      a = r0[x];
+      b = has1 ? r1[x] : T(0);
+      if (p > L / 2) {
+        a = -a;
+        b = -b;
       }
     }
-    if (t == 0) {
-      buf[lds_slot(0)] = mkc<T>(T(0), T(0));
-      buf[lds_slot(L / 2)] = mkc<T>(T(0), T(0));
-    }
-    __syncthreads();
-    fft_lds<T, L, NT>(buf, tw, t);
-    T *o0 = what + k0 * g.WS + (long)(j - 1) * g.Pn;
-    T *o1 = what + k1 * g.WS + (long)(j - 1) * g.Pn;
+    h[m] = mkc<T>(a, b);
+  }
+  fft_home<T, L>(h, buf, tw, t);
+  T *o0 = what + l0 * g.WS + (long)(j - 1) * g.Pn;
+  T *o1 = what + l1 * g.WS + (long)(j - 1) * g.Pn;
 #pragma unroll
-    for (int s = 0; s < PPT; s++) {
-      const int x = t + s * NT;
-      if (x < n) {
-        cx<T> c = buf[lds_slot(x + 1)];
-        o0[x] = -c.y;
-        if (has1) o1[x] = c.x;
-      }
+  for (int m = 0; m < E; m++) {
+    const int p = t + m * NT;
+    if (p >= 1 && p < L / 2) {
+      o0[p - 1] = -h[m].y;
+      if (has1) o1[p - 1] = h[m].x;
     }
-    __syncthreads();
   }
 }
 
 // ===========================================================================
-// Tile transpose: dst[b][c][r] = src[b][r][c], r < R, c < C.
+// Transpose with a layer/mode contraction:
+//   dst[k][c][r] = sum_l M[k*nl + l] * src[l][r][c],   r < R, c < C.
+// T1 uses M = cl2m (layers -> modes, w_k = sum_l cl2m[k,l] q_l,
+// src/elliptic.h:90-98); T2 uses M = cm2l (modes -> layers, psi_l +=
+// cm2l[l,k] w_k, src/elliptic.h:140-147).  Both projections commute with the
+// sine transforms, so they sit where the data is transposed anyway.
 // ===========================================================================
-template <typename T, int TILE>
+template <typename T, int TC, int TR, int NLC>
 __global__ void __launch_bounds__(256)
-k_transpose(const T *__restrict__ src, T *__restrict__ dst, int R, int C, int Ps, long Ss, int Pd, long Sd) {
-  __shared__ T tile[TILE][TILE + 1];
-  const int b = blockIdx.z;
-  const int c0 = blockIdx.x * TILE, r0 = blockIdx.y * TILE;
-  const T *s = src + b * Ss;
-  T *d = dst + b * Sd;
-  const int tx = threadIdx.x % TILE, ty = threadIdx.x / TILE;
-  constexpr int RS = 256 / TILE;
+k_tproj(const T *__restrict__ src, T *__restrict__ dst, const T *__restrict__ M, int nl, int R, int C, int Ps,
+        long Ss, int Pd, long Sd) {
+  static_assert(TC * 8 == 256 && TR % 8 == 0, "tile shape");
+  constexpr int U = TR / 8;  // source rows per thread
+  __shared__ T tile[TR][TC + 1];
+  const int c0 = blockIdx.x * TC, r0 = blockIdx.y * TR;
+  const int tid = threadIdx.x;
+  // every layer of the thread's points, lanes along c (contiguous in src)
+  const int ci = tid % TC, rg = tid / TC;
+  T v[NLC][U];
 #pragma unroll
-  for (int i = ty; i < TILE; i += RS) {
-    const int r = r0 + i, c = c0 + tx;
-    if (r < R && c < C) tile[i][tx] = s[(long)r * Ps + c];
-  }
-  __syncthreads();
+  for (int l = 0; l < NLC; l++)
 #pragma unroll
-  for (int i = ty; i < TILE; i += RS) {
-    const int c = c0 + i, r = r0 + tx;
-    if (r < R && c < C) d[(long)c * Pd + r] = tile[tx][i];
+    for (int u = 0; u < U; u++) {
+      const int r = r0 + rg + 8 * u, c = c0 + ci;
+      v[l][u] = (l < nl && r < R && c < C) ? src[l * Ss + (long)r * Ps + c] : T(0);
+    }
+  // store mapping: lanes along r (contiguous in dst)
+  const int rl = tid % TR, cg = tid / TR;
+  constexpr int CS = 256 / TR;  // columns per store pass
+  for (int k = 0; k < nl; k++) {
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      T acc = T(0);
+#pragma unroll
+      for (int l = 0; l < NLC; l++)
+        if (l < nl) acc += M[k * nl + l] * v[l][u];  // uniform address: scalar loads
+      tile[rg + 8 * u][ci] = acc;
+    }
+    __syncthreads();
+    const int r = r0 + rl;
+#pragma unroll
+    for (int cc = cg; cc < TC; cc += CS) {
+      const int c = c0 + cc;
+      if (r < R && c < C) dst[k * Sd + (long)c * Pd + r] = tile[rl][cc];
+    }
+    __syncthreads();
   }
 }
 
 // ===========================================================================
 // P2: per pair of whatT rows (mode k, x frequency a): forward transform along
-// y, Helmholtz division, inverse transform along y, in place.
+// y, Helmholtz division, inverse transform along y.  The division is applied
+// at the home positions in registers: position p holds C_p, and the input of
+// the inverse transform at p is (-Im C_p) f_0(|p|) + i Re(C_p) f_1(|p|) with
+// |p| = p or L - p, which is the odd extension of the divided spectra.
+// f = 1 / (-(L_a^2 + K_b^2 + 1/Rd_k^2) * 4(n+1)(m+1)) (src/elliptic.h:120-138);
+// the denominator is summed in FP64.
 // ===========================================================================
 template <typename T, int L>
-__global__ void __launch_bounds__(fft_threads(L))
+__global__ void __launch_bounds__(RegPlan<L>::NT)
 k_p2(Geo g, T *__restrict__ whatT, const double *__restrict__ La2, const double *__restrict__ Kb2,
      const double *__restrict__ iRd2, double norm, const cx<T> *__restrict__ tw) {
-  constexpr int NT = fft_threads(L);
-  constexpr int PPT = (L / 2 - 1 + NT - 1) / NT;
-  __shared__ cx<T> buf[lds_slots(L)];
+  constexpr int E = RegPlan<L>::E, NT = RegPlan<L>::NT;
+  __shared__ cx<T> buf[home_slots<L>()];
   const int t = threadIdx.x;
-  const int m = g.m, n = g.n;
+  const int n = g.n;
   const long rows = (long)g.nl * n;
   const long r0 = 2L * blockIdx.x, r1 = r0 + 1;
   const bool has1 = r1 < rows;
   const int k0 = (int)(r0 / n), a0 = (int)(r0 % n) + 1;
-  const int k1 = has1 ? (int)(r1 / n) : 0, a1 = has1 ? (int)(r1 % n) + 1 : 1;
+  const int k1 = has1 ? (int)(r1 / n) : k0, a1 = has1 ? (int)(r1 % n) + 1 : a0;
   T *row0 = whatT + k0 * g.WTS + (long)(a0 - 1) * g.Pm;
   T *row1 = whatT + k1 * g.WTS + (long)(a1 - 1) * g.Pm;
+  cx<T> h[E];
 #pragma unroll
-  for (int s = 0; s < PPT; s++) {
-    const int y = t + s * NT;
-    if (y < m) {
-      T v0 = row0[y];
-      T v1 = has1 ? row1[y] : T(0);
-      buf[lds_slot(y + 1)] = mkc<T>(v0, v1);
-      buf[lds_slot(L - y - 1)] = mkc<T>(-v0, -v1);
+  for (int m = 0; m < E; m++) {
+    const int p = t + m * NT;
+    T a = T(0), b = T(0);
+    if (p != 0 && p != L / 2) {
+      const int y = (p < L / 2) ? p - 1 : L - p - 1;
+      a = row0[y];
+      b = has1 ? row1[y] : T(0);
+      if (p > L / 2) {
+        a = -a;
+        b = -b;
+      }
     }
+    h[m] = mkc<T>(a, b);
   }
-  if (t == 0) {
-    buf[lds_slot(0)] = mkc<T>(T(0), T(0));
-    buf[lds_slot(L / 2)] = mkc<T>(T(0), T(0));
-  }
-  __syncthreads();
-  fft_lds<T, L, NT>(buf, tw, t);
-  // Helmholtz division in FP64: 1 / (-(L_a^2 + K_b^2 + iRd2_k) * norm)
-  const double la0 = La2[a0], la1 = La2[a1];
+  fft_home<T, L>(h, buf, tw, t);
+  const double d0 = La2[a0] + 0.0, d1 = La2[a1] + 0.0;
   const double ir0 = iRd2[k0], ir1 = iRd2[k1];
 #pragma unroll
-  for (int s = 0; s < PPT; s++) {
-    const int y = t + s * NT;
-    if (y < m) {
-      const int b = y + 1;
-      cx<T> c = buf[lds_slot(b)];
+  for (int m = 0; m < E; m++) {
+    const int p = t + m * NT;
+    if (p == 0 || p == L / 2) {
+      h[m] = mkc<T>(T(0), T(0));
+    } else {
+      const int b = (p < L / 2) ? p : L - p;
       const double kb = Kb2[b];
-      const T f0 = (T)(1.0 / (-(la0 + kb + ir0) * norm));
-      const T f1 = (T)(1.0 / (-(la1 + kb + ir1) * norm));
-      const T z0 = -c.y * f0, z1 = c.x * f1;
-      buf[lds_slot(b)] = mkc<T>(z0, z1);
-      buf[lds_slot(L - b)] = mkc<T>(-z0, -z1);
+      const double den0 = -(d0 + kb + ir0) * norm;
+      const double den1 = -(d1 + kb + ir1) * norm;
+      T f0, f1;
+      if constexpr (sizeof(T) == 8) {
+        f0 = (T)(1.0 / den0);
+        f1 = (T)(1.0 / den1);
+      } else {
+        f0 = T(1) / (T)den0;
+        f1 = T(1) / (T)den1;
+      }
+      h[m] = mkc<T>(-h[m].y * f0, h[m].x * f1);
     }
   }
-  if (t == 0) {
-    buf[lds_slot(0)] = mkc<T>(T(0), T(0));
-    buf[lds_slot(L / 2)] = mkc<T>(T(0), T(0));
-  }
-  __syncthreads();
-  fft_lds<T, L, NT>(buf, tw, t);
+  fft_home<T, L>(h, buf, tw, t);
 #pragma unroll
-  for (int s = 0; s < PPT; s++) {
-    const int y = t + s * NT;
-    if (y < m) {
-      cx<T> c = buf[lds_slot(y + 1)];
-      row0[y] = -c.y;
-      if (has1) row1[y] = c.x;
+  for (int m = 0; m < E; m++) {
+    const int p = t + m * NT;
+    if (p >= 1 && p < L / 2) {
+      row0[p - 1] = -h[m].y;
+      if (has1) row1[p - 1] = h[m].x;
     }
   }
 }
 
 // ===========================================================================
-// P3: per grid row j: inverse transform along x for every mode (two at a
-// time), mode->layer projection accumulated in registers, psi row with its
-// zero wall values, and the wall ring of q (src/domain.h:308-348: q ring =
-// 2 bc_fac / Delta^2 * adjacent psi; corners are zero because psi's ring is).
+// P3: inverse transform along x of two layers of one grid row, the psi row
+// with its zero wall values, and the wall ring of q (src/domain.h:308-348:
+// q ring = 2 bc_fac/Delta^2 * adjacent psi; corners are zero because psi's
+// ring is).
 // ===========================================================================
-template <typename T, int L, int NLC>
-__global__ void __launch_bounds__(fft_threads(L))
-k_p3(Geo g, const T *__restrict__ what, T *__restrict__ psi, T *__restrict__ q, const T *__restrict__ cm2l,
-     T ring_c, const cx<T> *__restrict__ tw) {
-  constexpr int NT = fft_threads(L);
-  constexpr int PPT = (L / 2 - 1 + NT - 1) / NT;
-  __shared__ cx<T> buf[lds_slots(L)];
+template <typename T, int L>
+__global__ void __launch_bounds__(RegPlan<L>::NT)
+k_p3(Geo g, const T *__restrict__ what, T *__restrict__ psi, T *__restrict__ q, T ring_c,
+     const cx<T> *__restrict__ tw) {
+  constexpr int E = RegPlan<L>::E, NT = RegPlan<L>::NT;
+  __shared__ cx<T> buf[home_slots<L>()];
   const int t = threadIdx.x;
-  const int j = blockIdx.x + 1;
-  const int n = g.n, m = g.m, nl = g.nl;
-  T acc[NLC][PPT];
+  const int n = g.n, m_ = g.m;
+  const int npair = (g.nl + 1) / 2;
+  const int lp = blockIdx.x % npair;
+  const int j = blockIdx.x / npair + 1;
+  const int l0 = 2 * lp, l1 = l0 + 1;
+  const bool has1 = l1 < g.nl;
+  const T *i0 = what + l0 * g.WS + (long)(j - 1) * g.Pn;
+  const T *i1 = what + (has1 ? l1 : l0) * g.WS + (long)(j - 1) * g.Pn;
+  cx<T> h[E];
 #pragma unroll
-  for (int l = 0; l < NLC; l++)
+  for (int m = 0; m < E; m++) {
+    const int p = t + m * NT;
+    T a = T(0), b = T(0);
+    if (p != 0 && p != L / 2) {
+      const int x = (p < L / 2) ? p - 1 : L - p - 1;
+      a = i0[x];
+      b = has1 ? i1[x] : T(0);
+      if (p > L / 2) {
+        a = -a;
+        b = -b;
+      }
+    }
+    h[m] = mkc<T>(a, b);
+  }
+  fft_home<T, L>(h, buf, tw, t);
 #pragma unroll
-    for (int s = 0; s < PPT; s++) acc[l][s] = T(0);
-  for (int k0 = 0; k0 < nl; k0 += 2) {
-    const int k1 = k0 + 1;
-    const bool has1 = k1 < nl;
-    const T *i0 = what + k0 * g.WS + (long)(j - 1) * g.Pn;
-    const T *i1 = what + k1 * g.WS + (long)(j - 1) * g.Pn;
+  for (int side = 0; side < 2; side++) {
+    const int l = side ? l1 : l0;
+    if (side && !has1) break;
+    T *pr = psi + l * g.LS + (long)j * g.P;
+    T *qr = q + l * g.LS;
 #pragma unroll
-    for (int s = 0; s < PPT; s++) {
-      const int x = t + s * NT;
-      if (x < n) {
-        T v0 = i0[x];
-        T v1 = has1 ? i1[x] : T(0);
-        buf[lds_slot(x + 1)] = mkc<T>(v0, v1);
-        buf[lds_slot(L - x - 1)] = mkc<T>(-v0, -v1);
+    for (int m = 0; m < E; m++) {
+      const int p = t + m * NT;
+      if (p >= 1 && p < L / 2) {
+        const T v = side ? h[m].x : -h[m].y;
+        pr[p] = v;
+        if (p == 1) qr[(long)j * g.P] = ring_c * v;
+        if (p == n) qr[(long)j * g.P + n + 1] = ring_c * v;
+        if (j == 1) qr[p] = ring_c * v;
+        if (j == m_) qr[(long)(m_ + 1) * g.P + p] = ring_c * v;
       }
     }
     if (t == 0) {
-      buf[lds_slot(0)] = mkc<T>(T(0), T(0));
-      buf[lds_slot(L / 2)] = mkc<T>(T(0), T(0));
+      pr[0] = T(0);
+      pr[n + 1] = T(0);
     }
-    __syncthreads();
-    fft_lds<T, L, NT>(buf, tw, t);
-#pragma unroll
-    for (int s = 0; s < PPT; s++) {
-      const int x = t + s * NT;
-      if (x < n) {
-        cx<T> c = buf[lds_slot(x + 1)];
-        const T v0 = -c.y, v1 = c.x;
-#pragma unroll
-        for (int l = 0; l < NLC; l++) {
-          if (l < nl) {
-            acc[l][s] += cm2l[l * nl + k0] * v0;
-            if (has1) acc[l][s] += cm2l[l * nl + k1] * v1;
-          }
-        }
-      }
-    }
-    __syncthreads();
-  }
-  // psi row j (interior + zero wall columns) and the q ring
-#pragma unroll
-  for (int l = 0; l < NLC; l++) {
-    if (l < nl) {
-      T *pr = psi + l * g.LS + (long)j * g.P;
-      T *qr = q + l * g.LS;
-#pragma unroll
-      for (int s = 0; s < PPT; s++) {
-        const int x = t + s * NT;
-        if (x < n) {
-          const T v = acc[l][s];
-          pr[x + 1] = v;
-          if (x == 0) qr[(long)j * g.P] = ring_c * v;
-          if (x == n - 1) qr[(long)j * g.P + n + 1] = ring_c * v;
-          if (j == 1) qr[x + 1] = ring_c * v;
-          if (j == m) qr[(long)(m + 1) * g.P + x + 1] = ring_c * v;
-        }
-      }
+    if (j == 1 || j == m_) {
+      const long rr = (long)(j == 1 ? 0 : m_ + 1) * g.P;
+      T *pw = psi + l * g.LS + rr;
+      for (int x = t; x < n + 2; x += NT) pw[x] = T(0);
       if (t == 0) {
-        pr[0] = T(0);
-        pr[n + 1] = T(0);
-      }
-      // psi wall rows and the four q corners
-      if (j == 1 || j == m) {
-        T *pw = psi + l * g.LS + (long)(j == 1 ? 0 : m + 1) * g.P;
-        for (int x = t; x < n + 2; x += NT) pw[x] = T(0);
-        if (t == 0) {
-          const long rr = (long)(j == 1 ? 0 : m + 1) * g.P;
-          qr[rr] = T(0);
-          qr[rr + n + 1] = T(0);
-        }
+        qr[rr] = T(0);
+        qr[rr + n + 1] = T(0);
       }
     }
   }
@@ -335,9 +334,27 @@ k_p4(Geo g, const T *__restrict__ psi, const T *__restrict__ q, T *__restrict__ 
   __shared__ T sT[TY + 2][TX + 2];
 
   const int n = g.n, m = g.m, nl = g.nl;
-  const int k = blockIdx.z;
-  const int x0 = 1 + blockIdx.x * TX, y0 = 1 + blockIdx.y * TY;  // first interior point of the tile
+  // Tiles are taken in raster order per XCD: workgroup ids are dealt to the
+  // 8 XCDs round-robin, so each XCD gets one contiguous run of tiles and the
+  // halo rows shared by vertically adjacent tiles stay in its L2.
+  const int ntx = gridDim.x, nty = gridDim.y;
+  const int tile = xcd_remap(blockIdx.x + ntx * (blockIdx.y + nty * blockIdx.z), ntx * nty * gridDim.z);
+  const int k = tile / (ntx * nty);
+  const int tyi = (tile / ntx) % nty, txi = tile % ntx;
+  const int x0 = 1 + txi * TX, y0 = 1 + tyi * TY;  // first interior point of the tile
   const int tid = threadIdx.x;
+  // AB3 history and compensation of this thread's outputs, fetched before the tiles
+  const int ptx = tid % TX, pty0 = tid / TX;
+  T hf1[TY / 4], hf2[TY / 4], hcp[TY / 4];
+#pragma unroll
+  for (int r = 0; r < TY / 4; r++) {
+    const int gx = x0 + ptx, gy = y0 + pty0 + 4 * r;
+    const bool ok = cf.update && gx <= n && gy <= m;
+    const long o = k * g.LS + (long)gy * g.P + gx;
+    hf1[r] = ok ? f1[o] : T(0);
+    hf2[r] = ok ? f2[o] : T(0);
+    hcp[r] = (ok && KAHAN) ? comp[o] : T(0);
+  }
   const T *P = psi + k * g.LS;
   const T *Q = q + k * g.LS;
   const bool bottom = (k == nl - 1);
@@ -499,9 +516,9 @@ k_p4(Geo g, const T *__restrict__ psi, const T *__restrict__ q, T *__restrict__ 
     fnew[o] = f;
     if (!cf.update) continue;
     const T qo = sQ[qy][qx];
-    const T inc = c0 * f + c1 * f1[o] + c2 * f2[o];
+    const T inc = c0 * f + c1 * hf1[r] + c2 * hf2[r];
     if constexpr (KAHAN) {
-      const T cc = comp[o];
+      const T cc = hcp[r];
       const T y = inc - cc;
       const T tq = qo + y;
       comp[o] = (tq - qo) - y;

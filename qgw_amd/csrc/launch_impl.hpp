@@ -17,16 +17,12 @@ using T = QGW_T;
   X(24) X(48) X(96) X(192) X(384) X(768) X(1536) X(3072) X(6144)
 
 template <> int launch_p1_t<T>(const LaunchArgs &a) {
-  const dim3 grid(a.g.m);
+  const dim3 grid(a.g.m * ((a.g.nl + 1) / 2));
   switch (a.Lx) {
-#define QGW_CASE(L)                                                                                  \
-  case L:                                                                                            \
-    if (a.g.nl <= 4)                                                                                 \
-      hipLaunchKernelGGL((k_p1<T, L, 4>), grid, dim3(fft_threads(L)), 0, a.stream, a.g,             \
-                         (const T *)a.q, (T *)a.what, (const T *)a.cl2m, (const cx<T> *)a.twx);    \
-    else                                                                                             \
-      hipLaunchKernelGGL((k_p1<T, L, kMaxLayers>), grid, dim3(fft_threads(L)), 0, a.stream, a.g,    \
-                         (const T *)a.q, (T *)a.what, (const T *)a.cl2m, (const cx<T> *)a.twx);    \
+#define QGW_CASE(L)                                                                                 \
+  case L:                                                                                           \
+    hipLaunchKernelGGL((k_p1<T, L>), grid, dim3(RegPlan<L>::NT), 0, a.stream, a.g, (const T *)a.q, \
+                       (T *)a.what, (const cx<T> *)a.twx);                                          \
     return 0;
     QGW_LENGTHS(QGW_CASE)
 #undef QGW_CASE
@@ -35,18 +31,12 @@ template <> int launch_p1_t<T>(const LaunchArgs &a) {
 }
 
 template <> int launch_p3_t<T>(const LaunchArgs &a) {
-  const dim3 grid(a.g.m);
+  const dim3 grid(a.g.m * ((a.g.nl + 1) / 2));
   switch (a.Lx) {
-#define QGW_CASE(L)                                                                                   \
-  case L:                                                                                             \
-    if (a.g.nl <= 4)                                                                                  \
-      hipLaunchKernelGGL((k_p3<T, L, 4>), grid, dim3(fft_threads(L)), 0, a.stream, a.g,              \
-                         (const T *)a.what, (T *)a.psi, (T *)a.q, (const T *)a.cm2l, (T)a.ring_c,    \
-                         (const cx<T> *)a.twx);                                                       \
-    else                                                                                              \
-      hipLaunchKernelGGL((k_p3<T, L, kMaxLayers>), grid, dim3(fft_threads(L)), 0, a.stream, a.g,     \
-                         (const T *)a.what, (T *)a.psi, (T *)a.q, (const T *)a.cm2l, (T)a.ring_c,    \
-                         (const cx<T> *)a.twx);                                                       \
+#define QGW_CASE(L)                                                                                    \
+  case L:                                                                                              \
+    hipLaunchKernelGGL((k_p3<T, L>), grid, dim3(RegPlan<L>::NT), 0, a.stream, a.g, (const T *)a.what, \
+                       (T *)a.psi, (T *)a.q, (T)a.ring_c, (const cx<T> *)a.twx);                       \
     return 0;
     QGW_LENGTHS(QGW_CASE)
 #undef QGW_CASE
@@ -60,7 +50,7 @@ template <> int launch_p2_t<T>(const LaunchArgs &a) {
   switch (a.Ly) {
 #define QGW_CASE(L)                                                                               \
   case L:                                                                                         \
-    hipLaunchKernelGGL((k_p2<T, L>), grid, dim3(fft_threads(L)), 0, a.stream, a.g, (T *)a.whatT, \
+    hipLaunchKernelGGL((k_p2<T, L>), grid, dim3(RegPlan<L>::NT), 0, a.stream, a.g, (T *)a.whatT, \
                        a.La2, a.Kb2, a.iRd2, a.norm, (const cx<T> *)a.twy);                       \
     return 0;
     QGW_LENGTHS(QGW_CASE)
@@ -69,21 +59,38 @@ template <> int launch_p2_t<T>(const LaunchArgs &a) {
   return QGW_ERR_ARG;
 }
 
-constexpr int kTile = (sizeof(T) == 8) ? 32 : 64;
+// Transpose tiles: 32 columns (contiguous in the source) x kTR rows
+// (contiguous in the destination): 128-byte segments on both sides.
+constexpr int kTC = 32;
+constexpr int kTR = (sizeof(T) == 8) ? 16 : 32;
 
-template <> int launch_t1_t<T>(const LaunchArgs &a) {
-  // what [nl][m][Pn] -> whatT [nl][n][Pm]
-  const dim3 grid((a.g.n + kTile - 1) / kTile, (a.g.m + kTile - 1) / kTile, a.g.nl);
-  hipLaunchKernelGGL((k_transpose<T, kTile>), grid, dim3(256), 0, a.stream, (const T *)a.what, (T *)a.whatT,
-                     a.g.m, a.g.n, a.g.Pn, a.g.WS, a.g.Pm, a.g.WTS);
+template <int NLC>
+static void tproj_go(const LaunchArgs &a, const void *src, void *dst, const void *M, int R, int C, int Ps, long Ss,
+                     int Pd, long Sd) {
+  const dim3 grid((C + kTC - 1) / kTC, (R + kTR - 1) / kTR);
+  hipLaunchKernelGGL((k_tproj<T, kTC, kTR, NLC>), grid, dim3(256), 0, a.stream, (const T *)src, (T *)dst,
+                     (const T *)M, a.g.nl, R, C, Ps, Ss, Pd, Sd);
+}
+
+static int tproj(const LaunchArgs &a, const void *src, void *dst, const void *M, int R, int C, int Ps, long Ss,
+                 int Pd, long Sd) {
+  if (a.g.nl <= 4)
+    tproj_go<4>(a, src, dst, M, R, C, Ps, Ss, Pd, Sd);
+  else if (a.g.nl <= kMaxLayers)
+    tproj_go<kMaxLayers>(a, src, dst, M, R, C, Ps, Ss, Pd, Sd);
+  else
+    return QGW_ERR_ARG;
   return 0;
 }
 
+template <> int launch_t1_t<T>(const LaunchArgs &a) {
+  // what (layers) [l][j][a] -> whatT (modes) [k][a][j], w_k = sum_l cl2m[k,l] w_l
+  return tproj(a, a.what, a.whatT, a.cl2m, a.g.m, a.g.n, a.g.Pn, a.g.WS, a.g.Pm, a.g.WTS);
+}
+
 template <> int launch_t2_t<T>(const LaunchArgs &a) {
-  const dim3 grid((a.g.m + kTile - 1) / kTile, (a.g.n + kTile - 1) / kTile, a.g.nl);
-  hipLaunchKernelGGL((k_transpose<T, kTile>), grid, dim3(256), 0, a.stream, (const T *)a.whatT, (T *)a.what,
-                     a.g.n, a.g.m, a.g.Pm, a.g.WTS, a.g.Pn, a.g.WS);
-  return 0;
+  // whatT (modes) [k][a][j] -> what (layers) [l][j][a], w_l = sum_k cm2l[l,k] w_k
+  return tproj(a, a.whatT, a.what, a.cm2l, a.g.n, a.g.m, a.g.Pm, a.g.WTS, a.g.Pn, a.g.WS);
 }
 
 constexpr int kTX = 64, kTY = 16;
