@@ -189,8 +189,12 @@ k_p2(Geo g, T *__restrict__ whatT, const double *__restrict__ La2, const double 
     h[m] = mkc<T>(a, b);
   }
   fft_home<T, L>(h, buf, tw, t);
-  const double d0 = La2[a0] + 0.0, d1 = La2[a1] + 0.0;
+  const double d0 = La2[a0], d1 = La2[a1];
   const double ir0 = iRd2[k0], ir1 = iRd2[k1];
+  // FP32 forms: the denominator in FP32 except for the lowest barotropic
+  // wavenumbers (mode 0, a, b <= 16), where it is smallest and psi largest
+  const bool lo0 = (k0 == 0 && a0 <= 16), lo1 = (k1 == 0 && a1 <= 16);
+  const T d0f = (T)(d0 + ir0), d1f = (T)(d1 + ir1), normf = (T)norm;
 #pragma unroll
   for (int m = 0; m < E; m++) {
     const int p = t + m * NT;
@@ -199,15 +203,14 @@ k_p2(Geo g, T *__restrict__ whatT, const double *__restrict__ La2, const double 
     } else {
       const int b = (p < L / 2) ? p : L - p;
       const double kb = Kb2[b];
-      const double den0 = -(d0 + kb + ir0) * norm;
-      const double den1 = -(d1 + kb + ir1) * norm;
       T f0, f1;
       if constexpr (sizeof(T) == 8) {
-        f0 = (T)(1.0 / den0);
-        f1 = (T)(1.0 / den1);
+        f0 = (T)(1.0 / (-(d0 + kb + ir0) * norm));
+        f1 = (T)(1.0 / (-(d1 + kb + ir1) * norm));
       } else {
-        f0 = T(1) / (T)den0;
-        f1 = T(1) / (T)den1;
+        const T kbf = (T)kb;
+        f0 = (lo0 && b <= 16) ? (T)(1.0 / (-(d0 + kb + ir0) * norm)) : T(1) / (-(d0f + kbf) * normf);
+        f1 = (lo1 && b <= 16) ? (T)(1.0 / (-(d1 + kb + ir1) * norm)) : T(1) / (-(d1f + kbf) * normf);
       }
       h[m] = mkc<T>(-h[m].y * f0, h[m].x * f1);
     }
@@ -296,15 +299,7 @@ k_p3(Geo g, const T *__restrict__ what, T *__restrict__ psi, T *__restrict__ q, 
 }
 
 // ===========================================================================
-// P4: tendency and AB3 update on a TX x TY tile of one layer.
-//   f = -J(psi,q) - beta psi_x + nu lap q + nu_hyper lap^NLAP psi
-//   layer 0:    += wind(y_j)
-//   layer nl-1: += -hEkb f0/(2 dh) lap psi - J(psi, topo) f0/dh
-//   q_next = q + c0 f + c1 f1 + c2 f2 (Kahan-compensated when KAHAN)
-// NLAP = n_hyper/2 + 1 nested 5-point Laplacians (calc_diff,
-// src/dynamics.h:13-44), the intermediate ones carrying the wall value
-// 2 bc_fac/Delta^2 * adjacent psi on the ring; NLAP = 0 skips the term
-// (nu_hyper = 0).
+// Scalars of the tendency/AB3 kernel (P4).
 // ===========================================================================
 struct StepCoef {
   double c0, c1, c2;     // AB3 coefficients
@@ -315,164 +310,140 @@ struct StepCoef {
   int update;            // 0: write the tendency only (rhs), 1: also the AB3 update
 };
 
-template <int TX, int TY, int H> struct P4Tile {
+// Visit every point (ly, lx) of a (TX + 2R) x HH region with the 4 x TX
+// threads of a P4 workgroup: the TX middle columns one column per thread, the
+// 2R side columns as one flattened pass, so no pass runs mostly masked.
+template <int TX, int R, int HH, typename F>
+__device__ __forceinline__ void p4_region(int tx, int ty, F &&f) {
+#pragma unroll
+  for (int ly = ty; ly < HH; ly += 4) f(ly, R + tx);
+  if constexpr (R > 0) {
+    const int tid = ty * TX + tx;
+#pragma unroll
+    for (int e = tid; e < 2 * R * HH; e += 4 * TX) {
+      const int ly = e / (2 * R), c = e % (2 * R);
+      f(ly, c < R ? c : TX + c);
+    }
+  }
+}
+
+template <typename T, int TX, int TY, int NLAP> struct P4Smem {
+  static constexpr int H = (NLAP > 1) ? NLAP : 1;  // psi halo
   static constexpr int PW = TX + 2 * H, PH = TY + 2 * H;
+  static constexpr int AW = (NLAP > 1) ? PW - 2 : 1, AH = (NLAP > 1) ? PH - 2 : 1;
+  static constexpr int BW = (NLAP > 2) ? PW - 4 : 1, BH = (NLAP > 2) ? PH - 4 : 1;
+  T P[PH][PW];
+  T A[AH][AW];
+  T B[BH][BW];
+  T Q[TY + 2][TX + 2];
 };
 
-template <typename T, int NLAP, int KAHAN, int TX, int TY>
-__global__ void __launch_bounds__(TX * 4)
-k_p4(Geo g, const T *__restrict__ psi, const T *__restrict__ q, T *__restrict__ qn, T *__restrict__ fnew,
-     const T *__restrict__ f1, const T *__restrict__ f2, T *__restrict__ comp, const T *__restrict__ topo,
-     const T *__restrict__ wind, StepCoef cf) {
-  constexpr int H = (NLAP > 1) ? NLAP : 1;  // psi halo
-  constexpr int PW = TX + 2 * H, PH = TY + 2 * H;
-  constexpr int NTH = TX * 4;
-  __shared__ T sP[PH][PW];
-  __shared__ T sA[(NLAP > 1) ? (PH - 2) : 1][(NLAP > 1) ? (PW - 2) : 1];
-  __shared__ T sB[(NLAP > 2) ? (PH - 4) : 1][(NLAP > 2) ? (PW - 4) : 1];
-  __shared__ T sQ[TY + 2][TX + 2];
-  __shared__ T sT[TY + 2][TX + 2];
+// Laplacian level of radius RD (tile interior +- RD) from the level below
+// (radius RD+1, row width SW) into dst (row width DW).  Interior tiles (FAST)
+// need no wall handling; elsewhere a wall-ring point takes the value
+// 2 bc_fac/Delta^2 * adjacent psi (the ring set by adjust_bc(...,1,psi) after
+// every Laplacian of calc_diff, src/dynamics.h:25-43) and points beyond the
+// ring are never used.
+template <typename T, int TX, int TY, int H, bool FAST, int RD, int SW, int DW>
+__device__ __forceinline__ void p4_level(const T *src, T *dst, const T *sP, int x0, int y0, int n, int m, T inv_d2,
+                                         T ring_c, int tx, int ty) {
+  constexpr int HH = TY + 2 * RD, PW = TX + 2 * H;
+  p4_region<TX, RD, HH>(tx, ty, [&](int ly, int lx) {
+    const T *c = src + (ly + 1) * SW + (lx + 1);
+    T v;
+    if constexpr (FAST) {
+      v = (c[1] + c[-1] + c[SW] + c[-SW] - T(4) * c[0]) * inv_d2;
+    } else {
+      const int gx = x0 - RD + lx, gy = y0 - RD + ly;
+      if (gx >= 1 && gx <= n && gy >= 1 && gy <= m) {
+        v = (c[1] + c[-1] + c[SW] + c[-SW] - T(4) * c[0]) * inv_d2;
+      } else if (gx >= 0 && gx <= n + 1 && gy >= 0 && gy <= m + 1) {
+        const T *pp = sP + (ly + H - RD) * PW + (lx + H - RD);  // psi at this point
+        if (gx == 0)
+          v = (gy == 0 || gy == m + 1) ? T(0) : ring_c * pp[1];
+        else if (gx == n + 1)
+          v = (gy == 0 || gy == m + 1) ? T(0) : ring_c * pp[-1];
+        else if (gy == 0)
+          v = ring_c * pp[PW];
+        else
+          v = ring_c * pp[-PW];
+      } else {
+        v = T(0);
+      }
+    }
+    dst[ly * DW + lx] = v;
+  });
+}
 
-  const int n = g.n, m = g.m, nl = g.nl;
-  // Tiles are taken in raster order per XCD: workgroup ids are dealt to the
-  // 8 XCDs round-robin, so each XCD gets one contiguous run of tiles and the
-  // halo rows shared by vertically adjacent tiles stay in its L2.
-  const int ntx = gridDim.x, nty = gridDim.y;
-  const int tile = xcd_remap(blockIdx.x + ntx * (blockIdx.y + nty * blockIdx.z), ntx * nty * gridDim.z);
-  const int k = tile / (ntx * nty);
-  const int tyi = (tile / ntx) % nty, txi = tile % ntx;
-  const int x0 = 1 + txi * TX, y0 = 1 + tyi * TY;  // first interior point of the tile
-  const int tid = threadIdx.x;
-  // AB3 history and compensation of this thread's outputs, fetched before the tiles
-  const int ptx = tid % TX, pty0 = tid / TX;
-  T hf1[TY / 4], hf2[TY / 4], hcp[TY / 4];
-#pragma unroll
-  for (int r = 0; r < TY / 4; r++) {
-    const int gx = x0 + ptx, gy = y0 + pty0 + 4 * r;
-    const bool ok = cf.update && gx <= n && gy <= m;
-    const long o = k * g.LS + (long)gy * g.P + gx;
-    hf1[r] = ok ? f1[o] : T(0);
-    hf2[r] = ok ? f2[o] : T(0);
-    hcp[r] = (ok && KAHAN) ? comp[o] : T(0);
-  }
-  const T *P = psi + k * g.LS;
-  const T *Q = q + k * g.LS;
-  const bool bottom = (k == nl - 1);
-
-  // ---- load psi (halo H) and q (halo 1), zero outside the storage grid
-  for (int e = tid; e < PW * PH; e += NTH) {
-    const int ly = e / PW, lx = e % PW;
+template <typename T, int NLAP, int KAHAN, int TX, int TY, bool FAST>
+__device__ __forceinline__ void p4_body(P4Smem<T, TX, TY, NLAP> &sm, const Geo &g, const T *__restrict__ P,
+                                        const T *__restrict__ Q, const T *__restrict__ topo, int k, int x0, int y0,
+                                        int tx, int ty, const T (&hf1)[TY / 4], const T (&hf2)[TY / 4],
+                                        const T (&hcp)[TY / 4], T *__restrict__ qn, T *__restrict__ fnew,
+                                        T *__restrict__ comp, const T *__restrict__ wind, const StepCoef &cf) {
+  using SM = P4Smem<T, TX, TY, NLAP>;
+  constexpr int H = SM::H, PW = SM::PW, PH = SM::PH;
+  (void)PW;
+  const int n = g.n, m = g.m;
+  const bool bottom = (k == g.nl - 1);
+  // ---- psi (halo H) and q (halo 1) tiles; zero outside the storage grid
+  p4_region<TX, H, PH>(tx, ty, [&](int ly, int lx) {
     const int gx = x0 - H + lx, gy = y0 - H + ly;
-    sP[ly][lx] = (gx >= 0 && gx <= n + 1 && gy >= 0 && gy <= m + 1) ? P[(long)gy * g.P + gx] : T(0);
-  }
-  for (int e = tid; e < (TX + 2) * (TY + 2); e += NTH) {
-    const int ly = e / (TX + 2), lx = e % (TX + 2);
+    if constexpr (FAST)
+      sm.P[ly][lx] = P[gy * g.P + gx];
+    else
+      sm.P[ly][lx] = (gx >= 0 && gx <= n + 1 && gy >= 0 && gy <= m + 1) ? P[gy * g.P + gx] : T(0);
+  });
+  p4_region<TX, 1, TY + 2>(tx, ty, [&](int ly, int lx) {
     const int gx = x0 - 1 + lx, gy = y0 - 1 + ly;
-    const bool in = (gx <= n + 1 && gy <= m + 1);
-    sQ[ly][lx] = in ? Q[(long)gy * g.P + gx] : T(0);
-    if (bottom) sT[ly][lx] = in ? topo[(long)gy * g.P + gx] : T(0);
-  }
+    const bool in = FAST || (gx <= n + 1 && gy <= m + 1);
+    sm.Q[ly][lx] = in ? Q[gy * g.P + gx] : T(0);
+  });
   __syncthreads();
-
   const T inv_d2 = (T)cf.inv_d2;
   const T ring_c = (T)cf.ring_c;
-  // value of a Laplacian level at storage point (gx,gy): interior -> computed,
-  // ring -> 2 bc_fac/Delta^2 * adjacent psi (psi from sP), outside -> 0
-  auto ring_val = [&](int gx, int gy, int lx, int ly) -> T {
-    // (lx,ly) index sP at the same point
-    if (gx == 0) return (gy == 0 || gy == m + 1) ? T(0) : ring_c * sP[ly][lx + 1];
-    if (gx == n + 1) return (gy == 0 || gy == m + 1) ? T(0) : ring_c * sP[ly][lx - 1];
-    if (gy == 0) return ring_c * sP[ly + 1][lx];
-    return ring_c * sP[ly - 1][lx];  // gy == m+1
-  };
-
-  // ---- nested Laplacians of psi down to radius 1 (kept in sA / sB)
-  if constexpr (NLAP > 1) {
-    // level 1 on radius H-1 -> sA
-    for (int e = tid; e < (PW - 2) * (PH - 2); e += NTH) {
-      const int ly = e / (PW - 2), lx = e % (PW - 2);
-      const int gx = x0 - H + 1 + lx, gy = y0 - H + 1 + ly;
-      T v;
-      if (gx >= 1 && gx <= n && gy >= 1 && gy <= m) {
-        const int sy = ly + 1, sx = lx + 1;
-        v = (sP[sy][sx + 1] + sP[sy][sx - 1] + sP[sy + 1][sx] + sP[sy - 1][sx] - T(4) * sP[sy][sx]) * inv_d2;
-      } else if (gx >= 0 && gx <= n + 1 && gy >= 0 && gy <= m + 1) {
-        v = ring_val(gx, gy, lx + 1, ly + 1);
-      } else {
-        v = T(0);
-      }
-      sA[ly][lx] = v;
-    }
+  const T *sPp = &sm.P[0][0];
+  // ---- nested Laplacians of psi down to radius 1 (calc_diff, src/dynamics.h:25-43)
+  if constexpr (NLAP >= 2) {
+    p4_level<T, TX, TY, H, FAST, H - 1, PW, SM::AW>(sPp, &sm.A[0][0], sPp, x0, y0, n, m, inv_d2, ring_c, tx, ty);
     __syncthreads();
   }
-  if constexpr (NLAP > 2) {
-    // levels 2..NLAP-1 alternate between sA (radius H-1) and sB (radius H-2) in place
-    // level 2 -> sB (radius H-2)
-    for (int e = tid; e < (PW - 4) * (PH - 4); e += NTH) {
-      const int ly = e / (PW - 4), lx = e % (PW - 4);
-      const int gx = x0 - H + 2 + lx, gy = y0 - H + 2 + ly;
-      T v;
-      if (gx >= 1 && gx <= n && gy >= 1 && gy <= m) {
-        const int sy = ly + 1, sx = lx + 1;
-        v = (sA[sy][sx + 1] + sA[sy][sx - 1] + sA[sy + 1][sx] + sA[sy - 1][sx] - T(4) * sA[sy][sx]) * inv_d2;
-      } else if (gx >= 0 && gx <= n + 1 && gy >= 0 && gy <= m + 1) {
-        v = ring_val(gx, gy, lx + 2, ly + 2);
-      } else {
-        v = T(0);
-      }
-      sB[ly][lx] = v;
-    }
+  if constexpr (NLAP >= 3) {
+    p4_level<T, TX, TY, H, FAST, H - 2, SM::AW, SM::BW>(&sm.A[0][0], &sm.B[0][0], sPp, x0, y0, n, m, inv_d2,
+                                                         ring_c, tx, ty);
     __syncthreads();
-    // further levels (NLAP >= 4): radius shrinks by one each time; reuse sA then sB
-    int rad = H - 2;  // radius held in sB
-    for (int lev = 3; lev < NLAP; lev++) {
-      const int nr = rad - 1;
-      const int wid = TX + 2 * nr, hei = TY + 2 * nr;
-      // source is sB (radius rad, width TX+2rad) when lev is odd, sA otherwise
-      const bool fromB = (lev % 2) == 1;
-      for (int e = tid; e < wid * hei; e += NTH) {
-        const int ly = e / wid, lx = e % wid;
-        const int gx = x0 - nr + lx, gy = y0 - nr + ly;
-        T v;
-        if (gx >= 1 && gx <= n && gy >= 1 && gy <= m) {
-          const int sy = ly + 1, sx = lx + 1;
-          if (fromB)
-            v = (sB[sy][sx + 1] + sB[sy][sx - 1] + sB[sy + 1][sx] + sB[sy - 1][sx] - T(4) * sB[sy][sx]) * inv_d2;
-          else
-            v = (sA[sy][sx + 1] + sA[sy][sx - 1] + sA[sy + 1][sx] + sA[sy - 1][sx] - T(4) * sA[sy][sx]) * inv_d2;
-        } else if (gx >= 0 && gx <= n + 1 && gy >= 0 && gy <= m + 1) {
-          v = ring_val(gx, gy, lx + H - nr, ly + H - nr);
-        } else {
-          v = T(0);
-        }
-        // source and destination differ, so no barrier is needed inside the loop
-        if (fromB)
-          sA[ly][lx] = v;
-        else
-          sB[ly][lx] = v;
-      }
-      __syncthreads();
-      rad = nr;
-    }
   }
+  if constexpr (NLAP >= 4) {
+    p4_level<T, TX, TY, H, FAST, H - 3, SM::BW, SM::AW>(&sm.B[0][0], &sm.A[0][0], sPp, x0, y0, n, m, inv_d2,
+                                                         ring_c, tx, ty);
+    __syncthreads();
+  }
+  if constexpr (NLAP >= 5) {
+    p4_level<T, TX, TY, H, FAST, H - 4, SM::AW, SM::BW>(&sm.A[0][0], &sm.B[0][0], sPp, x0, y0, n, m, inv_d2,
+                                                         ring_c, tx, ty);
+    __syncthreads();
+  }
+  // the radius-1 level feeding the last Laplacian: A for even NLAP, B for odd
+  constexpr bool lastA = (NLAP % 2) == 0;
+  constexpr int LW = lastA ? SM::AW : SM::BW;
+  const T *last = lastA ? &sm.A[0][0] : &sm.B[0][0];
 
-  // ---- outputs: each thread owns column tx and rows ty, ty+4, ...
-  const int tx = tid % TX;
-  const int ty0 = tid / TX;
-  const int gx = x0 + tx;
   const T beta = (T)cf.beta, nu = (T)cf.nu, nuh = (T)cf.nu_hyper;
   const T i12 = (T)cf.inv_12d2, i2d = (T)cf.inv_2d;
   const T c0 = (T)cf.c0, c1 = (T)cf.c1, c2 = (T)cf.c2;
+  const T drag = (T)cf.drag, tfac = (T)cf.topo_fac;
+  const bool use_nu = cf.nu != 0.0;
+  const int gx = x0 + tx;
 #pragma unroll
   for (int r = 0; r < TY / 4; r++) {
-    const int ty = ty0 + 4 * r;
-    const int gy = y0 + ty;
-    if (gx > n || gy > m) continue;
-    const int px = tx + H, py = ty + H;  // sP index
-    const int qx = tx + 1, qy = ty + 1;  // sQ index
-#define PS(a, b) sP[py + (b)][px + (a)]
-#define QS(a, b) sQ[qy + (b)][qx + (a)]
-#define TS(a, b) sT[qy + (b)][qx + (a)]
+    const int tyr = ty + 4 * r;
+    const int gy = y0 + tyr;
+    if (!FAST && (gx > n || gy > m)) continue;
+    const int px = tx + H, py = tyr + H;  // sP index
+    const int qx = tx + 1, qy = tyr + 1;  // sQ index
+#define PS(a, b) sm.P[py + (b)][px + (a)]
+#define QS(a, b) sm.Q[qy + (b)][qx + (a)]
+#define TS(a, b) tp[(b)*g.P + (a)]
     // Arakawa Jacobian (src/domain.h:17-27)
     const T jpq = ((PS(1, 0) - PS(-1, 0)) * (QS(0, 1) - QS(0, -1)) + (PS(0, -1) - PS(0, 1)) * (QS(1, 0) - QS(-1, 0)) +
                    PS(1, 0) * (QS(1, 1) - QS(1, -1)) - PS(-1, 0) * (QS(-1, 1) - QS(-1, -1)) -
@@ -481,45 +452,35 @@ k_p4(Geo g, const T *__restrict__ psi, const T *__restrict__ q, T *__restrict__ 
                    QS(1, 0) * (PS(1, 1) - PS(1, -1)) + QS(-1, 0) * (PS(-1, 1) - PS(-1, -1))) *
                   i12;
     T f = -jpq - beta * (PS(1, 0) - PS(-1, 0)) * i2d;
-    if (cf.nu != 0.0)
-      f += nu * ((QS(1, 0) + QS(-1, 0) + QS(0, 1) + QS(0, -1) - T(4) * QS(0, 0)) * inv_d2);
-    T lap1 = (PS(1, 0) + PS(-1, 0) + PS(0, 1) + PS(0, -1) - T(4) * PS(0, 0)) * inv_d2;
+    if (use_nu) f += nu * ((QS(1, 0) + QS(-1, 0) + QS(0, 1) + QS(0, -1) - T(4) * QS(0, 0)) * inv_d2);
+    const T lap1 = (PS(1, 0) + PS(-1, 0) + PS(0, 1) + PS(0, -1) - T(4) * PS(0, 0)) * inv_d2;
     if constexpr (NLAP == 1) {
       f += nuh * lap1;
-    } else if constexpr (NLAP == 2) {
-      const int ax = tx + H - 1, ay = ty + H - 1;
-      f += nuh * ((sA[ay][ax + 1] + sA[ay][ax - 1] + sA[ay + 1][ax] + sA[ay - 1][ax] - T(4) * sA[ay][ax]) * inv_d2);
-    } else if constexpr (NLAP > 2) {
-      // the last stored level is in sB when NLAP is odd, in sA when even
-      if constexpr ((NLAP % 2) == 1) {
-        const int bx = tx + 1, by = ty + 1;
-        f += nuh * ((sB[by][bx + 1] + sB[by][bx - 1] + sB[by + 1][bx] + sB[by - 1][bx] - T(4) * sB[by][bx]) * inv_d2);
-      } else {
-        const int ax = tx + 1, ay = ty + 1;
-        f += nuh * ((sA[ay][ax + 1] + sA[ay][ax - 1] + sA[ay + 1][ax] + sA[ay - 1][ax] - T(4) * sA[ay][ax]) * inv_d2);
-      }
+    } else if constexpr (NLAP >= 2) {
+      const T *c = last + (tyr + 1) * LW + (tx + 1);
+      f += nuh * ((c[1] + c[-1] + c[LW] + c[-LW] - T(4) * c[0]) * inv_d2);
     }
     if (k == 0) f += wind[gy];
     if (bottom) {
+      const T *tp = topo + gy * g.P + gx;  // bottom topography, read through the cache
       const T jt = ((PS(1, 0) - PS(-1, 0)) * (TS(0, 1) - TS(0, -1)) + (PS(0, -1) - PS(0, 1)) * (TS(1, 0) - TS(-1, 0)) +
                     PS(1, 0) * (TS(1, 1) - TS(1, -1)) - PS(-1, 0) * (TS(-1, 1) - TS(-1, -1)) -
                     PS(0, 1) * (TS(1, 1) - TS(-1, 1)) + PS(0, -1) * (TS(1, -1) - TS(-1, -1)) +
                     TS(0, 1) * (PS(1, 1) - PS(-1, 1)) - TS(0, -1) * (PS(1, -1) - PS(-1, -1)) -
                     TS(1, 0) * (PS(1, 1) - PS(1, -1)) + TS(-1, 0) * (PS(-1, 1) - PS(-1, -1))) *
                    i12;
-      f += (T)cf.drag * lap1 - jt * (T)cf.topo_fac;
+      f += drag * lap1 - jt * tfac;
     }
+    const T qo = QS(0, 0);
 #undef PS
 #undef QS
 #undef TS
-    const long o = k * g.LS + (long)gy * g.P + gx;
+    const int o = gy * g.P + gx;  // offset inside layer k (the pointers carry the layer)
     fnew[o] = f;
     if (!cf.update) continue;
-    const T qo = sQ[qy][qx];
     const T inc = c0 * f + c1 * hf1[r] + c2 * hf2[r];
     if constexpr (KAHAN) {
-      const T cc = hcp[r];
-      const T y = inc - cc;
+      const T y = inc - hcp[r];
       const T tq = qo + y;
       comp[o] = (tq - qo) - y;
       qn[o] = tq;
@@ -527,11 +488,68 @@ k_p4(Geo g, const T *__restrict__ psi, const T *__restrict__ q, T *__restrict__ 
       qn[o] = qo + inc;
     }
     // carry q's wall ring into the next buffer (corners stay zero)
-    if (gy == 1) qn[k * g.LS + gx] = sQ[qy - 1][qx];
-    if (gy == m) qn[k * g.LS + (long)(m + 1) * g.P + gx] = sQ[qy + 1][qx];
-    if (gx == 1) qn[k * g.LS + (long)gy * g.P] = sQ[qy][qx - 1];
-    if (gx == n) qn[k * g.LS + (long)gy * g.P + n + 1] = sQ[qy][qx + 1];
+    if (gy == 1) qn[gx] = sm.Q[qy - 1][qx];
+    if (gy == m) qn[(m + 1) * g.P + gx] = sm.Q[qy + 1][qx];
+    if (gx == 1) qn[gy * g.P] = sm.Q[qy][qx - 1];
+    if (gx == n) qn[gy * g.P + n + 1] = sm.Q[qy][qx + 1];
   }
+}
+
+// ===========================================================================
+// P4: tendency and AB3 update on a TX x TY tile of one layer.
+//   f = -J(psi,q) - beta psi_x + nu lap q + nu_hyper lap^NLAP psi
+//   layer 0:    += wind(y_j)
+//   layer nl-1: += -hEkb f0/(2 dh) lap psi - J(psi, topo) f0/dh
+//   q_next = q + c0 f + c1 f1 + c2 f2 (Kahan-compensated when KAHAN)
+// (rhs, src/dynamics.h:47-73; timestep, src/timestep.h:135-141).  NLAP =
+// n_hyper/2 + 1 nested 5-point Laplacians (calc_diff); NLAP = 0 skips the
+// term (nu_hyper = 0).  Tiles whose psi halo lies inside the storage grid and
+// whose Laplacian levels lie inside the interior take a branch-free path.
+// ===========================================================================
+template <typename T, int NLAP, int KAHAN, int TX, int TY>
+__global__ void __launch_bounds__(TX * 4)
+k_p4(Geo g, const T *__restrict__ psi, const T *__restrict__ q, T *__restrict__ qn, T *__restrict__ fnew,
+     const T *__restrict__ f1, const T *__restrict__ f2, T *__restrict__ comp, const T *__restrict__ topo,
+     const T *__restrict__ wind, StepCoef cf) {
+  using SM = P4Smem<T, TX, TY, NLAP>;
+  constexpr int H = SM::H;
+  __shared__ SM sm;
+  const int n = g.n, m = g.m;
+  // Tiles are taken in raster order per XCD: workgroup ids are dealt to the
+  // 8 XCDs round-robin, so each XCD gets one contiguous run of tiles and the
+  // halo rows shared by vertically adjacent tiles stay in its L2.
+  const int ntx = gridDim.x, nty = gridDim.y;
+  const int tile = xcd_remap(blockIdx.x + ntx * (blockIdx.y + nty * blockIdx.z), ntx * nty * gridDim.z);
+  const int k = tile / (ntx * nty);
+  const int tyi = (tile / ntx) % nty, txi = tile % ntx;
+  const int x0 = 1 + txi * TX, y0 = 1 + tyi * TY;  // first interior point of the tile
+  const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
+  // AB3 history and compensation of this thread's outputs, fetched before the tiles
+  const long lo = k * g.LS;
+  f1 += lo;
+  f2 += lo;
+  qn += lo;
+  fnew += lo;
+  if (KAHAN) comp += lo;
+  T hf1[TY / 4], hf2[TY / 4], hcp[TY / 4];
+#pragma unroll
+  for (int r = 0; r < TY / 4; r++) {
+    const int gx = x0 + tx, gy = y0 + ty + 4 * r;
+    const bool ok = cf.update && gx <= n && gy <= m;
+    const int o = gy * g.P + gx;
+    hf1[r] = ok ? f1[o] : T(0);
+    hf2[r] = ok ? f2[o] : T(0);
+    hcp[r] = (ok && KAHAN) ? comp[o] : T(0);
+  }
+  const T *P = psi + lo;
+  const T *Q = q + lo;
+  const bool fast = (x0 - H >= 0) && (x0 + TX - 1 + H <= n + 1) && (y0 - H >= 0) && (y0 + TY - 1 + H <= m + 1);
+  if (fast)
+    p4_body<T, NLAP, KAHAN, TX, TY, true>(sm, g, P, Q, topo, k, x0, y0, tx, ty, hf1, hf2, hcp, qn, fnew, comp,
+                                          wind, cf);
+  else
+    p4_body<T, NLAP, KAHAN, TX, TY, false>(sm, g, P, Q, topo, k, x0, y0, tx, ty, hf1, hf2, hcp, qn, fnew, comp,
+                                           wind, cf);
 }
 
 // ===========================================================================

@@ -62,6 +62,7 @@ struct qgw_ctx {
   };
   std::vector<Rec> recs;
   std::vector<hipEvent_t> evpool;
+  std::vector<void *> allocs;  // every device allocation (field pointers may be offset or rotated)
   double prof_ms[QGW_NPASS];
   int prof_cnt[QGW_NPASS];
   char err[256];
@@ -88,9 +89,22 @@ int dalloc(qgw_ctx *c, void **ptr, size_t bytes) {
     fail_hip(c, e, "hipMalloc");
     return QGW_ERR_NOMEM;
   }
+  c->allocs.push_back(*ptr);
   e = hipMemset(*ptr, 0, bytes);
   if (e != hipSuccess) return fail_hip(c, e, "hipMemset");
   c->bytes += bytes;
+  return QGW_OK;
+}
+
+// A storage field (rows with the one-point wall ring): the pointer is offset
+// so that interior element i = 1 of every row starts a 256-byte line (the
+// pitch is a multiple of 256 bytes), which keeps the row passes and the
+// stencil tiles on whole cache lines.
+int dalloc_field(qgw_ctx *c, void **ptr, size_t bytes) {
+  void *raw = nullptr;
+  int st = dalloc(c, &raw, bytes + 256);
+  if (st) return st;
+  *ptr = (char *)raw + 256 - c->esz;
   return QGW_OK;
 }
 
@@ -181,10 +195,7 @@ int pitch_of(int w) { return (w + 63) / 64 * 64; }
 
 void release(qgw_ctx *c) {
   if (!c) return;
-  void *ptrs[] = {c->q[0], c->q[1], c->psi, c->fa[0], c->fa[1], c->fa[2], c->comp, c->topo, c->wind,
-                  c->what, c->whatT, c->cl2m, c->cm2l, c->twx, c->twy, c->iRd2_d, c->La2, c->Kb2,
-                  c->diag_part, c->diag_out, c->stage, c->cfl_max};
-  for (void *p : ptrs)
+  for (void *p : c->allocs)
     if (p) hipFree(p);
   for (auto &r : c->recs) {
     hipEventDestroy(r.a);
@@ -420,12 +431,12 @@ qgw_ctx *qgw_create(const qgw_params *p, int precision, int *status) {
   }
   const size_t field = c->esz * (size_t)g.LS * nl;
   int st = QGW_OK;
-  st = st ? st : dalloc(c, &c->q[0], field);
-  st = st ? st : dalloc(c, &c->q[1], field);
-  st = st ? st : dalloc(c, &c->psi, field);
-  for (int i = 0; i < 3; i++) st = st ? st : dalloc(c, &c->fa[i], field);
-  if (precision == QGW_FP32) st = st ? st : dalloc(c, &c->comp, field);
-  st = st ? st : dalloc(c, &c->topo, c->esz * (size_t)g.LS);
+  st = st ? st : dalloc_field(c, &c->q[0], field);
+  st = st ? st : dalloc_field(c, &c->q[1], field);
+  st = st ? st : dalloc_field(c, &c->psi, field);
+  for (int i = 0; i < 3; i++) st = st ? st : dalloc_field(c, &c->fa[i], field);
+  if (precision == QGW_FP32) st = st ? st : dalloc_field(c, &c->comp, field);
+  st = st ? st : dalloc_field(c, &c->topo, c->esz * (size_t)g.LS);
   st = st ? st : dalloc(c, &c->wind, c->esz * (size_t)(m + 2));
   st = st ? st : dalloc(c, &c->what, c->esz * (size_t)g.WS * nl);
   st = st ? st : dalloc(c, &c->whatT, c->esz * (size_t)g.WTS * nl);
@@ -543,6 +554,7 @@ static int ensure_stage(qgw_ctx *c) {
   size_t b = sizeof(double) * (size_t)(c->g.n + 2) * (c->g.m + 2) * c->p.nl;
   hipError_t e = hipMalloc((void **)&c->stage, b);
   if (e != hipSuccess) return fail_hip(c, e, "hipMalloc(stage)");
+  c->allocs.push_back(c->stage);
   c->bytes += b;
   return QGW_OK;
 }
