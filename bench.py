@@ -174,6 +174,15 @@ def main():
     total_alg = sum(words.values()) * s * work
     step_gbs = total_alg / (ms_step * 1e-3) / 1e9
 
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tpath):
+        prefix = "k_%s<%s" % (dom.lower(), "double" if head == "fp64" else "float")
+        if dom in ("T1", "T2"):
+            prefix = "k_tproj<%s" % ("double" if head == "fp64" else "float")
+        for kname, tv in json.load(open(tpath))["kernels"].items():
+            if kname.startswith(prefix):
+                traffic = tv["fetch_bytes"] + tv["write_bytes"]
     out = {
         "metric": "grid-pt*layer*steps/s at 2048^2x10, FP32 batched vs FP64 plain, <=1e-4 energy err",
         "value": value,
@@ -191,7 +200,9 @@ def main():
                    "nx": p.NX, "ny": p.NY, "nl": p.nl, "form": head + ("+kahan" if head == "fp32" else ""),
                    "interior_points": g.n * g.m},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "algorithmic_bytes": alg_bytes,
+                     "traffic_source": "rocprofv3 FETCH_SIZE + WRITE_SIZE, profiles/traffic.json"},
         "step_algorithmic_GBs": step_gbs,
         "step_roofline_frac": step_gbs / HBM_PEAK_GBS,
         "passes": pass_table,

@@ -29,9 +29,10 @@
  * scipy.linalg.eigh fixtures, the nl=2 closed form and the invariants;
  * invert_pv against single-sine-mode closed forms; the Jacobian against its
  * antisymmetry and exactness identities and a brute-force 12-term sum; AB3
- * coefficients against the sympy result of doc/timestep.org; adjust_timestep
- * (source restatement only) against hand-computed cases of the macro
- * expansions.
+ * coefficients against an independent sympy integration (the construction of
+ * doc/timestep.org); check_timestep against hand-expanded cases of the macro
+ * quirks; adjust_timestep against hand-computed CFL/doubling/snap cases;
+ * diagnostics against a sine-mode closed form.  No function is unpinned.
  */
 #ifndef QGW_ORACLE_H
 #define QGW_ORACLE_H

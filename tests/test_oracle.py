@@ -298,3 +298,43 @@ def test_energy_enstrophy_conserved_by_jacobian_alone():
     _, _, E1, Z1 = o.diagnostics()
     assert abs(E1 - E0) / abs(E0) < 1e-4
     assert abs(Z1 - Z0) / abs(Z0) < 1e-3
+
+
+def test_adjust_timestep_hand_computed():
+    # psi = A*i (uniform x-difference A, zero y-difference): u = A/Delta, so the
+    # CFL limit is cfl*Delta/(A/Delta); then doubling, the DT_max cap and the
+    # snap to t_out of src/timestep.h:93-109, all computed by hand below
+    p = Q.default_params(NX=8, NY=8, nl=1, Lx=8e3, dh=[1000.0], beta=1e-11, cfl=0.2)
+    o = O.Oracle(p)
+    A = 10.0
+    jj, ii = np.meshgrid(np.arange(9), np.arange(9), indexing="ij")
+    o.field(O.PSI)[...] = (A * ii)[None]
+    dt_lim = 0.2 * 1000.0 / (A / 1000.0)  # 2e4
+    assert o.get_time()["DT_max"] == pytest.approx(1 / (2 * 1e-11 * 8e3))
+    o.set_time(0.0, 1000.0, 1000.0, 1000.0, 0, 0.0)
+    assert o.adjust_timestep() == 2000.0           # doubled, below every limit
+    o.set_time(0.0, 15000.0, 1000.0, 1000.0, 0, 0.0)
+    assert o.adjust_timestep() == pytest.approx(dt_lim)  # doubled, then capped by the CFL limit
+    o.set_time(0.0, 1000.0, 1000.0, 1000.0, 0, 50000.0)
+    assert o.adjust_timestep() == 2000.0           # 25 steps land exactly on t_out
+    o.set_time(0.0, 1000.0, 1000.0, 1000.0, 0, 5100.0)
+    assert o.adjust_timestep() == pytest.approx(5100.0 / 3)  # n_step = 2 -> (t_out - t)/3
+    o.set_time(0.0, 1000.0, 1000.0, 1000.0, 0, 1500.0)
+    assert o.adjust_timestep() == 1500.0           # n_step = 0 -> t_out - t
+
+
+def test_diagnostics_closed_form_sine_mode():
+    NX, a, b, A = 16, 3, 5, 2.5
+    p = Q.default_params(NX=NX, NY=NX, nl=1, Lx=1e6, dh=[1000.0])
+    o = O.Oracle(p)
+    n = NX - 1
+    jj, ii = np.meshgrid(np.arange(n + 2), np.arange(n + 2), indexing="ij")
+    s = np.sin(np.pi * a * ii / (n + 1)) * np.sin(np.pi * b * jj / (n + 1))
+    kap2 = 7.0e-10
+    o.field(O.PSI)[...] = (A * s)[None]
+    o.field(O.Q)[...] = (-kap2 * A * s)[None]
+    El, Zl, E, Z = o.diagnostics()
+    w = (1e6 / NX) ** 2 / 1e6 ** 2
+    assert El[0] == pytest.approx(kap2 * A ** 2 * (n + 1) ** 2 / 4 * w, rel=1e-12)
+    assert Zl[0] == pytest.approx(kap2 ** 2 * A ** 2 * (n + 1) ** 2 / 4 * w, rel=1e-12)
+    assert E == pytest.approx(El[0] / 2, rel=1e-14) and Z == pytest.approx(Zl[0], rel=1e-14)
