@@ -55,8 +55,9 @@ def cpu_model():
     return platform.processor() or "unknown"
 
 
-def run_form(Q, p, modes, q0, prec, steps, warmup, prof_steps):
+def run_form(Q, p, modes, q0, prec, steps, warmup, prof_steps, tune=True):
     m = Q.Model(p, prec, modes)
+    tuned = m.autotune() if tune else None
     m.set_q(q0)
     m.run(warmup)
     m.synchronize()
@@ -67,7 +68,7 @@ def run_form(Q, p, modes, q0, prec, steps, warmup, prof_steps):
     m.profile(False)
     passes = m.profile_read()
     m.close()
-    return ms, passes
+    return ms, passes, tuned
 
 
 def accuracy(Q, p, modes, q0, nsteps):
@@ -125,6 +126,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--forms", default="fp64,fp32")
+    ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (default tile shapes)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the hot path is a single-GPU build (the ensemble and "
@@ -146,11 +148,14 @@ def main():
     g = Q.grid(p)[0]
 
     res = {}
+    tunes = {}
     for form in args.forms.split(","):
         prec = {"fp64": Q.FP64, "fp32": Q.FP32, "fp32plain": Q.FP32_PLAIN}[form]
         log("timing form", form)
-        ms, passes = run_form(Q, p, modes, q0, prec, args.steps, args.warmup, args.profile_steps)
+        ms, passes, tuned = run_form(Q, p, modes, q0, prec, args.steps, args.warmup, args.profile_steps,
+                                     not args.no_tune)
         res[form] = (ms, passes)
+        tunes[form] = tuned
         log("  %s: %.3f ms/step" % (form, ms / args.steps), {k: round(v[0] / max(v[1], 1), 4)
                                                           for k, v in passes.items() if v[1]})
 
@@ -198,7 +203,8 @@ def main():
         "data": "synthetic (seeded k^-3 initial condition, seeded stratification)",
         "config": {"workload": "2048^2 x 10-layer QG step (L=4000 km, dt=900 s, nu4=2e8, double-gyre wind)",
                    "nx": p.NX, "ny": p.NY, "nl": p.nl, "form": head + ("+kahan" if head == "fp32" else ""),
-                   "interior_points": g.n * g.m},
+                   "interior_points": g.n * g.m,
+                   "p4_tile": {f: (Q.P4_TILES[t[Q.TUNE_P4_TILE]] if t else "stream") for f, t in tunes.items()}},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "algorithmic_bytes": alg_bytes,

@@ -141,6 +141,19 @@ int qgw_time_run(qgw_ctx *ctx, int nsteps, double *ms);
 int qgw_profile_enable(qgw_ctx *ctx, int on);
 int qgw_profile_read(qgw_ctx *ctx, double *ms, int *count, int npass);
 
+/* Tuning knobs.  QGW_TUNE_P4_TILE selects the tile of the tendency/AB3
+ * kernel: 0: 64x8, 1: 64x16, 2: 64x32, 3: 128x16 (columns x rows) LDS
+ * tiles, 4: register-streaming wavefront strips (default). */
+#define QGW_TUNE_P4_TILE 1
+int qgw_tune_set(qgw_ctx *ctx, int knob, int value);
+int qgw_tune_get(qgw_ctx *ctx, int knob, int *value);
+
+/* Autotuner: times `reps` steps with each P4 tile shape and keeps the fastest
+ * (returned in *p4_tile, its ms per step in *best_ms).  It advances the run,
+ * so call it on a scratch context of the same shape and copy the choice with
+ * qgw_tune_set. */
+int qgw_autotune(qgw_ctx *ctx, int reps, int *p4_tile, double *best_ms);
+
 /* Bytes of device memory held by the context. */
 int qgw_device_bytes(qgw_ctx *ctx, size_t *bytes);
 

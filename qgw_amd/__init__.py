@@ -28,6 +28,9 @@ NL_MAX = 1000
 FP64, FP32, FP32_PLAIN = 0, 1, 2
 FIELD_Q, FIELD_PSI, FIELD_TOPO, FIELD_F0, FIELD_F1, FIELD_F2 = 1, 2, 3, 4, 5, 6
 PASS_NAMES = ["P1", "T1", "P2", "T2", "P3", "P4", "CFL", "DIAG"]
+TUNE_P4_TILE = 1
+P4_TILES = ["64x8", "64x16", "64x32", "128x16", "stream"]
+_TUNE_CACHE = {}
 
 QGW_OK = 0
 
@@ -146,6 +149,8 @@ def hip_lib():
             ("qgw_synchronize", [vp]), ("qgw_time_run", [vp, ctypes.c_int, _dp]),
             ("qgw_profile_enable", [vp, ctypes.c_int]), ("qgw_profile_read", [vp, _dp, ip, ctypes.c_int]),
             ("qgw_device_bytes", [vp, ctypes.POINTER(ctypes.c_size_t)]),
+            ("qgw_tune_set", [vp, ctypes.c_int, ctypes.c_int]), ("qgw_tune_get", [vp, ctypes.c_int, ip]),
+            ("qgw_autotune", [vp, ctypes.c_int, ip, _dp]),
         ]:
             f = getattr(lib, name)
             f.argtypes = args
@@ -381,6 +386,29 @@ class Model:
         cnt = (ctypes.c_int * len(PASS_NAMES))()
         self._chk(self.lib.qgw_profile_read(self.ctx, _ptr(ms), cnt, len(PASS_NAMES)), "profile_read")
         return {PASS_NAMES[i]: (ms[i], cnt[i]) for i in range(len(PASS_NAMES))}
+
+    def tune_set(self, knob, value):
+        self._chk(self.lib.qgw_tune_set(self.ctx, knob, value), "tune_set")
+
+    def tune_get(self, knob):
+        v = ctypes.c_int()
+        self._chk(self.lib.qgw_tune_get(self.ctx, knob, ctypes.byref(v)), "tune_get")
+        return v.value
+
+    def autotune(self, reps=5):
+        """Pick the tuning of this model by timing a scratch model of the same
+        shape (the timing runs advance its state, not this one's)."""
+        key = (self.precision, self.n, self.m, self.nl, self.p.n_hyper, self.p.nu_hyper != 0)
+        if key not in _TUNE_CACHE:
+            scratch = Model(self.p, self.precision)
+            t, ms = ctypes.c_int(), ctypes.c_double()
+            scratch._chk(scratch.lib.qgw_autotune(scratch.ctx, reps, ctypes.byref(t), ctypes.byref(ms)), "autotune")
+            scratch.close()
+            _TUNE_CACHE[key] = {TUNE_P4_TILE: t.value, "ms_per_step": ms.value}
+        for knob, v in _TUNE_CACHE[key].items():
+            if knob != "ms_per_step":
+                self.tune_set(knob, v)
+        return dict(_TUNE_CACHE[key])
 
     def device_bytes(self):
         b = ctypes.c_size_t()

@@ -553,6 +553,231 @@ k_p4(Geo g, const T *__restrict__ psi, const T *__restrict__ q, T *__restrict__ 
 }
 
 // ===========================================================================
+// P4, streaming form.  One wavefront owns a strip of 64 storage columns (its
+// 64 - 2H middle columns are outputs, H = max(NLAP, 1)) of one layer and
+// walks a chunk of rows.  psi, q, the nested Laplacian levels and the
+// Jacobian's 3x3 neighbourhoods live in rolling register windows; the
+// horizontal neighbours come from DPP wavefront shifts (wave_shl:1 gives lane
+// i the value of lane i+1, wave_shr:1 that of lane i-1); there is no LDS and
+// no barrier.  Each psi row is read once per chunk (plus 2H+2 warm-up rows).
+// Same arithmetic and wall rules as the tiled form above.
+// ===========================================================================
+__device__ __forceinline__ float dpp_right(float v) {  // value of lane i+1
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x130, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float dpp_left(float v) {  // value of lane i-1
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x138, 0xF, 0xF, false));
+}
+__device__ __forceinline__ double dpp_right(double v) {
+  const long long b = __builtin_bit_cast(long long, v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffLL), 0x130, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x130, 0xF, 0xF, false);
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ double dpp_left(double v) {
+  const long long b = __builtin_bit_cast(long long, v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffLL), 0x138, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x138, 0xF, 0xF, false);
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
+}
+
+// 12 Delta^2 J(p, q) in the 12-term form of src/domain.h:17-27, from the 3x3
+// neighbourhoods (c: row j, u: row j+1, d: row j-1; l: column i-1, r: i+1).
+template <typename T>
+__device__ __forceinline__ T arakawa12(T pcl, T pcr, T pu, T pd, T pul, T pur, T pdl, T pdr, T qcl, T qcr, T qu,
+                                       T qd, T qul, T qur, T qdl, T qdr) {
+  return (pcr - pcl) * (qu - qd) + (pd - pu) * (qcr - qcl) + pcr * (qur - qdr) - pcl * (qul - qdl) -
+         pu * (qur - qul) + pd * (qdr - qdl) + qu * (pur - pul) - qd * (pdr - pdl) - qcr * (pur - pdr) +
+         qcl * (pul - pdl);
+}
+
+template <typename T, int NLAP, int KAHAN>
+__global__ void __launch_bounds__(256)
+k_p4s(Geo g, const T *__restrict__ psi, const T *__restrict__ q, T *__restrict__ qn, T *__restrict__ fnew,
+      const T *__restrict__ f1, const T *__restrict__ f2, T *__restrict__ comp, const T *__restrict__ topo,
+      const T *__restrict__ wind, StepCoef cf, int nstrip, int chunk, int nchunk) {
+  constexpr int H = (NLAP > 1) ? NLAP : 1;       // halo, and the delay of the output row
+  constexpr int OW = 64 - 2 * H;                 // output columns per strip
+  constexpr int NP = H + 2;                      // psi rows r, r-1, ..., r-H-1
+  constexpr int NL = (NLAP > 1) ? NLAP - 1 : 1;  // stored levels 1..NLAP-1
+  const int n = g.n, m = g.m;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
+  if (wave >= nstrip * nchunk * g.nl) return;  // whole wavefronts only; no barrier follows
+  const int sidx = wave % nstrip;
+  const int cidx = (wave / nstrip) % nchunk;
+  const int k = wave / (nstrip * nchunk);
+  const int x = 1 + sidx * OW - H + lane;  // storage column of this lane
+  const bool xin = (x >= 0 && x <= n + 1);
+  const bool outl = (lane >= H && lane < 64 - H && x >= 1 && x <= n);
+  const bool ringstrip = (sidx == 0 || sidx == nstrip - 1);  // wavefront-uniform
+  const int xc = xin ? x : 0;                                  // clamped column for addresses
+  const int y0 = 1 + cidx * chunk;
+  const int y1 = (y0 + chunk < m + 1) ? y0 + chunk : m + 1;
+  const long lo = k * g.LS;
+  const T *P = psi + lo, *Qf = q + lo;
+  f1 += lo;
+  f2 += lo;
+  qn += lo;
+  fnew += lo;
+  if (KAHAN) comp += lo;
+  const bool bottom = (k == g.nl - 1);
+  const bool upd = cf.update != 0;
+  const T inv_d2 = (T)cf.inv_d2, ring_c = (T)cf.ring_c;
+  const T beta = (T)cf.beta, nu = (T)cf.nu, nuh = (T)cf.nu_hyper;
+  const T i12 = (T)cf.inv_12d2, i2d = (T)cf.inv_2d;
+  const T c0 = (T)cf.c0, c1 = (T)cf.c1, c2 = (T)cf.c2;
+  const T drag = (T)cf.drag, tfac = (T)cf.topo_fac;
+  const bool use_nu = cf.nu != 0.0;
+
+  T pw[NP];                                           // psi rows r .. r-H-1
+  T nbw[NP];                                          // psi of the interior neighbour, ring columns
+  T lv[NL][3];                                        // level a rows r-a, r-a-1, r-a-2
+  T jpc[3], jpl[3], jpr[3], jqc[3], jql[3], jqr[3];  // rows y+1, y, y-1 of psi and q
+  T jtc[3], jtl[3], jtr[3];                           // topography rows (bottom layer)
+#pragma unroll
+  for (int i = 0; i < NP; i++) pw[i] = nbw[i] = T(0);
+#pragma unroll
+  for (int a = 0; a < NL; a++)
+#pragma unroll
+    for (int i = 0; i < 3; i++) lv[a][i] = T(0);
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    jpc[i] = jpl[i] = jpr[i] = T(0);
+    jqc[i] = jql[i] = jqr[i] = T(0);
+    jtc[i] = jtl[i] = jtr[i] = T(0);
+  }
+
+  // Loads of iteration r: psi row r, q and topography row r-H+1, and the
+  // AB3 history of output row r-H.  Issued one iteration ahead.
+  const int rbeg = y0 - H, rend = y1 + H;
+  auto ldrow = [&](const T *base, int row) -> T {
+    return (xin && row >= 0 && row <= m + 1) ? base[row * g.P + xc] : T(0);
+  };
+  auto ldout = [&](const T *base, int row) -> T {
+    return (outl && row >= y0 && row < y1) ? base[row * g.P + x] : T(0);
+  };
+  T np = ldrow(P, rbeg), nq = ldrow(Qf, rbeg - H + 1);
+  T nt = bottom ? ldrow(topo, rbeg - H + 1) : T(0);
+  T nf1 = T(0), nf2 = T(0), ncp = T(0);
+
+  for (int r = rbeg; r < rend; r++) {
+    const int y = r - H;  // output row of this iteration
+    const T p0 = np, qnew = nq, tnew = nt, h1 = nf1, h2 = nf2, hc = ncp;
+    if (r + 1 < rend) {
+      np = ldrow(P, r + 1);
+      nq = ldrow(Qf, r + 2 - H);
+      if (bottom) nt = ldrow(topo, r + 2 - H);
+      if (upd) {
+        nf1 = ldout(f1, r + 1 - H);
+        nf2 = ldout(f2, r + 1 - H);
+        if (KAHAN) ncp = ldout(comp, r + 1 - H);
+      }
+    }
+    // ---- shift in psi row r
+#pragma unroll
+    for (int i = NP - 1; i > 0; i--) {
+      pw[i] = pw[i - 1];
+      nbw[i] = nbw[i - 1];
+    }
+    pw[0] = p0;
+    if (ringstrip) {
+      const T pr = dpp_right(p0), pl = dpp_left(p0);
+      nbw[0] = (x == 0) ? pr : pl;
+    }
+    // ---- nested Laplacians: level a at row r - a (calc_diff, src/dynamics.h:25-43)
+    T hyp = T(0);
+    if constexpr (NLAP >= 1) {
+      T s0 = pw[0], s1 = pw[1], s2 = pw[2];  // source rows yl+1, yl, yl-1 of level a-1
+#pragma unroll
+      for (int a = 1; a <= NLAP; a++) {
+        const int yl = r - a;
+        T v = (dpp_left(s1) + dpp_right(s1) + s0 + s2 - T(4) * s1) * inv_d2;
+        if (yl == 0 || yl == m + 1) {
+          // wall row: 2 bc_fac/Delta^2 * psi of the adjacent interior row; corners 0
+          v = (x == 0 || x == n + 1) ? T(0) : ring_c * ((yl == 0) ? pw[a - 1] : pw[a + 1]);
+        } else if (ringstrip && (x == 0 || x == n + 1)) {
+          v = ring_c * nbw[a];  // wall column: its interior neighbour's psi
+        }
+        if (a < NLAP) {
+          lv[a - 1][2] = lv[a - 1][1];
+          lv[a - 1][1] = lv[a - 1][0];
+          lv[a - 1][0] = v;
+          s0 = lv[a - 1][0];
+          s1 = lv[a - 1][1];
+          s2 = lv[a - 1][2];
+        } else {
+          hyp = v;
+        }
+      }
+    }
+    // ---- 3x3 windows for the Jacobians at row y
+    const T pn = pw[H - 1];  // psi row y+1
+#pragma unroll
+    for (int i = 2; i > 0; i--) {
+      jpc[i] = jpc[i - 1];
+      jpl[i] = jpl[i - 1];
+      jpr[i] = jpr[i - 1];
+      jqc[i] = jqc[i - 1];
+      jql[i] = jql[i - 1];
+      jqr[i] = jqr[i - 1];
+    }
+    jpc[0] = pn;
+    jpl[0] = dpp_left(pn);
+    jpr[0] = dpp_right(pn);
+    jqc[0] = qnew;
+    jql[0] = dpp_left(qnew);
+    jqr[0] = dpp_right(qnew);
+    if (bottom) {
+#pragma unroll
+      for (int i = 2; i > 0; i--) {
+        jtc[i] = jtc[i - 1];
+        jtl[i] = jtl[i - 1];
+        jtr[i] = jtr[i - 1];
+      }
+      jtc[0] = tnew;
+      jtl[0] = dpp_left(tnew);
+      jtr[0] = dpp_right(tnew);
+    }
+    if (y < y0) continue;  // wavefront-uniform
+    // ---- tendency at row y (rhs, src/dynamics.h:52-71)
+    const T jpq = arakawa12<T>(jpl[1], jpr[1], jpc[0], jpc[2], jpl[0], jpr[0], jpl[2], jpr[2], jql[1], jqr[1],
+                               jqc[0], jqc[2], jql[0], jqr[0], jql[2], jqr[2]) *
+                  i12;
+    T f = -jpq - beta * (jpr[1] - jpl[1]) * i2d;
+    if (use_nu) f += nu * ((jql[1] + jqr[1] + jqc[0] + jqc[2] - T(4) * jqc[1]) * inv_d2);
+    if constexpr (NLAP >= 1) f += nuh * hyp;
+    if (k == 0) f += wind[y];
+    if (bottom) {
+      const T lap1 = (jpl[1] + jpr[1] + jpc[0] + jpc[2] - T(4) * jpc[1]) * inv_d2;
+      const T jt = arakawa12<T>(jpl[1], jpr[1], jpc[0], jpc[2], jpl[0], jpr[0], jpl[2], jpr[2], jtl[1], jtr[1],
+                                jtc[0], jtc[2], jtl[0], jtr[0], jtl[2], jtr[2]) *
+                   i12;
+      f += drag * lap1 - jt * tfac;
+    }
+    if (!outl) continue;
+    const int o = y * g.P + x;
+    fnew[o] = f;
+    if (!upd) continue;
+    const T qo = jqc[1];
+    const T inc = c0 * f + c1 * h1 + c2 * h2;
+    if constexpr (KAHAN) {
+      const T yk = inc - hc;
+      const T tq = qo + yk;
+      comp[o] = (tq - qo) - yk;
+      qn[o] = tq;
+    } else {
+      qn[o] = qo + inc;
+    }
+    // carry q's wall ring into the next buffer (corners stay zero)
+    if (y == 1) qn[x] = jqc[2];
+    if (y == m) qn[(m + 1) * g.P + x] = jqc[0];
+    if (x == 1) qn[y * g.P] = jql[1];
+    if (x == n) qn[y * g.P + n + 1] = jqr[1];
+  }
+}
+
+// ===========================================================================
 // CFL reduction (adaptive dt): max over interior points and layers of
 // |psi(i+1,j) - psi(i,j)| and |psi(i,j+1) - psi(i,j)| (src/timestep.h:74-87).
 // The maximum of a non-negative double is reduced through its bit pattern.

@@ -12,6 +12,13 @@ namespace qgw {
 // Layers handled by the register-resident projections of P1/P3.
 constexpr int kMaxLayers = 16;
 
+// P4 tile shapes (columns x rows); the workgroup has 4*columns threads.  Variant
+// kP4TileCount is the streaming form (k_p4s).
+constexpr int kP4TileCount = 4;
+constexpr int kP4Variants = kP4TileCount + 1;
+constexpr int kP4TileW[kP4TileCount] = {64, 64, 64, 128};
+constexpr int kP4TileH[kP4TileCount] = {8, 16, 32, 16};
+
 struct LaunchArgs {
   Geo g{};
   int prec = 0;
@@ -24,6 +31,7 @@ struct LaunchArgs {
   double norm = 1.0, ring_c = 0.0;
   StepCoef cf{};
   int nlap = 0;
+  int p4_tile = kP4TileCount;  // P4 variant: a tile index, or kP4TileCount for streaming
   unsigned long long *cfl_max = nullptr;
 };
 

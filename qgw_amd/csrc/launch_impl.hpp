@@ -93,23 +93,49 @@ template <> int launch_t2_t<T>(const LaunchArgs &a) {
   return tproj(a, a.whatT, a.what, a.cm2l, a.g.n, a.g.m, a.g.Pm, a.g.WTS, a.g.Pn, a.g.WS);
 }
 
-constexpr int kTX = 64, kTY = 16;
-
-template <int NLAP, int KAHAN> static void p4_go(const LaunchArgs &a) {
-  const dim3 grid((a.g.n + kTX - 1) / kTX, (a.g.m + kTY - 1) / kTY, a.g.nl);
-  hipLaunchKernelGGL((k_p4<T, NLAP, KAHAN, kTX, kTY>), grid, dim3(kTX * 4), 0, a.stream, a.g, (const T *)a.psi,
+template <int NLAP, int KAHAN, int TX, int TY> static void p4_go(const LaunchArgs &a) {
+  const dim3 grid((a.g.n + TX - 1) / TX, (a.g.m + TY - 1) / TY, a.g.nl);
+  hipLaunchKernelGGL((k_p4<T, NLAP, KAHAN, TX, TY>), grid, dim3(TX * 4), 0, a.stream, a.g, (const T *)a.psi,
                      (const T *)a.q, (T *)a.qn, (T *)a.fnew, (const T *)a.f1, (const T *)a.f2, (T *)a.comp,
                      (const T *)a.topo, (const T *)a.wind, a.cf);
 }
 
+template <int NLAP, int KAHAN> static void p4s_go(const LaunchArgs &a) {
+  constexpr int H = (NLAP > 1) ? NLAP : 1;
+  const int ow = 64 - 2 * H;
+  const int nstrip = (a.g.n + ow - 1) / ow;
+  // chunk of rows per wavefront: enough wavefronts to fill the device (~8 per SIMD)
+  long lanes = (long)nstrip * a.g.nl;
+  int chunk = (int)((long)a.g.m * lanes / 8192);
+  chunk = chunk < 16 ? 16 : (chunk > 256 ? 256 : chunk);
+  const int nchunk = (a.g.m + chunk - 1) / chunk;
+  const long waves = lanes * nchunk;
+  const dim3 grid((unsigned)((waves + 3) / 4));
+  hipLaunchKernelGGL((k_p4s<T, NLAP, KAHAN>), grid, dim3(256), 0, a.stream, a.g, (const T *)a.psi, (const T *)a.q,
+                     (T *)a.qn, (T *)a.fnew, (const T *)a.f1, (const T *)a.f2, (T *)a.comp, (const T *)a.topo,
+                     (const T *)a.wind, a.cf, nstrip, chunk, nchunk);
+}
+
+template <int NLAP, int KAHAN> static int p4_tile(const LaunchArgs &a) {
+  static_assert(kP4TileCount == 4, "tile table");
+  switch (a.p4_tile) {
+    case kP4TileCount: p4s_go<NLAP, KAHAN>(a); return 0;
+    case 0: p4_go<NLAP, KAHAN, kP4TileW[0], kP4TileH[0]>(a); return 0;
+    case 1: p4_go<NLAP, KAHAN, kP4TileW[1], kP4TileH[1]>(a); return 0;
+    case 2: p4_go<NLAP, KAHAN, kP4TileW[2], kP4TileH[2]>(a); return 0;
+    case 3: p4_go<NLAP, KAHAN, kP4TileW[3], kP4TileH[3]>(a); return 0;
+  }
+  return QGW_ERR_ARG;
+}
+
 template <int KAHAN> static int p4_nlap(const LaunchArgs &a) {
   switch (a.nlap) {
-    case 0: p4_go<0, KAHAN>(a); return 0;
-    case 1: p4_go<1, KAHAN>(a); return 0;
-    case 2: p4_go<2, KAHAN>(a); return 0;
-    case 3: p4_go<3, KAHAN>(a); return 0;
-    case 4: p4_go<4, KAHAN>(a); return 0;
-    case 5: p4_go<5, KAHAN>(a); return 0;
+    case 0: return p4_tile<0, KAHAN>(a);
+    case 1: return p4_tile<1, KAHAN>(a);
+    case 2: return p4_tile<2, KAHAN>(a);
+    case 3: return p4_tile<3, KAHAN>(a);
+    case 4: return p4_tile<4, KAHAN>(a);
+    case 5: return p4_tile<5, KAHAN>(a);
   }
   return QGW_ERR_ARG;
 }

@@ -49,6 +49,7 @@ struct qgw_ctx {
   double *iRd2_d, *La2, *Kb2, *diag_part, *diag_out, *stage;
   unsigned long long *cfl_max;
   int nblk_diag;
+  int p4_tile;
   bool modes_set;
   double t, dt, dt0, dt1, DT_max, t_out;
   int it;
@@ -292,6 +293,7 @@ int run_p4(qgw_ctx *c, const double *cab3, int update) {
   cf.ring_c = 2.0 * c->p.bc_fac / (D * D);
   cf.update = update;
   a.nlap = (c->nu_hyper != 0.0) ? (c->p.n_hyper / 2 + 1) : 0;
+  a.p4_tile = c->p4_tile;
   int st;
   {
     PassScope ps(c, QGW_PASS_P4);
@@ -500,6 +502,7 @@ qgw_ctx *qgw_create(const qgw_params *p, int precision, int *status) {
   c->t_out = 0;
   c->it = 0;
   c->qcur = 0;
+  c->p4_tile = qgw::kP4TileCount;  // the register-streaming form
   c->modes_set = false;
   // the zero fills above ran on the null stream; the step runs on c->stream
   if (hipDeviceSynchronize() != hipSuccess) {
@@ -745,6 +748,47 @@ int qgw_profile_read(qgw_ctx *c, double *ms, int *count, int npass) {
     c->prof_ms[i] = 0;
     c->prof_cnt[i] = 0;
   }
+  return QGW_OK;
+}
+
+int qgw_tune_set(qgw_ctx *c, int knob, int value) {
+  if (!c) return QGW_ERR_ARG;
+  if (knob == QGW_TUNE_P4_TILE) {
+    if (value < 0 || value >= qgw::kP4Variants) return QGW_ERR_ARG;
+    c->p4_tile = value;
+    return QGW_OK;
+  }
+  return QGW_ERR_ARG;
+}
+
+int qgw_tune_get(qgw_ctx *c, int knob, int *value) {
+  if (!c || !value) return QGW_ERR_ARG;
+  if (knob == QGW_TUNE_P4_TILE) {
+    *value = c->p4_tile;
+    return QGW_OK;
+  }
+  return QGW_ERR_ARG;
+}
+
+int qgw_autotune(qgw_ctx *c, int reps, int *p4_tile, double *best_ms) {
+  if (!c || reps < 1) return QGW_ERR_ARG;
+  double best = 1e300;
+  int arg = c->p4_tile;
+  for (int v = 0; v < qgw::kP4Variants; v++) {
+    c->p4_tile = v;
+    int st = qgw_run(c, 2);  // warm-up (and first-launch code loading)
+    if (st) return st;
+    double ms = 0;
+    st = qgw_time_run(c, reps, &ms);
+    if (st) return st;
+    if (ms < best) {
+      best = ms;
+      arg = v;
+    }
+  }
+  c->p4_tile = arg;
+  if (p4_tile) *p4_tile = arg;
+  if (best_ms) *best_ms = best / reps;
   return QGW_OK;
 }
 

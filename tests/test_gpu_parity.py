@@ -169,3 +169,20 @@ def test_upload_download_roundtrip_fp32_keeps_residual():
     _, q0 = Q.make_ic(p)
     m.set_q(q0)
     assert rel(m.q, q0) < 1e-13
+
+
+@pytest.mark.parametrize("tile", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("prec", [Q.FP64, Q.FP32])
+def test_p4_tiles_agree(tile, prec):
+    # every tile shape of the tendency kernel, on a grid with partial edge
+    # tiles, walls with slip, topography and the q ring
+    p = _small(96, 3, bc_fac=0.5, nu=10.0, nu_hyper=1e8, h_topo=150.0, w_topo=80e3, NY=64)
+    _announce("p4 tile", Q.P4_TILES[tile], "prec", prec)
+    m, o, _, _ = _pair(p, precision=prec)
+    m.tune_set(Q.TUNE_P4_TILE, tile)
+    assert m.tune_get(Q.TUNE_P4_TILE) == tile
+    tol = 1e-12 if prec == Q.FP64 else 2e-5
+    assert rel(m.rhs(), o.rhs()) < tol
+    m.run(5)
+    o.run(5)
+    assert rel(m.q, o.field(O.Q)) < tol
