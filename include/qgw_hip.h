@@ -148,10 +148,11 @@ int qgw_profile_read(qgw_ctx *ctx, double *ms, int *count, int npass);
 int qgw_tune_set(qgw_ctx *ctx, int knob, int value);
 int qgw_tune_get(qgw_ctx *ctx, int knob, int *value);
 
-/* Autotuner: times `reps` steps with each P4 tile shape and keeps the fastest
- * (returned in *p4_tile, its ms per step in *best_ms).  It advances the run,
- * so call it on a scratch context of the same shape and copy the choice with
- * qgw_tune_set. */
+/* Autotuner: times `reps` steps with each P4 variant and keeps the fastest
+ * (returned in *p4_tile, its ms per step in *best_ms).  The state (fields,
+ * buffer rotation and time) is copied before the timing runs and restored
+ * after them, so the run continues as if the call had not been made; the copy
+ * takes one more state's worth of device memory while it runs. */
 int qgw_autotune(qgw_ctx *ctx, int reps, int *p4_tile, double *best_ms);
 
 /* Bytes of device memory held by the context. */
@@ -161,6 +162,51 @@ int qgw_device_bytes(qgw_ctx *ctx, size_t *bytes);
  * context is appended by qgw_last_error. */
 const char *qgw_strerror(int status);
 const char *qgw_last_error(qgw_ctx *ctx);
+
+/* ---- Ensembles ------------------------------------------------------------
+ * No counterpart in the reference, which runs one model per process (its
+ * state is process-global, src/qg.c:48-52).  A batch advances independent
+ * runs ("members") together on one stream.  Members whose grid and step
+ * scalars agree (NX, NY, nl, Lx, n_hyper, nu_hyper, nu, beta, f0, hEkb,
+ * dh[nl-1], bc_fac, dt_fixed) form a group that shares every launch of a
+ * step: its members are stacked along the layer axis.  Each member keeps its
+ * own modes, wind (tau0, forc_mode), topography and state.  Every member needs
+ * a fixed step (dt_fixed > 0).  qgw_batch_step replays a hipGraph of six
+ * steps of every group (the period of the buffer rotation), captured on first
+ * use; remainders run as ordinary launches.  Members are numbered in the
+ * order of `params`. */
+typedef struct qgw_batch qgw_batch;
+
+qgw_batch *qgw_batch_create(int nmembers, const qgw_params *params, int precision, int *status);
+int qgw_batch_destroy(qgw_batch *b);
+/* Number of groups, and each member's group (group_of_member[nmembers], may be NULL). */
+int qgw_batch_groups(qgw_batch *b, int *ngroups, int *group_of_member);
+int qgw_batch_set_modes(qgw_batch *b, int member, const double *cl2m, const double *cm2l, const double *iRd2);
+int qgw_batch_upload(qgw_batch *b, int member, int field, const double *host);
+int qgw_batch_download(qgw_batch *b, int member, int field, double *host);
+/* invert_pv (src/elliptic.h:76) of every member. */
+int qgw_batch_invert_pv(qgw_batch *b);
+/* nsteps of timestep (src/timestep.h:114) for every member. */
+int qgw_batch_step(qgw_batch *b, int nsteps);
+/* As qgw_batch_step, timed with HIP events on the batch stream. */
+int qgw_batch_time_step(qgw_batch *b, int nsteps, double *ms);
+int qgw_batch_synchronize(qgw_batch *b);
+int qgw_batch_diagnostics(qgw_batch *b, int member, double *E_layer, double *Z_layer, double *E_tot,
+                          double *Z_tot);
+int qgw_batch_get_time(qgw_batch *b, double *t, int *it);
+/* Graph replay on (default) or off (every step launched one kernel at a time). */
+int qgw_batch_use_graph(qgw_batch *b, int on);
+/* Number of graph replays so far. */
+int qgw_batch_graph_launches(qgw_batch *b, int *count);
+/* Autotune every group (as qgw_autotune; state preserved); the choice of
+ * group k is returned in p4_tile_per_group[k] (may be NULL). */
+int qgw_batch_autotune(qgw_batch *b, int reps, int *p4_tile_per_group);
+int qgw_batch_tune_set(qgw_batch *b, int group, int knob, int value);
+/* Per-pass timing summed over groups (steps then run without the graph). */
+int qgw_batch_profile_enable(qgw_batch *b, int on);
+int qgw_batch_profile_read(qgw_batch *b, double *ms, int *count, int npass);
+int qgw_batch_device_bytes(qgw_batch *b, size_t *bytes);
+const char *qgw_batch_last_error(qgw_batch *b);
 
 #ifdef __cplusplus
 }

@@ -159,6 +159,27 @@ def hip_lib():
         lib.qgw_strerror.restype = ctypes.c_char_p
         lib.qgw_last_error.argtypes = [vp]
         lib.qgw_last_error.restype = ctypes.c_char_p
+        lib.qgw_batch_create.argtypes = [ctypes.c_int, P, ctypes.c_int, ip]
+        lib.qgw_batch_create.restype = vp
+        ci = ctypes.c_int
+        for name, args in [
+            ("qgw_batch_destroy", [vp]), ("qgw_batch_groups", [vp, ip, ip]),
+            ("qgw_batch_set_modes", [vp, ci, _dp, _dp, _dp]),
+            ("qgw_batch_upload", [vp, ci, ci, _dp]), ("qgw_batch_download", [vp, ci, ci, _dp]),
+            ("qgw_batch_invert_pv", [vp]), ("qgw_batch_step", [vp, ci]),
+            ("qgw_batch_time_step", [vp, ci, _dp]), ("qgw_batch_synchronize", [vp]),
+            ("qgw_batch_diagnostics", [vp, ci, _dp, _dp, _dp, _dp]),
+            ("qgw_batch_get_time", [vp, _dp, ip]), ("qgw_batch_use_graph", [vp, ci]),
+            ("qgw_batch_graph_launches", [vp, ip]), ("qgw_batch_autotune", [vp, ci, ip]),
+            ("qgw_batch_tune_set", [vp, ci, ci, ci]),
+            ("qgw_batch_profile_enable", [vp, ci]), ("qgw_batch_profile_read", [vp, _dp, ip, ci]),
+            ("qgw_batch_device_bytes", [vp, ctypes.POINTER(ctypes.c_size_t)]),
+        ]:
+            f = getattr(lib, name)
+            f.argtypes = args
+            f.restype = ctypes.c_int
+        lib.qgw_batch_last_error.argtypes = [vp]
+        lib.qgw_batch_last_error.restype = ctypes.c_char_p
         _hip = lib
     return _hip
 
@@ -396,14 +417,13 @@ class Model:
         return v.value
 
     def autotune(self, reps=5):
-        """Pick the tuning of this model by timing a scratch model of the same
-        shape (the timing runs advance its state, not this one's)."""
+        """Pick the tuning of this model by timing each variant on its own
+        state (qgw_autotune copies the state first and restores it after).
+        Choices are cached per shape and form."""
         key = (self.precision, self.n, self.m, self.nl, self.p.n_hyper, self.p.nu_hyper != 0)
         if key not in _TUNE_CACHE:
-            scratch = Model(self.p, self.precision)
             t, ms = ctypes.c_int(), ctypes.c_double()
-            scratch._chk(scratch.lib.qgw_autotune(scratch.ctx, reps, ctypes.byref(t), ctypes.byref(ms)), "autotune")
-            scratch.close()
+            self._chk(self.lib.qgw_autotune(self.ctx, reps, ctypes.byref(t), ctypes.byref(ms)), "autotune")
             _TUNE_CACHE[key] = {TUNE_P4_TILE: t.value, "ms_per_step": ms.value}
         for knob, v in _TUNE_CACHE[key].items():
             if knob != "ms_per_step":
@@ -413,4 +433,139 @@ class Model:
     def device_bytes(self):
         b = ctypes.c_size_t()
         self._chk(self.lib.qgw_device_bytes(self.ctx, ctypes.byref(b)), "device_bytes")
+        return b.value
+
+
+class Batch:
+    """An ensemble on the GPU: a ``qgw_batch`` of libqgw_hip.so.
+
+    ``params`` is a list of Params, one per member (each needs dt_fixed > 0).
+    Members of equal grid and step scalars share every launch of a step; a
+    step of all members is replayed from a captured hipGraph."""
+
+    def __init__(self, params, precision=FP32, modes=None):
+        self.lib = hip_lib()
+        self.params = list(params)
+        self.precision = precision
+        n = len(self.params)
+        arr = (Params * n)(*self.params)
+        st = ctypes.c_int(0)
+        self.b = self.lib.qgw_batch_create(n, arr, precision, ctypes.byref(st))
+        if not self.b:
+            raise QGWError("qgw_batch_create failed: %s" % self.lib.qgw_strerror(st.value).decode())
+        self.shapes = []
+        for p in self.params:
+            g = grid(p)[0]
+            self.shapes.append((p.nl, g.nyp2, g.nxp2))
+        for i, p in enumerate(self.params):
+            self.set_modes(i, *(modes[i] if modes is not None else eigmode(p)))
+
+    def __len__(self):
+        return len(self.params)
+
+    def _chk(self, st, what):
+        if st != QGW_OK:
+            raise QGWError("%s: %s (%s)" % (what, self.lib.qgw_strerror(st).decode(),
+                                            self.lib.qgw_batch_last_error(self.b).decode()))
+
+    def close(self):
+        if self.b:
+            self.lib.qgw_batch_destroy(self.b)
+            self.b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def groups(self):
+        ng = ctypes.c_int()
+        gm = (ctypes.c_int * len(self.params))()
+        self._chk(self.lib.qgw_batch_groups(self.b, ctypes.byref(ng), gm), "groups")
+        return ng.value, list(gm)
+
+    def set_modes(self, member, cl2m, cm2l, iRd2):
+        a = [np.ascontiguousarray(x, dtype=np.float64) for x in (cl2m, cm2l, iRd2)]
+        self._chk(self.lib.qgw_batch_set_modes(self.b, member, _ptr(a[0]), _ptr(a[1]), _ptr(a[2])), "set_modes")
+
+    def upload(self, member, field, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        nl, ny, nx = self.shapes[member]
+        need = ny * nx * (1 if field == FIELD_TOPO else nl)
+        if a.size != need:
+            raise QGWError("upload: expected %d values, got %d" % (need, a.size))
+        self._chk(self.lib.qgw_batch_upload(self.b, member, field, _ptr(a)), "upload")
+
+    def download(self, member, field):
+        nl, ny, nx = self.shapes[member]
+        a = np.zeros((ny, nx) if field == FIELD_TOPO else (nl, ny, nx))
+        self._chk(self.lib.qgw_batch_download(self.b, member, field, _ptr(a)), "download")
+        return a
+
+    def set_q(self, member, q):
+        self.upload(member, FIELD_Q, q)
+
+    def q(self, member):
+        return self.download(member, FIELD_Q)
+
+    def psi(self, member):
+        return self.download(member, FIELD_PSI)
+
+    def invert_pv(self):
+        self._chk(self.lib.qgw_batch_invert_pv(self.b), "invert_pv")
+
+    def step(self, nsteps):
+        self._chk(self.lib.qgw_batch_step(self.b, int(nsteps)), "step")
+
+    def time_step(self, nsteps):
+        ms = ctypes.c_double()
+        self._chk(self.lib.qgw_batch_time_step(self.b, int(nsteps), ctypes.byref(ms)), "time_step")
+        return ms.value
+
+    def synchronize(self):
+        self._chk(self.lib.qgw_batch_synchronize(self.b), "synchronize")
+
+    def diagnostics(self, member):
+        nl = self.shapes[member][0]
+        El, Zl = np.zeros(nl), np.zeros(nl)
+        Et, Zt = ctypes.c_double(), ctypes.c_double()
+        self._chk(self.lib.qgw_batch_diagnostics(self.b, member, _ptr(El), _ptr(Zl), ctypes.byref(Et),
+                                                 ctypes.byref(Zt)), "diagnostics")
+        return El, Zl, Et.value, Zt.value
+
+    def get_time(self):
+        t, it = ctypes.c_double(), ctypes.c_int()
+        self._chk(self.lib.qgw_batch_get_time(self.b, ctypes.byref(t), ctypes.byref(it)), "get_time")
+        return dict(t=t.value, it=it.value)
+
+    def use_graph(self, on=True):
+        self._chk(self.lib.qgw_batch_use_graph(self.b, 1 if on else 0), "use_graph")
+
+    def graph_launches(self):
+        c = ctypes.c_int()
+        self._chk(self.lib.qgw_batch_graph_launches(self.b, ctypes.byref(c)), "graph_launches")
+        return c.value
+
+    def autotune(self, reps=3):
+        ng = self.groups()[0]
+        v = (ctypes.c_int * ng)()
+        self._chk(self.lib.qgw_batch_autotune(self.b, reps, v), "autotune")
+        return [P4_TILES[x] for x in v]
+
+    def tune_set(self, group, knob, value):
+        self._chk(self.lib.qgw_batch_tune_set(self.b, group, knob, value), "tune_set")
+
+    def profile(self, on=True):
+        self._chk(self.lib.qgw_batch_profile_enable(self.b, 1 if on else 0), "profile_enable")
+
+    def profile_read(self):
+        ms = np.zeros(len(PASS_NAMES))
+        cnt = (ctypes.c_int * len(PASS_NAMES))()
+        self._chk(self.lib.qgw_batch_profile_read(self.b, _ptr(ms), cnt, len(PASS_NAMES)), "profile_read")
+        return {PASS_NAMES[i]: (ms[i], cnt[i]) for i in range(len(PASS_NAMES))}
+
+    def device_bytes(self):
+        b = ctypes.c_size_t()
+        self._chk(self.lib.qgw_batch_device_bytes(self.b, ctypes.byref(b)), "device_bytes")
         return b.value

@@ -90,4 +90,39 @@ def scaled(p, NX, **over):
     return q.set(**over)
 
 
-WORKLOADS = {"c1": c1, "c2": c2, "c3": c3}
+# Ensemble size classes: (members, NX, nl, nu4).  nu4 keeps the hyperviscous
+# AB3 factor 26 nu4 dt / Delta^4 at or below ~0.27 for dt = 1200 s (the AB3
+# stability limit is 0.545).
+C4_CLASSES = [(32, 512, 2, 1e10), (32, 1024, 3, 2e9), (24, 1536, 5, 4e8), (8, 2048, 4, 1.25e8)]
+
+
+def c4_member(NX, nl, nu4, member, tau, dt=1200.0):
+    """One ensemble member: C2 physics (L = 4000 km, f0 = 1e-4, beta = 2e-11, drag 1e-7, double-gyre
+    wind) at NX^2 x nl.  nl = 3 uses C2's layers; other layer counts use log-depth layers summing
+    to 4000 m with the seeded exponential stratification of c3.  Seed 20261203 + member."""
+    if nl == 3:
+        dh = [350.0, 750.0, 2900.0]
+        N2 = _n2_from_gprime(dh, [0.025, 0.0125])
+    else:
+        dh, zb = _logdepth_layers(nl)
+        N2, _ = _exp_stratification(zb, SEED0)
+    return default_params(NX=NX, NY=NX, nl=nl, Lx=4000e3, dh=dh, N2=N2, f0=1e-4, beta=2e-11,
+                          hEkb=2 * 1e-7 * dh[-1] / 1e-4, n_hyper=4, nu_hyper=nu4, tau0=tau / RHO0,
+                          forc_mode=2.0, bc_fac=0.0, dt_fixed=dt, nsteps=500, ic_kind=1,
+                          seed=SEED0 + member, ic_urms=0.05, ic_lambda0=200e3)
+
+
+def c4(classes=None, dt=1200.0):
+    """The 96-member ensemble: 32 x (512^2, 2), 32 x (1024^2, 3), 24 x (1536^2, 5), 8 x (2048^2, 4);
+    wind amplitude tau ~ U[0.04, 0.12] N/m^2 per member (seeded), FP32 + Kahan, dt 1200 s."""
+    classes = C4_CLASSES if classes is None else classes
+    total = sum(c[0] for c in classes)
+    taus = np.random.default_rng(SEED0).uniform(0.04, 0.12, total)
+    out = []
+    for cnt, NX, nl, nu4 in classes:
+        for _ in range(cnt):
+            out.append(c4_member(NX, nl, nu4, len(out), float(taus[len(out)]), dt))
+    return out
+
+
+WORKLOADS = {"c1": c1, "c2": c2, "c3": c3, "c4": c4}

@@ -42,7 +42,8 @@ __device__ __forceinline__ int xcd_remap(int b, int total) {
 
 // Sizes shared by every kernel.
 struct Geo {
-  int n, m, nl;          // interior points in x, y; layers
+  int n, m, nl;          // interior points in x, y; layers (all members)
+  int nlm;               // layers per member: ensemble members are stacked along the layer axis
   int P;                 // row pitch of the storage fields (elements)
   long LS;               // layer stride of the storage fields (= (m+2)*P)
   int Pn;                // pitch of what (row = y index, column = x frequency)
@@ -113,6 +114,10 @@ k_tproj(const T *__restrict__ src, T *__restrict__ dst, const T *__restrict__ M,
   static_assert(TC * 8 == 256 && TR % 8 == 0, "tile shape");
   constexpr int U = TR / 8;  // source rows per thread
   __shared__ T tile[TR][TC + 1];
+  // member blockIdx.z: its nl layers and its own nl x nl matrix
+  src += (long)blockIdx.z * nl * Ss;
+  dst += (long)blockIdx.z * nl * Sd;
+  M += (long)blockIdx.z * nl * nl;
   const int c0 = blockIdx.x * TC, r0 = blockIdx.y * TR;
   const int tid = threadIdx.x;
   // every layer of the thread's points, lanes along c (contiguous in src)
@@ -385,7 +390,7 @@ __device__ __forceinline__ void p4_body(P4Smem<T, TX, TY, NLAP> &sm, const Geo &
   constexpr int H = SM::H, PW = SM::PW, PH = SM::PH;
   (void)PW;
   const int n = g.n, m = g.m;
-  const bool bottom = (k == g.nl - 1);
+  const bool bottom = (k == g.nlm - 1);  // k: layer within the member
   // ---- psi (halo H) and q (halo 1) tiles; zero outside the storage grid
   p4_region<TX, H, PH>(tx, ty, [&](int ly, int lx) {
     const int gx = x0 - H + lx, gy = y0 - H + ly;
@@ -543,12 +548,15 @@ k_p4(Geo g, const T *__restrict__ psi, const T *__restrict__ q, T *__restrict__ 
   }
   const T *P = psi + lo;
   const T *Q = q + lo;
+  const int mb = k / g.nlm, km = k - mb * g.nlm;  // member and its layer
+  topo += mb * g.LS;
+  wind += (long)mb * (m + 2);
   const bool fast = (x0 - H >= 0) && (x0 + TX - 1 + H <= n + 1) && (y0 - H >= 0) && (y0 + TY - 1 + H <= m + 1);
   if (fast)
-    p4_body<T, NLAP, KAHAN, TX, TY, true>(sm, g, P, Q, topo, k, x0, y0, tx, ty, hf1, hf2, hcp, qn, fnew, comp,
+    p4_body<T, NLAP, KAHAN, TX, TY, true>(sm, g, P, Q, topo, km, x0, y0, tx, ty, hf1, hf2, hcp, qn, fnew, comp,
                                           wind, cf);
   else
-    p4_body<T, NLAP, KAHAN, TX, TY, false>(sm, g, P, Q, topo, k, x0, y0, tx, ty, hf1, hf2, hcp, qn, fnew, comp,
+    p4_body<T, NLAP, KAHAN, TX, TY, false>(sm, g, P, Q, topo, km, x0, y0, tx, ty, hf1, hf2, hcp, qn, fnew, comp,
                                            wind, cf);
 }
 
@@ -606,7 +614,10 @@ k_p4s(Geo g, const T *__restrict__ psi, const T *__restrict__ q, T *__restrict__
   if (wave >= nstrip * nchunk * g.nl) return;  // whole wavefronts only; no barrier follows
   const int sidx = wave % nstrip;
   const int cidx = (wave / nstrip) % nchunk;
-  const int k = wave / (nstrip * nchunk);
+  const int k = wave / (nstrip * nchunk);  // layer (all members)
+  const int mb = k / g.nlm, km = k - mb * g.nlm;
+  topo += mb * g.LS;
+  wind += (long)mb * (m + 2);
   const int x = 1 + sidx * OW - H + lane;  // storage column of this lane
   const bool xin = (x >= 0 && x <= n + 1);
   const bool outl = (lane >= H && lane < 64 - H && x >= 1 && x <= n);
@@ -621,7 +632,7 @@ k_p4s(Geo g, const T *__restrict__ psi, const T *__restrict__ q, T *__restrict__
   qn += lo;
   fnew += lo;
   if (KAHAN) comp += lo;
-  const bool bottom = (k == g.nl - 1);
+  const bool bottom = (km == g.nlm - 1);
   const bool upd = cf.update != 0;
   const T inv_d2 = (T)cf.inv_d2, ring_c = (T)cf.ring_c;
   const T beta = (T)cf.beta, nu = (T)cf.nu, nuh = (T)cf.nu_hyper;
@@ -747,7 +758,7 @@ k_p4s(Geo g, const T *__restrict__ psi, const T *__restrict__ q, T *__restrict__
     T f = -jpq - beta * (jpr[1] - jpl[1]) * i2d;
     if (use_nu) f += nu * ((jql[1] + jqr[1] + jqc[0] + jqc[2] - T(4) * jqc[1]) * inv_d2);
     if constexpr (NLAP >= 1) f += nuh * hyp;
-    if (k == 0) f += wind[y];
+    if (km == 0) f += wind[y];
     if (bottom) {
       const T lap1 = (jpl[1] + jpr[1] + jpc[0] + jpc[2] - T(4) * jpc[1]) * inv_d2;
       const T jt = arakawa12<T>(jpl[1], jpr[1], jpc[0], jpc[2], jpl[0], jpr[0], jpl[2], jpr[2], jtl[1], jtr[1],

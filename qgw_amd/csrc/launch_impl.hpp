@@ -67,16 +67,17 @@ constexpr int kTR = (sizeof(T) == 8) ? 16 : 32;
 template <int NLC>
 static void tproj_go(const LaunchArgs &a, const void *src, void *dst, const void *M, int R, int C, int Ps, long Ss,
                      int Pd, long Sd) {
-  const dim3 grid((C + kTC - 1) / kTC, (R + kTR - 1) / kTR);
+  const dim3 grid((C + kTC - 1) / kTC, (R + kTR - 1) / kTR, a.g.nl / a.g.nlm);
   hipLaunchKernelGGL((k_tproj<T, kTC, kTR, NLC>), grid, dim3(256), 0, a.stream, (const T *)src, (T *)dst,
-                     (const T *)M, a.g.nl, R, C, Ps, Ss, Pd, Sd);
+                     (const T *)M, a.g.nlm, R, C, Ps, Ss, Pd, Sd);
 }
 
 static int tproj(const LaunchArgs &a, const void *src, void *dst, const void *M, int R, int C, int Ps, long Ss,
                  int Pd, long Sd) {
-  if (a.g.nl <= 4)
+  if (a.g.nlm < 1 || a.g.nl % a.g.nlm != 0) return QGW_ERR_ARG;
+  if (a.g.nlm <= 4)
     tproj_go<4>(a, src, dst, M, R, C, Ps, Ss, Pd, Sd);
-  else if (a.g.nl <= kMaxLayers)
+  else if (a.g.nlm <= kMaxLayers)
     tproj_go<kMaxLayers>(a, src, dst, M, R, C, Ps, Ss, Pd, Sd);
   else
     return QGW_ERR_ARG;

@@ -35,8 +35,16 @@ bool supported_length(int L) {
 
 }  // namespace
 
+// One group of ensemble members of equal shape and step scalars (a plain run
+// is a group of one).  Members are stacked along the layer axis: member b's
+// layer l is layer b*nl + l of every field, and each member has its own
+// modes, wind row and topography.
 struct qgw_ctx {
-  qgw_params p;
+  qgw_params p;                 // member 0; the scalars every member shares
+  int nb;                       // members
+  std::vector<qgw_params> mp;   // per-member parameters
+  std::vector<char> mset;       // per-member: modes set
+  bool own_stream;
   int prec;
   size_t esz;
   qgw::Geo g;
@@ -203,7 +211,7 @@ void release(qgw_ctx *c) {
     hipEventDestroy(r.b);
   }
   for (auto e : c->evpool) hipEventDestroy(e);
-  if (c->stream) hipStreamDestroy(c->stream);
+  if (c->stream && c->own_stream) hipStreamDestroy(c->stream);
   delete c;
 }
 
@@ -389,37 +397,186 @@ int one_step(qgw_ctx *c) {
   return QGW_OK;
 }
 
-}  // namespace
+// Members that may share one group: equal grid and the scalars that enter
+// the step kernels, and a fixed step (an adaptive step is per run).
+bool same_group(const qgw_params &a, const qgw_params &b) {
+  if (a.dt_fixed <= 0 || b.dt_fixed <= 0) return false;
+  return a.NX == b.NX && a.NY == b.NY && a.nl == b.nl && a.Lx == b.Lx && a.n_hyper == b.n_hyper &&
+         a.nu_hyper == b.nu_hyper && a.nu == b.nu && a.beta == b.beta && a.f0 == b.f0 && a.hEkb == b.hEkb &&
+         a.dh[a.nl - 1] == b.dh[b.nl - 1] && a.bc_fac == b.bc_fac && a.dt_fixed == b.dt_fixed;
+}
 
-extern "C" {
+int check_params(const qgw_params *p, qgw_grid *grid, int *Lx, int *Ly) {
+  if (p->nl < 1 || p->nl > qgw::kMaxLayers) return QGW_ERR_ARG;
+  if (p->n_hyper < 0 || (p->n_hyper % 2) != 0 || p->n_hyper > 8) return QGW_ERR_ARG;
+  if (p->k_f != 0 || p->dt_forc_period != 0) return QGW_ERR_ARG;  // stochastic / file forcing not provided
+  if (qgwh_grid(p, grid, nullptr, nullptr, nullptr, nullptr) != QGW_OK) return QGW_ERR_ARG;
+  *Lx = 2 * (grid->n + 1);
+  *Ly = 2 * (grid->m + 1);
+  if (!supported_length(*Lx) || !supported_length(*Ly)) return QGW_ERR_ARG;
+  return QGW_OK;
+}
 
-qgw_ctx *qgw_create(const qgw_params *p, int precision, int *status) {
-  int dummy;
-  if (!status) status = &dummy;
+void *member_field(qgw_ctx *c, int member, int field, int *layers, void **resid) {
+  *resid = nullptr;
+  *layers = c->p.nl;
+  const size_t lay = c->esz * (size_t)c->g.LS;  // bytes per layer
+  const size_t off = lay * (size_t)c->p.nl * member;
+  switch (field) {
+    case QGW_FIELD_Q:
+      if (c->prec == QGW_FP32) *resid = (char *)c->comp + off;
+      return (char *)c->q[c->qcur] + off;
+    case QGW_FIELD_PSI: return (char *)c->psi + off;
+    case QGW_FIELD_TOPO: *layers = 1; return (char *)c->topo + lay * member;
+    case QGW_FIELD_F0: return (char *)c->fa[0] + off;
+    case QGW_FIELD_F1: return (char *)c->fa[1] + off;
+    case QGW_FIELD_F2: return (char *)c->fa[2] + off;
+  }
+  return nullptr;
+}
+
+int ensure_stage(qgw_ctx *c) {
+  if (c->stage) return QGW_OK;
+  size_t b = sizeof(double) * (size_t)(c->g.n + 2) * (c->g.m + 2) * c->p.nl;
+  hipError_t e = hipMalloc((void **)&c->stage, b);
+  if (e != hipSuccess) return fail_hip(c, e, "hipMalloc(stage)");
+  c->allocs.push_back(c->stage);
+  c->bytes += b;
+  return QGW_OK;
+}
+
+int member_upload(qgw_ctx *c, int member, int field, const double *host) {
+  if (!c || !host || member < 0 || member >= c->nb) return QGW_ERR_ARG;
+  int layers;
+  void *resid;
+  void *dst = member_field(c, member, field, &layers, &resid);
+  if (!dst) return QGW_ERR_ARG;
+  int st = ensure_stage(c);
+  if (st) return st;
+  size_t cnt = (size_t)(c->g.n + 2) * (c->g.m + 2) * layers;
+  HIPCHK(c, hipMemcpyAsync(c->stage, host, cnt * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  qgw::LaunchArgs a;
+  a.g = c->g;
+  a.prec = c->prec;
+  a.stream = c->stream;
+  st = qgw::launch_from_f64(a, layers, c->stage, dst, resid);
+  if (st || (st = check_launch(c, "from_f64"))) return st;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return QGW_OK;
+}
+
+int member_download(qgw_ctx *c, int member, int field, double *host) {
+  if (!c || !host || member < 0 || member >= c->nb) return QGW_ERR_ARG;
+  int layers;
+  void *resid;
+  void *src = member_field(c, member, field, &layers, &resid);
+  if (!src) return QGW_ERR_ARG;
+  int st = ensure_stage(c);
+  if (st) return st;
+  qgw::LaunchArgs a;
+  a.g = c->g;
+  a.prec = c->prec;
+  a.stream = c->stream;
+  st = qgw::launch_to_f64(a, layers, src, resid, c->stage);
+  if (st || (st = check_launch(c, "to_f64"))) return st;
+  size_t cnt = (size_t)(c->g.n + 2) * (c->g.m + 2) * layers;
+  HIPCHK(c, hipMemcpyAsync(host, c->stage, cnt * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return QGW_OK;
+}
+
+int member_set_modes(qgw_ctx *c, int member, const double *cl2m, const double *cm2l, const double *iRd2) {
+  if (!c || !cl2m || !cm2l || !iRd2 || member < 0 || member >= c->nb) return QGW_ERR_ARG;
+  const int nl = c->p.nl;
+  for (int i = 0; i < nl * nl; i++)
+    if (!isfinite(cl2m[i]) || !isfinite(cm2l[i])) return QGW_ERR_ARG;
+  for (int i = 0; i < nl; i++)
+    if (!isfinite(iRd2[i])) return QGW_ERR_ARG;
+  const size_t mo = c->esz * (size_t)nl * nl * member;
+  int st = upload_typed(c, (char *)c->cl2m + mo, cl2m, (size_t)nl * nl);
+  if (st) return st;
+  st = upload_typed(c, (char *)c->cm2l + mo, cm2l, (size_t)nl * nl);
+  if (st) return st;
+  HIPCHK(c, hipMemcpy(c->iRd2_d + (size_t)nl * member, iRd2, sizeof(double) * nl, hipMemcpyHostToDevice));
+  c->mset[member] = 1;
+  bool all = true;
+  for (int b = 0; b < c->nb; b++) all = all && c->mset[b];
+  c->modes_set = all;
+  return QGW_OK;
+}
+
+// Energy and enstrophy of every member of the group (one reduction launch).
+int group_diag(qgw_ctx *c, std::vector<double> &h) {
+  qgw::LaunchArgs a;
+  a.g = c->g;
+  a.prec = c->prec;
+  a.stream = c->stream;
+  a.psi = c->psi;
+  a.q = c->q[c->qcur];
+  a.comp = (c->prec == QGW_FP32) ? c->comp : nullptr;
+  int st;
+  {
+    PassScope ps(c, QGW_PASS_DIAG);
+    st = qgw::launch_diag(a, c->diag_part, c->nblk_diag, c->diag_out);
+  }
+  if (st || (st = check_launch(c, "diag"))) return st;
+  h.assign(2 * (size_t)c->g.nl, 0.0);
+  HIPCHK(c, hipMemcpyAsync(h.data(), c->diag_out, sizeof(double) * 2 * c->g.nl, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return QGW_OK;
+}
+
+void member_energy(const qgw_ctx *c, int member, const double *h, double *E_layer, double *Z_layer, double *E_tot,
+                   double *Z_tot) {
+  const qgw_params &p = c->mp[member];
+  const int nl = p.nl;
+  const double *hm = h + 2 * (size_t)nl * member;
+  const double w = c->grid.Delta * c->grid.Delta / (p.Lx * p.Lx);
+  double H = 0, Et = 0, Zt = 0;
+  for (int l = 0; l < nl; l++) H += p.dh[l];
+  for (int l = 0; l < nl; l++) {
+    const double e = -hm[2 * l] * w, z = hm[2 * l + 1] * w;
+    if (E_layer) E_layer[l] = e;
+    if (Z_layer) Z_layer[l] = z;
+    Et += p.dh[l] * e;
+    Zt += p.dh[l] * z;
+  }
+  if (E_tot) *E_tot = Et / (2 * H);
+  if (Z_tot) *Z_tot = Zt / H;
+}
+
+// A group of nb members (ps[0..nb-1], all same_group, or nb = 1) on stream
+// `ext` (owned by the caller) or on a stream of its own (ext = null).
+qgw_ctx *create_group(const qgw_params *ps, int nb, int precision, hipStream_t ext, int *status) {
   *status = QGW_ERR_ARG;
-  if (!p) return nullptr;
+  if (!ps || nb < 1) return nullptr;
   if (precision != QGW_FP64 && precision != QGW_FP32 && precision != QGW_FP32_PLAIN) return nullptr;
-  if (p->nl < 1 || p->nl > qgw::kMaxLayers) return nullptr;
-  if (p->n_hyper < 0 || (p->n_hyper % 2) != 0 || p->n_hyper > 8) return nullptr;
-  if (p->k_f != 0 || p->dt_forc_period != 0) return nullptr;  // stochastic / file forcing not provided
+  const qgw_params *p = &ps[0];
   qgw_grid grid;
-  if (qgwh_grid(p, &grid, nullptr, nullptr, nullptr, nullptr) != QGW_OK) return nullptr;
-  const int Lx = 2 * (grid.n + 1), Ly = 2 * (grid.m + 1);
-  if (!supported_length(Lx) || !supported_length(Ly)) return nullptr;
+  int Lx, Ly;
+  if (check_params(p, &grid, &Lx, &Ly) != QGW_OK) return nullptr;
+  for (int b = 1; b < nb; b++)
+    if (!same_group(ps[0], ps[b])) return nullptr;
+  const int n = grid.n, m = grid.m, nl = p->nl;
+  if ((long)nb * nl > (1L << 20)) return nullptr;
 
   qgw_ctx *c = new qgw_ctx();
   memset(c->err, 0, sizeof c->err);
   c->p = *p;
+  c->nb = nb;
+  c->mp.assign(ps, ps + nb);
+  c->mset.assign(nb, 0);
   c->prec = precision;
   c->esz = (precision == QGW_FP64) ? 8 : 4;
   c->grid = grid;
   c->Lx = Lx;
   c->Ly = Ly;
-  const int n = grid.n, m = grid.m, nl = p->nl;
+  const int NL = nb * nl;  // layers of all members
   qgw::Geo &g = c->g;
   g.n = n;
   g.m = m;
-  g.nl = nl;
+  g.nl = NL;
+  g.nlm = nl;
   g.P = pitch_of(n + 2);
   g.LS = (long)(m + 2) * g.P;
   g.Pn = pitch_of(n);
@@ -427,29 +584,36 @@ qgw_ctx *qgw_create(const qgw_params *p, int precision, int *status) {
   g.Pm = pitch_of(m);
   g.WTS = (long)n * g.Pm;
   *status = QGW_ERR_HIP;
-  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
-    release(c);
-    return nullptr;
+  if (ext) {
+    c->stream = ext;
+    c->own_stream = false;
+  } else {
+    c->own_stream = true;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+      c->stream = nullptr;
+      release(c);
+      return nullptr;
+    }
   }
-  const size_t field = c->esz * (size_t)g.LS * nl;
+  const size_t field = c->esz * (size_t)g.LS * NL;
   int st = QGW_OK;
   st = st ? st : dalloc_field(c, &c->q[0], field);
   st = st ? st : dalloc_field(c, &c->q[1], field);
   st = st ? st : dalloc_field(c, &c->psi, field);
   for (int i = 0; i < 3; i++) st = st ? st : dalloc_field(c, &c->fa[i], field);
   if (precision == QGW_FP32) st = st ? st : dalloc_field(c, &c->comp, field);
-  st = st ? st : dalloc_field(c, &c->topo, c->esz * (size_t)g.LS);
-  st = st ? st : dalloc(c, &c->wind, c->esz * (size_t)(m + 2));
-  st = st ? st : dalloc(c, &c->what, c->esz * (size_t)g.WS * nl);
-  st = st ? st : dalloc(c, &c->whatT, c->esz * (size_t)g.WTS * nl);
-  st = st ? st : dalloc(c, &c->cl2m, c->esz * (size_t)nl * nl);
-  st = st ? st : dalloc(c, &c->cm2l, c->esz * (size_t)nl * nl);
-  st = st ? st : dalloc(c, (void **)&c->iRd2_d, sizeof(double) * nl);
+  st = st ? st : dalloc_field(c, &c->topo, c->esz * (size_t)g.LS * nb);
+  st = st ? st : dalloc(c, &c->wind, c->esz * (size_t)(m + 2) * nb);
+  st = st ? st : dalloc(c, &c->what, c->esz * (size_t)g.WS * NL);
+  st = st ? st : dalloc(c, &c->whatT, c->esz * (size_t)g.WTS * NL);
+  st = st ? st : dalloc(c, &c->cl2m, c->esz * (size_t)nl * nl * nb);
+  st = st ? st : dalloc(c, &c->cm2l, c->esz * (size_t)nl * nl * nb);
+  st = st ? st : dalloc(c, (void **)&c->iRd2_d, sizeof(double) * NL);
   st = st ? st : dalloc(c, (void **)&c->La2, sizeof(double) * (n + 2));
   st = st ? st : dalloc(c, (void **)&c->Kb2, sizeof(double) * (m + 2));
   c->nblk_diag = 256;
-  st = st ? st : dalloc(c, (void **)&c->diag_part, sizeof(double) * 2 * c->nblk_diag * nl);
-  st = st ? st : dalloc(c, (void **)&c->diag_out, sizeof(double) * 2 * nl);
+  st = st ? st : dalloc(c, (void **)&c->diag_part, sizeof(double) * 2 * c->nblk_diag * NL);
+  st = st ? st : dalloc(c, (void **)&c->diag_out, sizeof(double) * 2 * NL);
   st = st ? st : dalloc(c, (void **)&c->cfl_max, sizeof(unsigned long long));
   if (!st) {
     if (precision == QGW_FP64) {
@@ -476,20 +640,22 @@ qgw_ctx *qgw_create(const qgw_params *p, int precision, int *status) {
       release(c);
       return nullptr;
     }
-    std::vector<double> wind(m + 2);
-    qgwh_wind_row(p, wind.data());
-    if (upload_typed(c, c->wind, wind.data(), m + 2)) {
+  }
+  // per-member wind row and topography (src/domain.h:33,200-207)
+  {
+    std::vector<double> wind((size_t)(m + 2) * nb);
+    for (int b = 0; b < nb; b++) qgwh_wind_row(&ps[b], wind.data() + (size_t)(m + 2) * b);
+    if (upload_typed(c, c->wind, wind.data(), wind.size())) {
       release(c);
       return nullptr;
     }
-  }
-  // topography (src/domain.h:200-207)
-  {
     std::vector<double> topo((size_t)(n + 2) * (m + 2));
-    qgwh_topo(p, topo.data());
-    if (qgw_upload(c, QGW_FIELD_TOPO, topo.data()) != QGW_OK) {
-      release(c);
-      return nullptr;
+    for (int b = 0; b < nb; b++) {
+      qgwh_topo(&ps[b], topo.data());
+      if (member_upload(c, b, QGW_FIELD_TOPO, topo.data()) != QGW_OK) {
+        release(c);
+        return nullptr;
+      }
     }
   }
   // time state (init_vars sign rule, check_timestep, init_timestep)
@@ -513,6 +679,254 @@ qgw_ctx *qgw_create(const qgw_params *p, int precision, int *status) {
   return c;
 }
 
+// Device copy of a group's state (fields, rotation, time), so that timing
+// runs can be undone.
+struct Snapshot {
+  void *buf = nullptr;
+  int qcur = 0;
+  void *fa[3] = {nullptr, nullptr, nullptr};
+  double t = 0, dt = 0, dt0 = 0, dt1 = 0;
+  int it = 0;
+};
+
+std::vector<void *> state_fields(qgw_ctx *c) {
+  std::vector<void *> f = {c->q[0], c->q[1], c->psi, c->fa[0], c->fa[1], c->fa[2]};
+  if (c->comp) f.push_back(c->comp);
+  return f;
+}
+
+int snapshot_take(qgw_ctx *c, Snapshot &s) {
+  const size_t field = c->esz * (size_t)c->g.LS * c->g.nl;
+  auto f = state_fields(c);
+  HIPCHK(c, hipMalloc(&s.buf, field * f.size()));
+  for (size_t i = 0; i < f.size(); i++)
+    HIPCHK(c, hipMemcpyAsync((char *)s.buf + i * field, f[i], field, hipMemcpyDeviceToDevice, c->stream));
+  s.qcur = c->qcur;
+  for (int i = 0; i < 3; i++) s.fa[i] = c->fa[i];
+  s.t = c->t;
+  s.dt = c->dt;
+  s.dt0 = c->dt0;
+  s.dt1 = c->dt1;
+  s.it = c->it;
+  return QGW_OK;
+}
+
+int snapshot_restore(qgw_ctx *c, Snapshot &s) {
+  const size_t field = c->esz * (size_t)c->g.LS * c->g.nl;
+  c->qcur = s.qcur;
+  for (int i = 0; i < 3; i++) c->fa[i] = s.fa[i];
+  c->t = s.t;
+  c->dt = s.dt;
+  c->dt0 = s.dt0;
+  c->dt1 = s.dt1;
+  c->it = s.it;
+  auto f = state_fields(c);
+  for (size_t i = 0; i < f.size(); i++)
+    HIPCHK(c, hipMemcpyAsync(f[i], (char *)s.buf + i * field, field, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipFree(s.buf));
+  s.buf = nullptr;
+  return QGW_OK;
+}
+
+int time_steps(qgw_ctx *c, int nsteps, double *ms) {
+  hipEvent_t a, b;
+  HIPCHK(c, hipEventCreate(&a));
+  HIPCHK(c, hipEventCreate(&b));
+  int st = QGW_OK;
+  if (hipEventRecord(a, c->stream) != hipSuccess) st = QGW_ERR_HIP;
+  for (int i = 0; i < nsteps && !st; i++) st = one_step(c);
+  if (!st && hipEventRecord(b, c->stream) != hipSuccess) st = QGW_ERR_HIP;
+  if (!st && hipEventSynchronize(b) != hipSuccess) st = QGW_ERR_HIP;
+  float f = 0.f;
+  if (!st && hipEventElapsedTime(&f, a, b) != hipSuccess) st = QGW_ERR_HIP;
+  *ms = f;
+  hipEventDestroy(a);
+  hipEventDestroy(b);
+  return st;
+}
+
+// Time `reps` steps with each P4 variant and keep the fastest; the state is
+// restored afterwards.
+int autotune_group(qgw_ctx *c, int reps, int *p4_tile, double *best_ms) {
+  Snapshot snap;
+  int st = snapshot_take(c, snap);
+  if (st) return st;
+  double best = 1e300;
+  int arg = c->p4_tile;
+  for (int v = 0; v < qgw::kP4Variants && !st; v++) {
+    c->p4_tile = v;
+    double ms = 0;
+    st = time_steps(c, 2, &ms);  // warm-up (and first-launch code loading)
+    if (!st) st = time_steps(c, reps, &ms);
+    if (!st && ms < best) {
+      best = ms;
+      arg = v;
+    }
+  }
+  int st2 = snapshot_restore(c, snap);
+  if (st) return st;
+  if (st2) return st2;
+  c->p4_tile = arg;
+  if (p4_tile) *p4_tile = arg;
+  if (best_ms) *best_ms = best / reps;
+  return QGW_OK;
+}
+
+}  // namespace
+
+// An ensemble: groups of members (each a qgw_ctx stacking its members along
+// the layer axis) on one stream.  A fixed step is periodic in the buffer
+// rotation (q alternates between two buffers, the tendency history cycles
+// through three), so kGraphSteps = 6 steps of every group are captured once
+// into a hipGraph and replayed; the graph is keyed by the rotation state it
+// starts from.
+struct qgw_batch {
+  int nmem = 0;
+  int prec = 0;
+  hipStream_t stream = nullptr;
+  std::vector<qgw_ctx *> groups;
+  std::vector<int> gof, iof;  // member -> group, index inside the group
+  bool use_graph = true;
+  struct Cached {
+    std::vector<const void *> sig;
+    hipGraphExec_t exec;
+  };
+  std::vector<Cached> cache;
+  int graph_launches = 0;
+  char err[256];
+};
+
+namespace {
+
+constexpr int kGraphSteps = 6;
+
+int bfail(qgw_batch *b, hipError_t e, const char *what) {
+  snprintf(b->err, sizeof b->err, "%s: %s", what, hipGetErrorString(e));
+  return QGW_ERR_HIP;
+}
+
+int bpass(qgw_batch *b, qgw_ctx *c, int st) {
+  if (st && c) snprintf(b->err, sizeof b->err, "%s", c->err);
+  return st;
+}
+
+std::vector<const void *> batch_sig(const qgw_batch *b) {
+  std::vector<const void *> sig;
+  for (auto *c : b->groups) {
+    sig.push_back(c->fa[0]);
+    sig.push_back(c->fa[1]);
+    sig.push_back(c->fa[2]);
+    sig.push_back(c->q[c->qcur]);
+  }
+  return sig;
+}
+
+bool graph_ok(const qgw_batch *b) {
+  if (!b->use_graph) return false;
+  for (auto *c : b->groups) {
+    const double dt = c->p.dt_fixed;
+    if (c->prof || !(dt > 0) || c->dt != dt || c->dt0 != dt || c->dt1 != dt || !c->modes_set) return false;
+  }
+  return true;
+}
+
+int batch_eager_step(qgw_batch *b) {
+  for (auto *c : b->groups) {
+    int st = one_step(c);
+    if (st) return bpass(b, c, st);
+  }
+  return QGW_OK;
+}
+
+void batch_release_graphs(qgw_batch *b) {
+  for (auto &e : b->cache) hipGraphExecDestroy(e.exec);
+  b->cache.clear();
+}
+
+// Capture kGraphSteps steps of every group from the current rotation state.
+// Nothing runs during capture, so the host time state is put back after it.
+int batch_capture(qgw_batch *b, hipGraphExec_t *exec) {
+  struct T {
+    double t;
+    int it;
+  };
+  std::vector<T> keep;
+  for (auto *c : b->groups) keep.push_back({c->t, c->it});
+  const auto sig0 = batch_sig(b);
+  hipError_t e = hipStreamBeginCapture(b->stream, hipStreamCaptureModeRelaxed);
+  if (e != hipSuccess) return bfail(b, e, "hipStreamBeginCapture");
+  int st = QGW_OK;
+  for (int s = 0; s < kGraphSteps && !st; s++) st = batch_eager_step(b);
+  hipGraph_t graph = nullptr;
+  e = hipStreamEndCapture(b->stream, &graph);
+  for (size_t i = 0; i < b->groups.size(); i++) {
+    b->groups[i]->t = keep[i].t;
+    b->groups[i]->it = keep[i].it;
+  }
+  if (st) {
+    if (graph) hipGraphDestroy(graph);
+    return st;
+  }
+  if (e != hipSuccess) return bfail(b, e, "hipStreamEndCapture");
+  if (batch_sig(b) != sig0) {
+    hipGraphDestroy(graph);
+    snprintf(b->err, sizeof b->err, "buffer rotation is not periodic over %d steps", kGraphSteps);
+    return QGW_ERR_STATE;
+  }
+  e = hipGraphInstantiateWithFlags(exec, graph, 0);
+  hipGraphDestroy(graph);
+  if (e != hipSuccess) return bfail(b, e, "hipGraphInstantiate");
+  return QGW_OK;
+}
+
+int batch_run(qgw_batch *b, int nsteps) {
+  while (nsteps >= kGraphSteps && graph_ok(b)) {
+    const auto sig = batch_sig(b);
+    hipGraphExec_t exec = nullptr;
+    for (auto &e : b->cache)
+      if (e.sig == sig) exec = e.exec;
+    if (!exec) {
+      int st = batch_capture(b, &exec);
+      if (st) return st;
+      b->cache.push_back({sig, exec});
+    }
+    hipError_t e = hipGraphLaunch(exec, b->stream);
+    if (e != hipSuccess) return bfail(b, e, "hipGraphLaunch");
+    b->graph_launches++;
+    for (auto *c : b->groups)
+      for (int s = 0; s < kGraphSteps; s++) {
+        c->t = c->t + c->dt;
+        c->it++;
+      }
+    nsteps -= kGraphSteps;
+  }
+  for (int s = 0; s < nsteps; s++) {
+    int st = batch_eager_step(b);
+    if (st) return st;
+  }
+  return QGW_OK;
+}
+
+void batch_free(qgw_batch *b) {
+  if (!b) return;
+  if (b->stream) hipStreamSynchronize(b->stream);
+  batch_release_graphs(b);
+  for (auto *c : b->groups) release(c);
+  if (b->stream) hipStreamDestroy(b->stream);
+  delete b;
+}
+
+}  // namespace
+
+extern "C" {
+
+qgw_ctx *qgw_create(const qgw_params *p, int precision, int *status) {
+  int dummy;
+  if (!status) status = &dummy;
+  return create_group(p, 1, precision, nullptr, status);
+}
+
 int qgw_destroy(qgw_ctx *c) {
   if (!c) return QGW_ERR_ARG;
   if (c->stream) hipStreamSynchronize(c->stream);
@@ -521,86 +935,12 @@ int qgw_destroy(qgw_ctx *c) {
 }
 
 int qgw_set_modes(qgw_ctx *c, const double *cl2m, const double *cm2l, const double *iRd2) {
-  if (!c || !cl2m || !cm2l || !iRd2) return QGW_ERR_ARG;
-  const int nl = c->p.nl;
-  for (int i = 0; i < nl * nl; i++)
-    if (!isfinite(cl2m[i]) || !isfinite(cm2l[i])) return QGW_ERR_ARG;
-  for (int i = 0; i < nl; i++)
-    if (!isfinite(iRd2[i])) return QGW_ERR_ARG;
-  int st = upload_typed(c, c->cl2m, cl2m, (size_t)nl * nl);
-  if (st) return st;
-  st = upload_typed(c, c->cm2l, cm2l, (size_t)nl * nl);
-  if (st) return st;
-  HIPCHK(c, hipMemcpy(c->iRd2_d, iRd2, sizeof(double) * nl, hipMemcpyHostToDevice));
-  c->modes_set = true;
-  return QGW_OK;
+  return member_set_modes(c, 0, cl2m, cm2l, iRd2);
 }
 
-static void *field_ptr(qgw_ctx *c, int field, int *layers, void **resid) {
-  *resid = nullptr;
-  *layers = c->p.nl;
-  switch (field) {
-    case QGW_FIELD_Q:
-      if (c->prec == QGW_FP32) *resid = c->comp;
-      return c->q[c->qcur];
-    case QGW_FIELD_PSI: return c->psi;
-    case QGW_FIELD_TOPO: *layers = 1; return c->topo;
-    case QGW_FIELD_F0: return c->fa[0];
-    case QGW_FIELD_F1: return c->fa[1];
-    case QGW_FIELD_F2: return c->fa[2];
-  }
-  return nullptr;
-}
+int qgw_upload(qgw_ctx *c, int field, const double *host) { return member_upload(c, 0, field, host); }
 
-static int ensure_stage(qgw_ctx *c) {
-  if (c->stage) return QGW_OK;
-  size_t b = sizeof(double) * (size_t)(c->g.n + 2) * (c->g.m + 2) * c->p.nl;
-  hipError_t e = hipMalloc((void **)&c->stage, b);
-  if (e != hipSuccess) return fail_hip(c, e, "hipMalloc(stage)");
-  c->allocs.push_back(c->stage);
-  c->bytes += b;
-  return QGW_OK;
-}
-
-int qgw_upload(qgw_ctx *c, int field, const double *host) {
-  if (!c || !host) return QGW_ERR_ARG;
-  int layers;
-  void *resid;
-  void *dst = field_ptr(c, field, &layers, &resid);
-  if (!dst) return QGW_ERR_ARG;
-  int st = ensure_stage(c);
-  if (st) return st;
-  size_t cnt = (size_t)(c->g.n + 2) * (c->g.m + 2) * layers;
-  HIPCHK(c, hipMemcpyAsync(c->stage, host, cnt * sizeof(double), hipMemcpyHostToDevice, c->stream));
-  qgw::LaunchArgs a;
-  a.g = c->g;
-  a.prec = c->prec;
-  a.stream = c->stream;
-  st = qgw::launch_from_f64(a, layers, c->stage, dst, resid);
-  if (st || (st = check_launch(c, "from_f64"))) return st;
-  HIPCHK(c, hipStreamSynchronize(c->stream));
-  return QGW_OK;
-}
-
-int qgw_download(qgw_ctx *c, int field, double *host) {
-  if (!c || !host) return QGW_ERR_ARG;
-  int layers;
-  void *resid;
-  void *src = field_ptr(c, field, &layers, &resid);
-  if (!src) return QGW_ERR_ARG;
-  int st = ensure_stage(c);
-  if (st) return st;
-  qgw::LaunchArgs a;
-  a.g = c->g;
-  a.prec = c->prec;
-  a.stream = c->stream;
-  st = qgw::launch_to_f64(a, layers, src, resid, c->stage);
-  if (st || (st = check_launch(c, "to_f64"))) return st;
-  size_t cnt = (size_t)(c->g.n + 2) * (c->g.m + 2) * layers;
-  HIPCHK(c, hipMemcpyAsync(host, c->stage, cnt * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(c, hipStreamSynchronize(c->stream));
-  return QGW_OK;
-}
+int qgw_download(qgw_ctx *c, int field, double *host) { return member_download(c, 0, field, host); }
 
 int qgw_invert_pv(qgw_ctx *c) {
   if (!c) return QGW_ERR_ARG;
@@ -645,35 +985,10 @@ int qgw_run(qgw_ctx *c, int nsteps) {
 
 int qgw_diagnostics(qgw_ctx *c, double *E_layer, double *Z_layer, double *E_tot, double *Z_tot) {
   if (!c) return QGW_ERR_ARG;
-  const int nl = c->p.nl;
-  qgw::LaunchArgs a;
-  a.g = c->g;
-  a.prec = c->prec;
-  a.stream = c->stream;
-  a.psi = c->psi;
-  a.q = c->q[c->qcur];
-  a.comp = (c->prec == QGW_FP32) ? c->comp : nullptr;
-  int st;
-  {
-    PassScope ps(c, QGW_PASS_DIAG);
-    st = qgw::launch_diag(a, c->diag_part, c->nblk_diag, c->diag_out);
-  }
-  if (st || (st = check_launch(c, "diag"))) return st;
-  std::vector<double> h(2 * nl);
-  HIPCHK(c, hipMemcpyAsync(h.data(), c->diag_out, sizeof(double) * 2 * nl, hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(c, hipStreamSynchronize(c->stream));
-  const double w = c->grid.Delta * c->grid.Delta / (c->p.Lx * c->p.Lx);
-  double H = 0, Et = 0, Zt = 0;
-  for (int l = 0; l < nl; l++) H += c->p.dh[l];
-  for (int l = 0; l < nl; l++) {
-    const double e = -h[2 * l] * w, z = h[2 * l + 1] * w;
-    if (E_layer) E_layer[l] = e;
-    if (Z_layer) Z_layer[l] = z;
-    Et += c->p.dh[l] * e;
-    Zt += c->p.dh[l] * z;
-  }
-  if (E_tot) *E_tot = Et / (2 * H);
-  if (Z_tot) *Z_tot = Zt / H;
+  std::vector<double> h;
+  int st = group_diag(c, h);
+  if (st) return st;
+  member_energy(c, 0, h.data(), E_layer, Z_layer, E_tot, Z_tot);
   return QGW_OK;
 }
 
@@ -772,24 +1087,7 @@ int qgw_tune_get(qgw_ctx *c, int knob, int *value) {
 
 int qgw_autotune(qgw_ctx *c, int reps, int *p4_tile, double *best_ms) {
   if (!c || reps < 1) return QGW_ERR_ARG;
-  double best = 1e300;
-  int arg = c->p4_tile;
-  for (int v = 0; v < qgw::kP4Variants; v++) {
-    c->p4_tile = v;
-    int st = qgw_run(c, 2);  // warm-up (and first-launch code loading)
-    if (st) return st;
-    double ms = 0;
-    st = qgw_time_run(c, reps, &ms);
-    if (st) return st;
-    if (ms < best) {
-      best = ms;
-      arg = v;
-    }
-  }
-  c->p4_tile = arg;
-  if (p4_tile) *p4_tile = arg;
-  if (best_ms) *best_ms = best / reps;
-  return QGW_OK;
+  return autotune_group(c, reps, p4_tile, best_ms);
 }
 
 int qgw_device_bytes(qgw_ctx *c, size_t *bytes) {
@@ -801,5 +1099,219 @@ int qgw_device_bytes(qgw_ctx *c, size_t *bytes) {
 const char *qgw_strerror(int status) { return qgwh_strerror(status); }
 
 const char *qgw_last_error(qgw_ctx *c) { return c ? c->err : ""; }
+
+
+qgw_batch *qgw_batch_create(int nmembers, const qgw_params *params, int precision, int *status) {
+  int dummy;
+  if (!status) status = &dummy;
+  *status = QGW_ERR_ARG;
+  if (nmembers < 1 || !params) return nullptr;
+  for (int i = 0; i < nmembers; i++)
+    if (!(params[i].dt_fixed > 0)) return nullptr;  // the graph step needs a fixed dt
+  qgw_batch *b = new qgw_batch();
+  memset(b->err, 0, sizeof b->err);
+  b->nmem = nmembers;
+  b->prec = precision;
+  b->gof.assign(nmembers, -1);
+  b->iof.assign(nmembers, -1);
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+    b->stream = nullptr;
+    *status = QGW_ERR_HIP;
+    batch_free(b);
+    return nullptr;
+  }
+  // group members in order of first appearance
+  std::vector<std::vector<int>> members;
+  for (int i = 0; i < nmembers; i++) {
+    int gi = -1;
+    for (size_t k = 0; k < members.size() && gi < 0; k++)
+      if (same_group(params[members[k][0]], params[i])) gi = (int)k;
+    if (gi < 0) {
+      gi = (int)members.size();
+      members.push_back({});
+    }
+    b->gof[i] = gi;
+    b->iof[i] = (int)members[gi].size();
+    members[gi].push_back(i);
+  }
+  for (auto &mem : members) {
+    std::vector<qgw_params> ps;
+    for (int i : mem) ps.push_back(params[i]);
+    int st = QGW_OK;
+    qgw_ctx *c = create_group(ps.data(), (int)ps.size(), precision, b->stream, &st);
+    if (!c) {
+      *status = st;
+      batch_free(b);
+      return nullptr;
+    }
+    b->groups.push_back(c);
+  }
+  *status = QGW_OK;
+  return b;
+}
+
+int qgw_batch_destroy(qgw_batch *b) {
+  if (!b) return QGW_ERR_ARG;
+  batch_free(b);
+  return QGW_OK;
+}
+
+int qgw_batch_groups(qgw_batch *b, int *ngroups, int *group_of_member) {
+  if (!b) return QGW_ERR_ARG;
+  if (ngroups) *ngroups = (int)b->groups.size();
+  if (group_of_member)
+    for (int i = 0; i < b->nmem; i++) group_of_member[i] = b->gof[i];
+  return QGW_OK;
+}
+
+int qgw_batch_set_modes(qgw_batch *b, int member, const double *cl2m, const double *cm2l, const double *iRd2) {
+  if (!b || member < 0 || member >= b->nmem) return QGW_ERR_ARG;
+  qgw_ctx *c = b->groups[b->gof[member]];
+  return bpass(b, c, member_set_modes(c, b->iof[member], cl2m, cm2l, iRd2));
+}
+
+int qgw_batch_upload(qgw_batch *b, int member, int field, const double *host) {
+  if (!b || member < 0 || member >= b->nmem) return QGW_ERR_ARG;
+  qgw_ctx *c = b->groups[b->gof[member]];
+  return bpass(b, c, member_upload(c, b->iof[member], field, host));
+}
+
+int qgw_batch_download(qgw_batch *b, int member, int field, double *host) {
+  if (!b || member < 0 || member >= b->nmem) return QGW_ERR_ARG;
+  qgw_ctx *c = b->groups[b->gof[member]];
+  return bpass(b, c, member_download(c, b->iof[member], field, host));
+}
+
+int qgw_batch_invert_pv(qgw_batch *b) {
+  if (!b) return QGW_ERR_ARG;
+  for (auto *c : b->groups) {
+    int st = run_invert(c);
+    if (st) return bpass(b, c, st);
+  }
+  return QGW_OK;
+}
+
+int qgw_batch_step(qgw_batch *b, int nsteps) {
+  if (!b || nsteps < 0) return QGW_ERR_ARG;
+  return batch_run(b, nsteps);
+}
+
+int qgw_batch_time_step(qgw_batch *b, int nsteps, double *ms) {
+  if (!b || !ms || nsteps < 0) return QGW_ERR_ARG;
+  hipEvent_t e0, e1;
+  hipError_t e = hipEventCreate(&e0);
+  if (e != hipSuccess) return bfail(b, e, "hipEventCreate");
+  e = hipEventCreate(&e1);
+  if (e != hipSuccess) {
+    hipEventDestroy(e0);
+    return bfail(b, e, "hipEventCreate");
+  }
+  int st = QGW_OK;
+  if ((e = hipEventRecord(e0, b->stream)) != hipSuccess) st = bfail(b, e, "hipEventRecord");
+  if (!st) st = batch_run(b, nsteps);
+  if (!st && (e = hipEventRecord(e1, b->stream)) != hipSuccess) st = bfail(b, e, "hipEventRecord");
+  if (!st && (e = hipEventSynchronize(e1)) != hipSuccess) st = bfail(b, e, "hipEventSynchronize");
+  float f = 0.f;
+  if (!st && (e = hipEventElapsedTime(&f, e0, e1)) != hipSuccess) st = bfail(b, e, "hipEventElapsedTime");
+  *ms = f;
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  return st;
+}
+
+int qgw_batch_synchronize(qgw_batch *b) {
+  if (!b) return QGW_ERR_ARG;
+  hipError_t e = hipStreamSynchronize(b->stream);
+  return e == hipSuccess ? QGW_OK : bfail(b, e, "hipStreamSynchronize");
+}
+
+int qgw_batch_diagnostics(qgw_batch *b, int member, double *E_layer, double *Z_layer, double *E_tot,
+                          double *Z_tot) {
+  if (!b || member < 0 || member >= b->nmem) return QGW_ERR_ARG;
+  qgw_ctx *c = b->groups[b->gof[member]];
+  std::vector<double> h;
+  int st = group_diag(c, h);
+  if (st) return bpass(b, c, st);
+  member_energy(c, b->iof[member], h.data(), E_layer, Z_layer, E_tot, Z_tot);
+  return QGW_OK;
+}
+
+int qgw_batch_get_time(qgw_batch *b, double *t, int *it) {
+  if (!b || b->groups.empty()) return QGW_ERR_ARG;
+  if (t) *t = b->groups[0]->t;
+  if (it) *it = b->groups[0]->it;
+  return QGW_OK;
+}
+
+int qgw_batch_use_graph(qgw_batch *b, int on) {
+  if (!b) return QGW_ERR_ARG;
+  b->use_graph = on != 0;
+  return QGW_OK;
+}
+
+int qgw_batch_graph_launches(qgw_batch *b, int *count) {
+  if (!b || !count) return QGW_ERR_ARG;
+  *count = b->graph_launches;
+  return QGW_OK;
+}
+
+int qgw_batch_autotune(qgw_batch *b, int reps, int *p4_tile_per_group) {
+  if (!b || reps < 1) return QGW_ERR_ARG;
+  for (size_t k = 0; k < b->groups.size(); k++) {
+    qgw_ctx *c = b->groups[k];
+    int v = 0;
+    double ms = 0;
+    int st = autotune_group(c, reps, &v, &ms);
+    if (st) return bpass(b, c, st);
+    if (p4_tile_per_group) p4_tile_per_group[k] = v;
+  }
+  batch_release_graphs(b);  // the captured launches used the previous choice
+  return QGW_OK;
+}
+
+int qgw_batch_tune_set(qgw_batch *b, int group, int knob, int value) {
+  if (!b || group < 0 || group >= (int)b->groups.size()) return QGW_ERR_ARG;
+  int st = qgw_tune_set(b->groups[group], knob, value);
+  if (!st) batch_release_graphs(b);
+  return st;
+}
+
+int qgw_batch_profile_enable(qgw_batch *b, int on) {
+  if (!b) return QGW_ERR_ARG;
+  for (auto *c : b->groups) {
+    int st = qgw_profile_enable(c, on);
+    if (st) return bpass(b, c, st);
+  }
+  return QGW_OK;
+}
+
+int qgw_batch_profile_read(qgw_batch *b, double *ms, int *count, int npass) {
+  if (!b) return QGW_ERR_ARG;
+  std::vector<double> m(QGW_NPASS, 0.0), mm(QGW_NPASS);
+  std::vector<int> n(QGW_NPASS, 0), nn(QGW_NPASS);
+  for (auto *c : b->groups) {
+    int st = qgw_profile_read(c, mm.data(), nn.data(), QGW_NPASS);
+    if (st) return bpass(b, c, st);
+    for (int i = 0; i < QGW_NPASS; i++) {
+      m[i] += mm[i];
+      n[i] += nn[i];
+    }
+  }
+  for (int i = 0; i < npass && i < QGW_NPASS; i++) {
+    if (ms) ms[i] = m[i];
+    if (count) count[i] = n[i];
+  }
+  return QGW_OK;
+}
+
+int qgw_batch_device_bytes(qgw_batch *b, size_t *bytes) {
+  if (!b || !bytes) return QGW_ERR_ARG;
+  size_t s = 0;
+  for (auto *c : b->groups) s += c->bytes;
+  *bytes = s;
+  return QGW_OK;
+}
+
+const char *qgw_batch_last_error(qgw_batch *b) { return b ? b->err : ""; }
 
 }  // extern "C"

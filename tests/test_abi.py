@@ -45,3 +45,12 @@ def test_hip_library_is_gfx950():
     so = os.path.join(ROOT, "qgw_amd", "libqgw_hip.so")
     data = open(so, "rb").read()
     assert b"gfx950" in data
+
+
+def test_batch_rejects_adaptive_step():
+    """A batch needs a fixed step in every member; the check precedes any device call."""
+    p = Q.default_params(NX=64, NY=64, nl=2, dt_fixed=0.0)
+    st = ctypes.c_int(0)
+    b = Q.hip_lib().qgw_batch_create(1, ctypes.byref(p), Q.FP32, ctypes.byref(st))
+    assert not b
+    assert st.value == -1
