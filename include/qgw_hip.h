@@ -28,9 +28,10 @@
 extern "C" {
 #endif
 
-/* Largest layer count the device library accepts (the layer<->mode
- * projections of the row passes are register-resident up to this count). */
-#define QGW_DEV_NL_MAX 16
+/* Largest layer count per run the device library accepts (the layer<->mode
+ * projections hold a point's layers in registers, or feed 32 x 32 matrix-core
+ * tiles). */
+#define QGW_DEV_NL_MAX 32
 
 /* Arithmetic forms of the step. */
 #define QGW_FP64 0        /* every field and pass in FP64 */
@@ -145,11 +146,21 @@ int qgw_profile_read(qgw_ctx *ctx, double *ms, int *count, int npass);
  * kernel: 0: 64x8, 1: 64x16, 2: 64x32, 3: 128x16 (columns x rows) LDS
  * tiles, 4: register-streaming wavefront strips (default). */
 #define QGW_TUNE_P4_TILE 1
+/* QGW_TUNE_T_KERNEL selects the layer<->mode projection: 0 automatic (matrix
+ * cores for FP32 at 16 or more layers, else VALU), 1 VALU, 2 matrix cores on
+ * 32-column tiles, 3 matrix cores on 16-column tiles (2 and 3: FP32 forms
+ * only). */
+#define QGW_TUNE_T_KERNEL 2
+/* QGW_TUNE_P4_WAVES sets the row chunk of the streaming P4 so that about
+ * 2, 4, 8, 16 or 32 (default) wavefronts per SIMD are launched. */
+#define QGW_TUNE_P4_WAVES 3
 int qgw_tune_set(qgw_ctx *ctx, int knob, int value);
 int qgw_tune_get(qgw_ctx *ctx, int knob, int *value);
 
-/* Autotuner: times `reps` steps with each P4 variant and keeps the fastest
- * (returned in *p4_tile, its ms per step in *best_ms).  The state (fields,
+/* Autotuner: times `reps` steps with each P4 variant (and, for the streaming
+ * one, each QGW_TUNE_P4_WAVES value) and keeps the fastest (returned in
+ * *p4_tile, its ms per step in *best_ms; the FP32 forms also choose
+ * QGW_TUNE_T_KERNEL).  The state (fields,
  * buffer rotation and time) is copied before the timing runs and restored
  * after them, so the run continues as if the call had not been made; the copy
  * takes one more state's worth of device memory while it runs. */

@@ -29,6 +29,8 @@ FP64, FP32, FP32_PLAIN = 0, 1, 2
 FIELD_Q, FIELD_PSI, FIELD_TOPO, FIELD_F0, FIELD_F1, FIELD_F2 = 1, 2, 3, 4, 5, 6
 PASS_NAMES = ["P1", "T1", "P2", "T2", "P3", "P4", "CFL", "DIAG"]
 TUNE_P4_TILE = 1
+TUNE_T_KERNEL = 2
+TUNE_P4_WAVES = 3
 P4_TILES = ["64x8", "64x16", "64x32", "128x16", "stream"]
 _TUNE_CACHE = {}
 
@@ -424,7 +426,9 @@ class Model:
         if key not in _TUNE_CACHE:
             t, ms = ctypes.c_int(), ctypes.c_double()
             self._chk(self.lib.qgw_autotune(self.ctx, reps, ctypes.byref(t), ctypes.byref(ms)), "autotune")
-            _TUNE_CACHE[key] = {TUNE_P4_TILE: t.value, "ms_per_step": ms.value}
+            _TUNE_CACHE[key] = {TUNE_P4_TILE: t.value, "ms_per_step": ms.value,
+                                TUNE_P4_WAVES: self.tune_get(TUNE_P4_WAVES),
+                                TUNE_T_KERNEL: self.tune_get(TUNE_T_KERNEL)}
         for knob, v in _TUNE_CACHE[key].items():
             if knob != "ms_per_step":
                 self.tune_set(knob, v)

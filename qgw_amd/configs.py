@@ -69,14 +69,16 @@ def c2(as_written=False, **over):
 def c3(**over):
     """2048^2 x 10 layers, L = 4000 km, log-depth layers summing to 4000 m, seeded exponential
     stratification (e-folding 800 m), double-gyre wind 0.08 N/m^2, drag 1e-7, nu4 = 2e8
-    (stable limit ~3.4e8), dt 900 s."""
+    (stable limit ~3.4e8), dt 900 s.  Initial RMS velocity 0.05 m/s (as c2): at 0.1 m/s the
+    eddy peaks in the two thin top layers (40 m and 27 m) exceed the AB3 advective limit of
+    dt = 900 s and the run diverges near step 185, whereas 0.05 m/s runs the full 1000 steps."""
     nl = 10
     dh, zb = _logdepth_layers(nl)
     N2, _ = _exp_stratification(zb, SEED0)
     p = default_params(NX=2048, NY=2048, nl=nl, Lx=4000e3, dh=dh, N2=N2, f0=1e-4, beta=2e-11,
                        hEkb=2 * 1e-7 * dh[-1] / 1e-4, n_hyper=4, nu_hyper=2e8, tau0=0.08 / RHO0,
                        forc_mode=2.0, bc_fac=0.0, dt_fixed=900.0, nsteps=1000, ic_kind=1, seed=SEED0,
-                       ic_urms=0.1, ic_lambda0=200e3)
+                       ic_urms=0.05, ic_lambda0=200e3)
     return p.set(**over)
 
 
@@ -125,4 +127,18 @@ def c4(classes=None, dt=1200.0):
     return out
 
 
-WORKLOADS = {"c1": c1, "c2": c2, "c3": c3, "c4": c4}
+def c5(**over):
+    """4096^2 x 32 layers, L = 8000 km, log-depth layers summing to 4000 m with the seeded
+    exponential stratification of c3, double-gyre wind 0.08 N/m^2, drag 1e-7, dt 600 s,
+    nu4 = 2e8 (26 nu4 dt / Delta^4 = 0.21)."""
+    nl = 32
+    dh, zb = _logdepth_layers(nl)
+    N2, _ = _exp_stratification(zb, SEED0)
+    p = default_params(NX=4096, NY=4096, nl=nl, Lx=8000e3, dh=dh, N2=N2, f0=1e-4, beta=2e-11,
+                       hEkb=2 * 1e-7 * dh[-1] / 1e-4, n_hyper=4, nu_hyper=2e8, tau0=0.08 / RHO0,
+                       forc_mode=2.0, bc_fac=0.0, dt_fixed=600.0, nsteps=200, ic_kind=1, seed=SEED0,
+                       ic_urms=0.1, ic_lambda0=200e3)
+    return p.set(**over)
+
+
+WORKLOADS = {"c1": c1, "c2": c2, "c3": c3, "c4": c4, "c5": c5}

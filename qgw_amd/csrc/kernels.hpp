@@ -154,6 +154,108 @@ k_tproj(const T *__restrict__ src, T *__restrict__ dst, const T *__restrict__ M,
 }
 
 // ===========================================================================
+// Transpose + layer<->mode projection on the matrix cores (FP32, up to 32
+// layers): out_k = sum_l M[k][l] in_l as D = M (32 x 32) * V (32 layers x
+// points) with v_mfma_f32_32x32x2f32.  A block takes 16 source columns x 32
+// source rows; a point group is 32 consecutive source rows of one column, so
+// the accumulator rows land contiguous in the destination (128-byte stores)
+// and no LDS pass is needed on the way out.  Layers stream through LDS in
+// chunks of 8; layers and modes beyond nl are zero.
+//   A (32 x 2):  lane i holds M[i % 32][2s + i / 32]
+//   B (2 x 32):  lane i holds V[2s + i / 32][point i % 32]
+//   D (32 x 32): register j of lane i holds D[8 (j / 4) + 4 (i / 32) + j % 4][i % 32]
+// ===========================================================================
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+template <int KC, int TCB>
+__global__ void __launch_bounds__(256)
+k_tproj_mfma(const float *__restrict__ src, float *__restrict__ dst, const float *__restrict__ M, int nl, int R,
+             int C, int Ps, long Ss, int Pd, long Sd, int ntc, int ntr) {
+  constexpr int TRB = 32, NCH = 32 / KC;
+  constexpr int GPW = TCB / 4;        // point groups (source columns) per wavefront
+  constexpr int LR = 256 / TCB;       // source rows per load pass
+  constexpr int LU = TRB / LR;        // load passes
+  static_assert(TCB % 4 == 0 && TRB % LR == 0, "tile shape");
+  __shared__ float lds[KC][TRB][TCB + 1];
+  // tiles in raster order per XCD, so that tiles sharing cache lines meet in one L2
+  const int tile = xcd_remap(blockIdx.x, gridDim.x);
+  const int z = tile / (ntc * ntr), cy = tile % (ntc * ntr);
+  src += (long)z * nl * Ss;
+  dst += (long)z * nl * Sd;
+  M += (long)z * nl * nl;
+  const int c0 = (cy % ntc) * TCB, r0 = (cy / ntc) * TRB;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int half = lane >> 5, l32 = lane & 31;
+  float a[16];
+#pragma unroll
+  for (int s = 0; s < 16; s++) {
+    const int l = 2 * s + half;
+    a[s] = (l32 < nl && l < nl) ? M[l32 * nl + l] : 0.f;
+  }
+  f32x16 acc[GPW];
+#pragma unroll
+  for (int g = 0; g < GPW; g++)
+#pragma unroll
+    for (int j = 0; j < 16; j++) acc[g][j] = 0.f;
+  const int lc = tid % TCB, lr = tid / TCB;
+  // Loads go to clamped (always valid) addresses and are zeroed by a select,
+  // so that a chunk's loads are all in flight before the first LDS write.
+  const int cc = (c0 + lc < C) ? c0 + lc : C - 1;
+  const bool cok = c0 + lc < C;
+  long roff[LU];
+  bool rok[LU];
+#pragma unroll
+  for (int u = 0; u < LU; u++) {
+    const int r = r0 + lr + LR * u;
+    rok[u] = cok && r < R;
+    roff[u] = (long)(r < R ? r : R - 1) * Ps + cc;
+  }
+  // chunk ch+1's loads are issued before chunk ch's matrix products
+  float v[KC][LU];
+  auto load_chunk = [&](int l0) {
+#pragma unroll
+    for (int kk = 0; kk < KC; kk++) {
+      const int l = (l0 + kk < nl) ? l0 + kk : nl - 1;
+#pragma unroll
+      for (int u = 0; u < LU; u++) v[kk][u] = src[l * Ss + roff[u]];
+    }
+  };
+  load_chunk(0);
+#pragma unroll
+  for (int ch = 0; ch < NCH; ch++) {
+    const int l0 = ch * KC;
+    if (l0 >= nl) break;  // uniform
+#pragma unroll
+    for (int kk = 0; kk < KC; kk++)
+#pragma unroll
+      for (int u = 0; u < LU; u++) lds[kk][lr + LR * u][lc] = (l0 + kk < nl && rok[u]) ? v[kk][u] : 0.f;
+    __syncthreads();
+    if (ch + 1 < NCH && l0 + KC < nl) load_chunk(l0 + KC);
+#pragma unroll
+    for (int g = 0; g < GPW; g++) {
+      const int cl = w * GPW + g;
+#pragma unroll
+      for (int s = 0; s < KC / 2; s++) {
+        const float b = lds[2 * s + half][l32][cl];
+        acc[g] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[ch * (KC / 2) + s], b, acc[g], 0, 0, 0);
+      }
+    }
+    __syncthreads();
+  }
+  const int r = r0 + l32;
+#pragma unroll
+  for (int g = 0; g < GPW; g++) {
+    const int c = c0 + w * GPW + g;
+    if (c >= C || r >= R) continue;
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+      const int k = 8 * (j >> 2) + 4 * half + (j & 3);
+      if (k < nl) dst[k * Sd + (long)c * Pd + r] = acc[g][j];
+    }
+  }
+}
+
+// ===========================================================================
 // P2: per pair of whatT rows (mode k, x frequency a): forward transform along
 // y, Helmholtz division, inverse transform along y.  The division is applied
 // at the home positions in registers: position p holds C_p, and the input of
@@ -570,23 +672,72 @@ k_p4(Geo g, const T *__restrict__ psi, const T *__restrict__ q, T *__restrict__ 
 // no barrier.  Each psi row is read once per chunk (plus 2H+2 warm-up rows).
 // Same arithmetic and wall rules as the tiled form above.
 // ===========================================================================
+// bound_ctrl set: a lane with no source lane (63 for the right shift, 0 for
+// the left) receives 0, and no old value has to be staged in the destination.
 __device__ __forceinline__ float dpp_right(float v) {  // value of lane i+1
-  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x130, 0xF, 0xF, false));
+  return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x130, 0xF, 0xF, true));
 }
 __device__ __forceinline__ float dpp_left(float v) {  // value of lane i-1
-  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x138, 0xF, 0xF, false));
+  return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x138, 0xF, 0xF, true));
 }
 __device__ __forceinline__ double dpp_right(double v) {
   const long long b = __builtin_bit_cast(long long, v);
-  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffLL), 0x130, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x130, 0xF, 0xF, false);
+  const int lo = __builtin_amdgcn_mov_dpp((int)(b & 0xffffffffLL), 0x130, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), 0x130, 0xF, 0xF, true);
   return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
 }
 __device__ __forceinline__ double dpp_left(double v) {
   const long long b = __builtin_bit_cast(long long, v);
-  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffLL), 0x138, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x138, 0xF, 0xF, false);
+  const int lo = __builtin_amdgcn_mov_dpp((int)(b & 0xffffffffLL), 0x138, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), 0x138, 0xF, 0xF, true);
   return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Element at byte offset xb of a row: a wavefront-uniform row pointer plus an
+// unsigned per-lane offset, which maps to the scalar-base form of the memory
+// instructions (no 64-bit address arithmetic per lane).
+// The row pointer passes through readfirstlane (free for a value already in
+// scalar registers) so that the compiler keeps it scalar instead of hoisting a
+// per-lane "base + offset" pointer out of the row loop; it is rebuilt as a
+// global-address-space pointer so that the scalar-base load/store forms apply.
+typedef __attribute__((address_space(1))) char gchar;
+__device__ __forceinline__ gchar *uniform_gptr(const void *p) {
+  const unsigned long long v = (unsigned long long)p;
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+  return (gchar *)(((unsigned long long)hi << 32) | lo);
+}
+template <typename T> __device__ __forceinline__ T ld_at(const T *rowp, unsigned xb) {
+  return *(const __attribute__((address_space(1))) T *)(uniform_gptr(rowp) + xb);
+}
+
+// Raw buffer view of one layer of a field: a scalar resource, a per-lane byte
+// offset (VGPR) and a per-row byte offset (SGPR) -- the loads carry no
+// per-lane address arithmetic.  Resource word 3 = 0x00020000 (32-bit data
+// format, the gfx9 family); stride 0; num_records = the bytes of the view.
+struct BufView {
+  __amdgpu_buffer_rsrc_t r;
+};
+__device__ __forceinline__ BufView buf_view(const void *base, unsigned bytes) {
+  return BufView{__builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, (int)bytes, 0x00020000)};
+}
+// AUX: the cache-policy bits of the instruction (0: default caching).
+template <typename T, int AUX = 0> __device__ __forceinline__ T buf_ld(const BufView &b, unsigned xb, unsigned rowb) {
+  if constexpr (sizeof(T) == 4)
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(b.r, (int)xb, (int)rowb, AUX));
+  else
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(b.r, (int)xb, (int)rowb, AUX));
+}
+typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+template <int AUX = 0> __device__ __forceinline__ void buf_st(const BufView &b, unsigned xb, unsigned rowb, float v) {
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), b.r, (int)xb, (int)rowb, AUX);
+}
+template <int AUX = 0> __device__ __forceinline__ void buf_st(const BufView &b, unsigned xb, unsigned rowb, double v) {
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), b.r, (int)xb, (int)rowb, AUX);
+}
+template <typename T> __device__ __forceinline__ void st_at(T *rowp, unsigned xb, T v) {
+  asm volatile("" : "+v"(xb));
+  *(__attribute__((address_space(1))) T *)(uniform_gptr(rowp) + xb) = v;
 }
 
 // 12 Delta^2 J(p, q) in the 12-term form of src/domain.h:17-27, from the 3x3
@@ -599,39 +750,36 @@ __device__ __forceinline__ T arakawa12(T pcl, T pcr, T pu, T pd, T pul, T pur, T
          qcl * (pul - pdl);
 }
 
-template <typename T, int NLAP, int KAHAN>
-__global__ void __launch_bounds__(256)
-k_p4s(Geo g, const T *__restrict__ psi, const T *__restrict__ q, T *__restrict__ qn, T *__restrict__ fnew,
-      const T *__restrict__ f1, const T *__restrict__ f2, T *__restrict__ comp, const T *__restrict__ topo,
-      const T *__restrict__ wind, StepCoef cf, int nstrip, int chunk, int nchunk) {
+template <int I> struct IC {
+  static constexpr int value = I;
+};
+// slot of row r - i in a circular window of W rows written at phase ph (= r mod W)
+template <int W> __device__ constexpr int wslot(int ph, int i) { return ((ph - i) % W + W) % W; }
+
+// The row walk of one wavefront.  The loop is unrolled by W (a multiple of 2
+// and 3 holding the psi rows in use), so every window slot and prefetch buffer
+// is a fixed register: no moves shift the windows.  EDGE = false: the strip
+// and its rows touch no wall, so every load is in bounds and no wall rule or
+// ring carry applies.
+template <typename T, int NLAP, int KAHAN, bool EDGE>
+__device__ __forceinline__ void p4s_walk(const Geo &g, const T *__restrict__ P, const T *__restrict__ Qf,
+                                         T *__restrict__ qn, T *__restrict__ fnew, const T *__restrict__ f1,
+                                         const T *__restrict__ f2, T *__restrict__ comp, const T *__restrict__ topo,
+                                         const T *__restrict__ wind, const StepCoef &cf, int x, int lane, int km,
+                                         int y0, int y1, bool ringstrip) {
   constexpr int H = (NLAP > 1) ? NLAP : 1;       // halo, and the delay of the output row
-  constexpr int OW = 64 - 2 * H;                 // output columns per strip
   constexpr int NP = H + 2;                      // psi rows r, r-1, ..., r-H-1
+  constexpr int D = 2;                           // prefetch depth (iterations)
+  constexpr int RING = D + 1;                    // prefetch ring slots; 3 or 6 divide W
+  constexpr int W = (NP <= 6) ? 6 : 12;          // unroll and psi window length (a multiple of 3 and RING)
+  static_assert(W % RING == 0 && W % 3 == 0, "unroll");
   constexpr int NL = (NLAP > 1) ? NLAP - 1 : 1;  // stored levels 1..NLAP-1
+  static_assert(NP <= W, "psi window");
   const int n = g.n, m = g.m;
-  const int lane = threadIdx.x & 63;
-  const int wave = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
-  if (wave >= nstrip * nchunk * g.nl) return;  // whole wavefronts only; no barrier follows
-  const int sidx = wave % nstrip;
-  const int cidx = (wave / nstrip) % nchunk;
-  const int k = wave / (nstrip * nchunk);  // layer (all members)
-  const int mb = k / g.nlm, km = k - mb * g.nlm;
-  topo += mb * g.LS;
-  wind += (long)mb * (m + 2);
-  const int x = 1 + sidx * OW - H + lane;  // storage column of this lane
-  const bool xin = (x >= 0 && x <= n + 1);
-  const bool outl = (lane >= H && lane < 64 - H && x >= 1 && x <= n);
-  const bool ringstrip = (sidx == 0 || sidx == nstrip - 1);  // wavefront-uniform
-  const int xc = xin ? x : 0;                                  // clamped column for addresses
-  const int y0 = 1 + cidx * chunk;
-  const int y1 = (y0 + chunk < m + 1) ? y0 + chunk : m + 1;
-  const long lo = k * g.LS;
-  const T *P = psi + lo, *Qf = q + lo;
-  f1 += lo;
-  f2 += lo;
-  qn += lo;
-  fnew += lo;
-  if (KAHAN) comp += lo;
+  const bool xin = !EDGE || (x >= 0 && x <= n + 1);
+  const bool xring = EDGE && (x == 0 || x == n + 1);
+  const bool outl = lane >= H && lane < 64 - H && (!EDGE || (x >= 1 && x <= n));
+  const int xc = xin ? x : 0;  // clamped column for addresses
   const bool bottom = (km == g.nlm - 1);
   const bool upd = cf.update != 0;
   const T inv_d2 = (T)cf.inv_d2, ring_c = (T)cf.ring_c;
@@ -641,13 +789,13 @@ k_p4s(Geo g, const T *__restrict__ psi, const T *__restrict__ q, T *__restrict__
   const T drag = (T)cf.drag, tfac = (T)cf.topo_fac;
   const bool use_nu = cf.nu != 0.0;
 
-  T pw[NP];                                           // psi rows r .. r-H-1
-  T nbw[NP];                                          // psi of the interior neighbour, ring columns
-  T lv[NL][3];                                        // level a rows r-a, r-a-1, r-a-2
-  T jpc[3], jpl[3], jpr[3], jqc[3], jql[3], jqr[3];  // rows y+1, y, y-1 of psi and q
+  T pw[W], nbw[W];                                    // psi rows; psi of the ring columns' neighbour
+  T lv[NL][3];                                        // level a: rows r-a, r-a-1, r-a-2 (circular)
+  T jpc[3], jpl[3], jpr[3], jqc[3], jql[3], jqr[3];  // rows y+1, y, y-1 of psi and q (circular)
   T jtc[3], jtl[3], jtr[3];                           // topography rows (bottom layer)
+  T bp[RING], bq[RING], bt[RING], b1[RING], b2[RING], bc[RING], bw[RING];  // loads of r .. r+D (slot r mod RING)
 #pragma unroll
-  for (int i = 0; i < NP; i++) pw[i] = nbw[i] = T(0);
+  for (int i = 0; i < W; i++) pw[i] = nbw[i] = T(0);
 #pragma unroll
   for (int a = 0; a < NL; a++)
 #pragma unroll
@@ -658,134 +806,234 @@ k_p4s(Geo g, const T *__restrict__ psi, const T *__restrict__ q, T *__restrict__
     jqc[i] = jql[i] = jqr[i] = T(0);
     jtc[i] = jtl[i] = jtr[i] = T(0);
   }
+#pragma unroll
+  for (int i = 0; i < RING; i++) bp[i] = bq[i] = bt[i] = b1[i] = b2[i] = bc[i] = bw[i] = T(0);
 
-  // Loads of iteration r: psi row r, q and topography row r-H+1, and the
-  // AB3 history of output row r-H.  Issued one iteration ahead.
   const int rbeg = y0 - H, rend = y1 + H;
-  auto ldrow = [&](const T *base, int row) -> T {
-    return (xin && row >= 0 && row <= m + 1) ? base[row * g.P + xc] : T(0);
-  };
-  auto ldout = [&](const T *base, int row) -> T {
-    return (outl && row >= y0 && row < y1) ? base[row * g.P + x] : T(0);
-  };
-  T np = ldrow(P, rbeg), nq = ldrow(Qf, rbeg - H + 1);
-  T nt = bottom ? ldrow(topo, rbeg - H + 1) : T(0);
-  T nf1 = T(0), nf2 = T(0), ncp = T(0);
-
-  for (int r = rbeg; r < rend; r++) {
-    const int y = r - H;  // output row of this iteration
-    const T p0 = np, qnew = nq, tnew = nt, h1 = nf1, h2 = nf2, hc = ncp;
-    if (r + 1 < rend) {
-      np = ldrow(P, r + 1);
-      nq = ldrow(Qf, r + 2 - H);
-      if (bottom) nt = ldrow(topo, r + 2 - H);
+  const unsigned xb = (unsigned)xc * sizeof(T);  // byte offset of this lane's column
+  const unsigned lbytes = (unsigned)(g.LS * sizeof(T));
+  const BufView vP = buf_view(P, lbytes), vQ = buf_view(Qf, lbytes), vT = buf_view(topo, lbytes);
+  const BufView v1 = buf_view(f1, lbytes), v2 = buf_view(f2, lbytes), vC = buf_view(comp, lbytes);
+  const unsigned rowb = (unsigned)(g.P * sizeof(T));
+  const BufView vW = buf_view(wind, (unsigned)((m + 2) * sizeof(T)));
+  // loads of iteration r: psi row r, q and topography row r-H+1, AB3 history
+  // and wind of row r-H; issued D iterations ahead
+  auto fetch = [&](auto Sc, int r) {
+    constexpr int s = decltype(Sc)::value;
+    // wind of row r-H: a vector load (in-order completion), so that waiting
+    // for it does not also wait for the scalar loads issued since
+    if (km == 0) {
+      if constexpr (EDGE)
+        bw[s] = (r - H >= 0 && r - H <= m + 1) ? wind[r - H] : T(0);
+      else
+        bw[s] = buf_ld<T>(vW, 0u, (unsigned)(r - H) * (unsigned)sizeof(T));
+    }
+    if constexpr (EDGE) {
+      auto ldrow = [&](const T *base, int row) -> T {
+        return (xin && row >= 0 && row <= m + 1) ? ld_at(base + row * g.P, xb) : T(0);
+      };
+      auto ldout = [&](const T *base, int row) -> T {
+        return (outl && row >= y0 && row < y1) ? ld_at(base + row * g.P, xb) : T(0);
+      };
+      bp[s] = ldrow(P, r);
+      bq[s] = ldrow(Qf, r + 1 - H);
+      if (bottom) bt[s] = ldrow(topo, r + 1 - H);
       if (upd) {
-        nf1 = ldout(f1, r + 1 - H);
-        nf2 = ldout(f2, r + 1 - H);
-        if (KAHAN) ncp = ldout(comp, r + 1 - H);
+        b1[s] = ldout(f1, r - H);
+        b2[s] = ldout(f2, r - H);
+        if (KAHAN) bc[s] = ldout(comp, r - H);
+      }
+    } else {
+      // every row an interior walk touches lies inside the layer
+      if (r >= rend) return;  // past the walk (wavefront-uniform)
+      bp[s] = buf_ld<T>(vP, xb, (unsigned)r * rowb);
+      bq[s] = buf_ld<T>(vQ, xb, (unsigned)(r + 1 - H) * rowb);
+      if (bottom) bt[s] = buf_ld<T>(vT, xb, (unsigned)(r + 1 - H) * rowb);
+      if (r - H >= y0) {  // the AB3 history only for output rows (wavefront-uniform)
+        b1[s] = buf_ld<T>(v1, xb, (unsigned)(r - H) * rowb);
+        b2[s] = buf_ld<T>(v2, xb, (unsigned)(r - H) * rowb);
+        if (KAHAN) bc[s] = buf_ld<T>(vC, xb, (unsigned)(r - H) * rowb);
       }
     }
-    // ---- shift in psi row r
-#pragma unroll
-    for (int i = NP - 1; i > 0; i--) {
-      pw[i] = pw[i - 1];
-      nbw[i] = nbw[i - 1];
-    }
-    pw[0] = p0;
-    if (ringstrip) {
-      const T pr = dpp_right(p0), pl = dpp_left(p0);
-      nbw[0] = (x == 0) ? pr : pl;
+  };
+  const BufView vF = buf_view(fnew, lbytes), vN = buf_view(qn, lbytes);
+  auto body = [&](auto PHc, int r) {
+    constexpr int ph = decltype(PHc)::value;
+    constexpr int cur = ph % RING;
+    constexpr int j0 = ph % 3, j1 = (ph + 2) % 3, j2 = (ph + 1) % 3;  // rows y+1, y, y-1
+    const int y = r - H;  // output row of this iteration
+    if (r + D < rend) fetch(IC<(ph + D) % RING>{}, r + D);
+    const T p0 = bp[cur];
+    pw[wslot<W>(ph, 0)] = p0;
+    if constexpr (EDGE) {
+      if (ringstrip) {
+        const T pr = dpp_right(p0), pl = dpp_left(p0);
+        nbw[wslot<W>(ph, 0)] = (x == 0) ? pr : pl;
+      }
     }
     // ---- nested Laplacians: level a at row r - a (calc_diff, src/dynamics.h:25-43)
     T hyp = T(0);
     if constexpr (NLAP >= 1) {
-      T s0 = pw[0], s1 = pw[1], s2 = pw[2];  // source rows yl+1, yl, yl-1 of level a-1
+      T s0 = pw[wslot<W>(ph, 0)], s1 = pw[wslot<W>(ph, 1)], s2 = pw[wslot<W>(ph, 2)];
 #pragma unroll
       for (int a = 1; a <= NLAP; a++) {
-        const int yl = r - a;
         T v = (dpp_left(s1) + dpp_right(s1) + s0 + s2 - T(4) * s1) * inv_d2;
-        if (yl == 0 || yl == m + 1) {
-          // wall row: 2 bc_fac/Delta^2 * psi of the adjacent interior row; corners 0
-          v = (x == 0 || x == n + 1) ? T(0) : ring_c * ((yl == 0) ? pw[a - 1] : pw[a + 1]);
-        } else if (ringstrip && (x == 0 || x == n + 1)) {
-          v = ring_c * nbw[a];  // wall column: its interior neighbour's psi
+        if constexpr (EDGE) {
+          const int yl = r - a;
+          if (yl == 0 || yl == m + 1) {
+            // wall row: 2 bc_fac/Delta^2 * psi of the adjacent interior row; corners 0
+            v = xring ? T(0) : ring_c * ((yl == 0) ? pw[wslot<W>(ph, a - 1)] : pw[wslot<W>(ph, a + 1)]);
+          } else if (xring) {
+            v = ring_c * nbw[wslot<W>(ph, a)];  // wall column: its interior neighbour's psi
+          }
         }
         if (a < NLAP) {
-          lv[a - 1][2] = lv[a - 1][1];
-          lv[a - 1][1] = lv[a - 1][0];
-          lv[a - 1][0] = v;
-          s0 = lv[a - 1][0];
-          s1 = lv[a - 1][1];
-          s2 = lv[a - 1][2];
+          lv[a - 1][j0] = v;
+          s0 = v;
+          s1 = lv[a - 1][j1];
+          s2 = lv[a - 1][j2];
         } else {
           hyp = v;
         }
       }
     }
     // ---- 3x3 windows for the Jacobians at row y
-    const T pn = pw[H - 1];  // psi row y+1
-#pragma unroll
-    for (int i = 2; i > 0; i--) {
-      jpc[i] = jpc[i - 1];
-      jpl[i] = jpl[i - 1];
-      jpr[i] = jpr[i - 1];
-      jqc[i] = jqc[i - 1];
-      jql[i] = jql[i - 1];
-      jqr[i] = jqr[i - 1];
-    }
-    jpc[0] = pn;
-    jpl[0] = dpp_left(pn);
-    jpr[0] = dpp_right(pn);
-    jqc[0] = qnew;
-    jql[0] = dpp_left(qnew);
-    jqr[0] = dpp_right(qnew);
+    const T pn = pw[wslot<W>(ph, H - 1)];  // psi row y+1
+    const T qnew = bq[cur];
+    jpc[j0] = pn;
+    jpl[j0] = dpp_left(pn);
+    jpr[j0] = dpp_right(pn);
+    jqc[j0] = qnew;
+    jql[j0] = dpp_left(qnew);
+    jqr[j0] = dpp_right(qnew);
     if (bottom) {
-#pragma unroll
-      for (int i = 2; i > 0; i--) {
-        jtc[i] = jtc[i - 1];
-        jtl[i] = jtl[i - 1];
-        jtr[i] = jtr[i - 1];
-      }
-      jtc[0] = tnew;
-      jtl[0] = dpp_left(tnew);
-      jtr[0] = dpp_right(tnew);
+      const T tnew = bt[cur];
+      jtc[j0] = tnew;
+      jtl[j0] = dpp_left(tnew);
+      jtr[j0] = dpp_right(tnew);
     }
-    if (y < y0) continue;  // wavefront-uniform
+    if (y < y0) return;  // warm-up row (wavefront-uniform)
     // ---- tendency at row y (rhs, src/dynamics.h:52-71)
-    const T jpq = arakawa12<T>(jpl[1], jpr[1], jpc[0], jpc[2], jpl[0], jpr[0], jpl[2], jpr[2], jql[1], jqr[1],
-                               jqc[0], jqc[2], jql[0], jqr[0], jql[2], jqr[2]) *
+    const T jpq = arakawa12<T>(jpl[j1], jpr[j1], jpc[j0], jpc[j2], jpl[j0], jpr[j0], jpl[j2], jpr[j2], jql[j1],
+                               jqr[j1], jqc[j0], jqc[j2], jql[j0], jqr[j0], jql[j2], jqr[j2]) *
                   i12;
-    T f = -jpq - beta * (jpr[1] - jpl[1]) * i2d;
-    if (use_nu) f += nu * ((jql[1] + jqr[1] + jqc[0] + jqc[2] - T(4) * jqc[1]) * inv_d2);
+    T f = -jpq - beta * (jpr[j1] - jpl[j1]) * i2d;
+    if (use_nu) f += nu * ((jql[j1] + jqr[j1] + jqc[j0] + jqc[j2] - T(4) * jqc[j1]) * inv_d2);
     if constexpr (NLAP >= 1) f += nuh * hyp;
-    if (km == 0) f += wind[y];
+    if (km == 0) f += bw[cur];
     if (bottom) {
-      const T lap1 = (jpl[1] + jpr[1] + jpc[0] + jpc[2] - T(4) * jpc[1]) * inv_d2;
-      const T jt = arakawa12<T>(jpl[1], jpr[1], jpc[0], jpc[2], jpl[0], jpr[0], jpl[2], jpr[2], jtl[1], jtr[1],
-                                jtc[0], jtc[2], jtl[0], jtr[0], jtl[2], jtr[2]) *
+      const T lap1 = (jpl[j1] + jpr[j1] + jpc[j0] + jpc[j2] - T(4) * jpc[j1]) * inv_d2;
+      const T jt = arakawa12<T>(jpl[j1], jpr[j1], jpc[j0], jpc[j2], jpl[j0], jpr[j0], jpl[j2], jpr[j2], jtl[j1],
+                                jtr[j1], jtc[j0], jtc[j2], jtl[j0], jtr[j0], jtl[j2], jtr[j2]) *
                    i12;
       f += drag * lap1 - jt * tfac;
     }
-    if (!outl) continue;
-    const int o = y * g.P + x;
-    fnew[o] = f;
-    if (!upd) continue;
-    const T qo = jqc[1];
-    const T inc = c0 * f + c1 * h1 + c2 * h2;
+    if (!outl) return;
+    const int ro = y * g.P;  // output row offset (wavefront-uniform)
+    const unsigned rb = (unsigned)y * rowb;
+    if constexpr (EDGE)
+      st_at(fnew + ro, xb, f);
+    else
+      buf_st(vF, xb, rb, f);
+    if (!upd) return;
+    const T qo = jqc[j1];
+    const T inc = c0 * f + c1 * b1[cur] + c2 * b2[cur];
+    T qv, cv = T(0);
     if constexpr (KAHAN) {
-      const T yk = inc - hc;
-      const T tq = qo + yk;
-      comp[o] = (tq - qo) - yk;
-      qn[o] = tq;
+      const T yk = inc - bc[cur];
+      qv = qo + yk;
+      cv = (qv - qo) - yk;
     } else {
-      qn[o] = qo + inc;
+      qv = qo + inc;
     }
-    // carry q's wall ring into the next buffer (corners stay zero)
-    if (y == 1) qn[x] = jqc[2];
-    if (y == m) qn[(m + 1) * g.P + x] = jqc[0];
-    if (x == 1) qn[y * g.P] = jql[1];
-    if (x == n) qn[y * g.P + n + 1] = jqr[1];
+    if constexpr (EDGE) {
+      if constexpr (KAHAN) st_at(comp + ro, xb, cv);
+      st_at(qn + ro, xb, qv);
+    } else {
+      if constexpr (KAHAN) buf_st(vC, xb, rb, cv);
+      buf_st(vN, xb, rb, qv);
+    }
+    if constexpr (EDGE) {
+      // carry q's wall ring into the next buffer (corners stay zero)
+      if (y == 1) qn[x] = jqc[j2];
+      if (y == m) qn[(m + 1) * g.P + x] = jqc[j0];
+      if (x == 1) qn[y * g.P] = jql[j1];
+      if (x == n) qn[y * g.P + n + 1] = jqr[j1];
+    }
+  };
+#define QGW_P4S_PRE(k)                          \
+  if constexpr ((k) < D) {                      \
+    if (rbeg + (k) < rend) fetch(IC<(k)>{}, rbeg + (k)); \
   }
+  QGW_P4S_PRE(0)
+  QGW_P4S_PRE(1)
+  QGW_P4S_PRE(2)
+  QGW_P4S_PRE(3)
+  QGW_P4S_PRE(4)
+#undef QGW_P4S_PRE
+  for (int r = rbeg; r < rend; r += W) {
+#define QGW_P4S_PHASE(k)                   \
+  if constexpr ((k) < W) {                 \
+    if (r + (k) >= rend) break;            \
+    body(IC<(k)>{}, r + (k));              \
+  }
+    QGW_P4S_PHASE(0)
+    QGW_P4S_PHASE(1)
+    QGW_P4S_PHASE(2)
+    QGW_P4S_PHASE(3)
+    QGW_P4S_PHASE(4)
+    QGW_P4S_PHASE(5)
+    QGW_P4S_PHASE(6)
+    QGW_P4S_PHASE(7)
+    QGW_P4S_PHASE(8)
+    QGW_P4S_PHASE(9)
+    QGW_P4S_PHASE(10)
+    QGW_P4S_PHASE(11)
+#undef QGW_P4S_PHASE
+  }
+}
+
+template <typename T, int NLAP, int KAHAN>
+__global__ void __launch_bounds__(256)
+k_p4s(Geo g, const T *__restrict__ psi, const T *__restrict__ q, T *__restrict__ qn, T *__restrict__ fnew,
+      const T *__restrict__ f1, const T *__restrict__ f2, T *__restrict__ comp, const T *__restrict__ topo,
+      const T *__restrict__ wind, StepCoef cf, int nstrip, int chunk, int nchunk) {
+  constexpr int H = (NLAP > 1) ? NLAP : 1;  // halo
+  constexpr int OW = 64 - 2 * H;            // output columns per strip
+  const int n = g.n, m = g.m;
+  const int lane = threadIdx.x & 63;
+  // Blocks are dealt to the 8 XCDs round-robin; the remap gives each XCD one
+  // contiguous run of (strip, chunk, layer) walks, so the columns shared by
+  // neighbouring strips and the warm-up rows shared by consecutive chunks are
+  // fetched into one L2.
+  const int wave = __builtin_amdgcn_readfirstlane(xcd_remap(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6));
+  if (wave >= nstrip * nchunk * g.nl) return;  // whole wavefronts only; no barrier follows
+  const int sidx = wave % nstrip;
+  const int cidx = (wave / nstrip) % nchunk;
+  const int k = wave / (nstrip * nchunk);  // layer (all members)
+  const int mb = k / g.nlm, km = k - mb * g.nlm;
+  topo += mb * g.LS;
+  wind += (long)mb * (m + 2);
+  const int xlo = 1 + sidx * OW - H;  // storage column of lane 0
+  const int x = xlo + lane;
+  const int y0 = 1 + cidx * chunk;
+  const int y1 = (y0 + chunk < m + 1) ? y0 + chunk : m + 1;
+  const long lo = k * g.LS;
+  const T *P = psi + lo, *Qf = q + lo;
+  f1 += lo;
+  f2 += lo;
+  qn += lo;
+  fnew += lo;
+  if (KAHAN) comp += lo;
+  const bool ringstrip = (xlo <= 0 || xlo + 63 >= n + 1);
+  // interior: no wall column or row in reach of the stencils, so every row the
+  // walk loads lies inside the layer
+  const bool interior = xlo >= 1 && xlo + 63 <= n && y0 - H - NLAP >= 1 && y1 + H - 1 <= m;
+  if (interior)
+    p4s_walk<T, NLAP, KAHAN, false>(g, P, Qf, qn, fnew, f1, f2, comp, topo, wind, cf, x, lane, km, y0, y1, false);
+  else
+    p4s_walk<T, NLAP, KAHAN, true>(g, P, Qf, qn, fnew, f1, f2, comp, topo, wind, cf, x, lane, km, y0, y1,
+                                   ringstrip);
 }
 
 // ===========================================================================

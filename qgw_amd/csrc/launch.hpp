@@ -9,8 +9,8 @@
 
 namespace qgw {
 
-// Layers handled by the register-resident projections of P1/P3.
-constexpr int kMaxLayers = 16;
+// Layers per member handled by the layer<->mode projections.
+constexpr int kMaxLayers = 32;
 
 // P4 tile shapes (columns x rows); the workgroup has 4*columns threads.  Variant
 // kP4TileCount is the streaming form (k_p4s).
@@ -32,6 +32,8 @@ struct LaunchArgs {
   StepCoef cf{};
   int nlap = 0;
   int p4_tile = kP4TileCount;  // P4 variant: a tile index, or kP4TileCount for streaming
+  int p4_waves = 32;           // streaming P4: target wavefronts per SIMD (sets the row chunk)
+  int t_kernel = 0;            // projection: 0 auto (matrix cores for FP32 at >= 16 layers), 1 VALU, 2 matrix cores
   unsigned long long *cfl_max = nullptr;
 };
 

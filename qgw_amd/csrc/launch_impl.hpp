@@ -74,13 +74,27 @@ static void tproj_go(const LaunchArgs &a, const void *src, void *dst, const void
 
 static int tproj(const LaunchArgs &a, const void *src, void *dst, const void *M, int R, int C, int Ps, long Ss,
                  int Pd, long Sd) {
-  if (a.g.nlm < 1 || a.g.nl % a.g.nlm != 0) return QGW_ERR_ARG;
-  if (a.g.nlm <= 4)
+  const int nl = a.g.nlm;
+  if (nl < 1 || a.g.nl % nl != 0 || nl > kMaxLayers) return QGW_ERR_ARG;
+  const bool mfma = sizeof(T) == 4 && (a.t_kernel >= 2 || (a.t_kernel == 0 && nl >= 16));
+  if (mfma) {
+    const int tcb = (a.t_kernel == 3) ? 16 : 32;
+    const int ntc = (C + tcb - 1) / tcb, ntr = (R + 31) / 32;
+    const long blocks = (long)ntc * ntr * (a.g.nl / nl);
+    if (blocks > 0x7fffffffL) return QGW_ERR_ARG;
+    if (tcb == 16)
+      hipLaunchKernelGGL((k_tproj_mfma<8, 16>), dim3((unsigned)blocks), dim3(256), 0, a.stream, (const float *)src,
+                         (float *)dst, (const float *)M, nl, R, C, Ps, Ss, Pd, Sd, ntc, ntr);
+    else
+      hipLaunchKernelGGL((k_tproj_mfma<8, 32>), dim3((unsigned)blocks), dim3(256), 0, a.stream, (const float *)src,
+                         (float *)dst, (const float *)M, nl, R, C, Ps, Ss, Pd, Sd, ntc, ntr);
+  } else if (nl <= 4) {
     tproj_go<4>(a, src, dst, M, R, C, Ps, Ss, Pd, Sd);
-  else if (a.g.nlm <= kMaxLayers)
-    tproj_go<kMaxLayers>(a, src, dst, M, R, C, Ps, Ss, Pd, Sd);
-  else
-    return QGW_ERR_ARG;
+  } else if (nl <= 16) {
+    tproj_go<16>(a, src, dst, M, R, C, Ps, Ss, Pd, Sd);
+  } else {
+    tproj_go<32>(a, src, dst, M, R, C, Ps, Ss, Pd, Sd);
+  }
   return 0;
 }
 
@@ -105,10 +119,11 @@ template <int NLAP, int KAHAN> static void p4s_go(const LaunchArgs &a) {
   constexpr int H = (NLAP > 1) ? NLAP : 1;
   const int ow = 64 - 2 * H;
   const int nstrip = (a.g.n + ow - 1) / ow;
-  // chunk of rows per wavefront: enough wavefronts to fill the device (~8 per SIMD)
-  long lanes = (long)nstrip * a.g.nl;
-  int chunk = (int)((long)a.g.m * lanes / 8192);
-  chunk = chunk < 16 ? 16 : (chunk > 256 ? 256 : chunk);
+  // rows per wavefront: about p4_waves wavefronts per SIMD (256 CUs x 4 SIMDs)
+  const long lanes = (long)nstrip * a.g.nl;
+  const long target = 1024L * (a.p4_waves > 0 ? a.p4_waves : 32);
+  int chunk = (int)(((long)a.g.m * lanes + target - 1) / target);
+  chunk = chunk < 16 ? 16 : (chunk > 512 ? 512 : chunk);
   const int nchunk = (a.g.m + chunk - 1) / chunk;
   const long waves = lanes * nchunk;
   const dim3 grid((unsigned)((waves + 3) / 4));

@@ -58,6 +58,8 @@ struct qgw_ctx {
   unsigned long long *cfl_max;
   int nblk_diag;
   int p4_tile;
+  int t_kernel;  // projection kernel choice (QGW_TUNE_T_KERNEL)
+  int p4_waves;  // streaming P4 wavefronts per SIMD (QGW_TUNE_P4_WAVES)
   bool modes_set;
   double t, dt, dt0, dt1, DT_max, t_out;
   int it;
@@ -239,6 +241,7 @@ int run_invert(qgw_ctx *c) {
   a.La2 = c->La2;
   a.Kb2 = c->Kb2;
   a.iRd2 = c->iRd2_d;
+  a.t_kernel = c->t_kernel;
   a.norm = 4.0 * (double)(c->g.n + 1) * (double)(c->g.m + 1);
   a.ring_c = 2.0 * c->p.bc_fac / (c->grid.Delta * c->grid.Delta);
   int st;
@@ -302,6 +305,7 @@ int run_p4(qgw_ctx *c, const double *cab3, int update) {
   cf.update = update;
   a.nlap = (c->nu_hyper != 0.0) ? (c->p.n_hyper / 2 + 1) : 0;
   a.p4_tile = c->p4_tile;
+  a.p4_waves = c->p4_waves;
   int st;
   {
     PassScope ps(c, QGW_PASS_P4);
@@ -669,6 +673,7 @@ qgw_ctx *create_group(const qgw_params *ps, int nb, int precision, hipStream_t e
   c->it = 0;
   c->qcur = 0;
   c->p4_tile = qgw::kP4TileCount;  // the register-streaming form
+  c->p4_waves = 32;
   c->modes_set = false;
   // the zero fills above ran on the null stream; the step runs on c->stream
   if (hipDeviceSynchronize() != hipSuccess) {
@@ -753,21 +758,44 @@ int autotune_group(qgw_ctx *c, int reps, int *p4_tile, double *best_ms) {
   int st = snapshot_take(c, snap);
   if (st) return st;
   double best = 1e300;
-  int arg = c->p4_tile;
+  int arg = c->p4_tile, warg = c->p4_waves;
+  static const int kWaves[] = {4, 8, 16, 32};
   for (int v = 0; v < qgw::kP4Variants && !st; v++) {
-    c->p4_tile = v;
-    double ms = 0;
-    st = time_steps(c, 2, &ms);  // warm-up (and first-launch code loading)
-    if (!st) st = time_steps(c, reps, &ms);
-    if (!st && ms < best) {
-      best = ms;
-      arg = v;
+    const bool stream = (v == qgw::kP4TileCount);
+    for (int wi = 0; wi < (stream ? 4 : 1) && !st; wi++) {
+      c->p4_tile = v;
+      if (stream) c->p4_waves = kWaves[wi];
+      double ms = 0;
+      st = time_steps(c, 2, &ms);  // warm-up (and first-launch code loading)
+      if (!st) st = time_steps(c, reps, &ms);
+      if (!st && ms < best) {
+        best = ms;
+        arg = v;
+        warg = c->p4_waves;
+      }
     }
+  }
+  c->p4_waves = warg;
+  c->p4_tile = arg;
+  if (!st && c->prec != QGW_FP64) {
+    // projection: VALU or matrix cores
+    double bt = 1e300;
+    int targ = c->t_kernel;
+    for (int v = 1; v <= 3 && !st; v++) {
+      c->t_kernel = v;
+      double ms = 0;
+      st = time_steps(c, 1, &ms);
+      if (!st) st = time_steps(c, reps, &ms);
+      if (!st && ms < bt) {
+        bt = ms;
+        targ = v;
+      }
+    }
+    c->t_kernel = targ;
   }
   int st2 = snapshot_restore(c, snap);
   if (st) return st;
   if (st2) return st2;
-  c->p4_tile = arg;
   if (p4_tile) *p4_tile = arg;
   if (best_ms) *best_ms = best / reps;
   return QGW_OK;
@@ -1073,6 +1101,16 @@ int qgw_tune_set(qgw_ctx *c, int knob, int value) {
     c->p4_tile = value;
     return QGW_OK;
   }
+  if (knob == QGW_TUNE_P4_WAVES) {
+    if (value != 2 && value != 4 && value != 8 && value != 16 && value != 32) return QGW_ERR_ARG;
+    c->p4_waves = value;
+    return QGW_OK;
+  }
+  if (knob == QGW_TUNE_T_KERNEL) {
+    if (value < 0 || value > 3 || (value >= 2 && c->prec == QGW_FP64)) return QGW_ERR_ARG;
+    c->t_kernel = value;
+    return QGW_OK;
+  }
   return QGW_ERR_ARG;
 }
 
@@ -1080,6 +1118,14 @@ int qgw_tune_get(qgw_ctx *c, int knob, int *value) {
   if (!c || !value) return QGW_ERR_ARG;
   if (knob == QGW_TUNE_P4_TILE) {
     *value = c->p4_tile;
+    return QGW_OK;
+  }
+  if (knob == QGW_TUNE_T_KERNEL) {
+    *value = c->t_kernel;
+    return QGW_OK;
+  }
+  if (knob == QGW_TUNE_P4_WAVES) {
+    *value = c->p4_waves;
     return QGW_OK;
   }
   return QGW_ERR_ARG;

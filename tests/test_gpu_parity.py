@@ -186,3 +186,64 @@ def test_p4_tiles_agree(tile, prec):
     m.run(5)
     o.run(5)
     assert rel(m.q, o.field(O.Q)) < tol
+
+
+@pytest.mark.parametrize("nl", [20, 32])
+def test_large_nl_fp64_vs_oracle(nl):
+    p = _small(64, nl)
+    _announce("large nl FP64 vs oracle", p.NX, nl)
+    m, o, _, _ = _pair(p)
+    m.run(8)
+    o.run(8)
+    m.invert_pv()
+    o.invert_pv()
+    assert rel(m.q, o.field(O.Q)) < 1e-12
+    assert rel(m.psi, o.field(O.PSI)) < 1e-12
+
+
+@pytest.mark.parametrize("nl", [3, 20, 32])
+def test_projection_matrix_cores_vs_valu(nl):
+    """FP32 projections on the matrix cores agree with the VALU projections and
+    track the FP64 run."""
+    p = _small(64, nl)
+    _announce("projection MFMA vs VALU", p.NX, nl)
+    modes = Q.eigmode(p)
+    q0 = Q.make_ic(p, modes)[1]
+    runs = {}
+    for name, prec, tk in (("valu", Q.FP32, 1), ("mfma", Q.FP32, 2), ("mfma16", Q.FP32, 3), ("fp64", Q.FP64, 0)):
+        m = Q.Model(p, prec, modes)
+        m.tune_set(Q.TUNE_T_KERNEL, tk)
+        m.set_q(q0)
+        m.run(8)
+        m.invert_pv()
+        runs[name] = (m.q, m.psi)
+        m.close()
+    assert rel(runs["mfma"][1], runs["valu"][1]) < 1e-5
+    assert rel(runs["mfma"][0], runs["valu"][0]) < 1e-5
+    assert rel(runs["mfma"][1], runs["fp64"][1]) < 1e-5
+    assert rel(runs["mfma16"][1], runs["mfma"][1]) < 1e-5
+
+
+@pytest.mark.parametrize("prec", [Q.FP64, Q.FP32])
+@pytest.mark.parametrize("hyper", [(4, 0.0), (0, 1e3), (2, -1e4), (4, 1e8), (6, -1e9), (8, 1e10)])
+def test_p4_stream_paths(prec, hyper):
+    """The streaming P4 (interior walks with buffer loads, edge walks with the
+    wall rules) against the LDS-tile P4 on a grid with interior strips and
+    chunks, for every nesting depth of the Laplacians (n_hyper 0..8, and no
+    hyperviscosity)."""
+    n_hyper, nu_h = hyper
+    p = _small(256, 3, n_hyper=n_hyper, nu_hyper=nu_h, NY=192, Lx=1000e3)
+    _announce("p4 stream paths", prec, n_hyper, nu_h)
+    modes = Q.eigmode(p)
+    q0 = Q.make_ic(p, modes)[1]
+    out = {}
+    for tile in (1, 4):
+        m = Q.Model(p, prec, modes)
+        m.tune_set(Q.TUNE_P4_TILE, tile)
+        m.set_q(q0)
+        m.run(3)
+        out[tile] = (m.q, m.download(Q.FIELD_F0))
+        m.close()
+    tol = 1e-12 if prec == Q.FP64 else 1e-5
+    assert rel(out[4][1], out[1][1]) < tol
+    assert rel(out[4][0], out[1][0]) < tol
