@@ -154,13 +154,22 @@ int qgw_profile_read(qgw_ctx *ctx, double *ms, int *count, int npass);
 /* QGW_TUNE_P4_WAVES sets the row chunk of the streaming P4 so that about
  * 2, 4, 8, 16 or 32 (default) wavefronts per SIMD are launched. */
 #define QGW_TUNE_P4_WAVES 3
+/* QGW_TUNE_DST selects the sine transforms of P1/P2/P3: 1 (default where
+ * provided, both half lengths n+1 and m+1 in 64..4096 or 96..3072) through
+ * complex FFTs of length n+1 (m+1) with a pre/post pass, 0 through complex
+ * FFTs of the odd extension, length 2(n+1) (2(m+1)). */
+#define QGW_TUNE_DST 4
+/* QGW_TUNE_FFT_E: radix schedule of the half-length FFTs of power-of-two
+ * lengths from 1024: 16 (radix-16 stages; default of QGW_FP64) or 8 (radix-8
+ * stages, twice the threads per row; default of the FP32 forms). */
+#define QGW_TUNE_FFT_E 5
 int qgw_tune_set(qgw_ctx *ctx, int knob, int value);
 int qgw_tune_get(qgw_ctx *ctx, int knob, int *value);
 
 /* Autotuner: times `reps` steps with each P4 variant (and, for the streaming
  * one, each QGW_TUNE_P4_WAVES value) and keeps the fastest (returned in
- * *p4_tile, its ms per step in *best_ms; the FP32 forms also choose
- * QGW_TUNE_T_KERNEL).  The state (fields,
+ * *p4_tile, its ms per step in *best_ms); it also chooses QGW_TUNE_DST and
+ * QGW_TUNE_FFT_E, and QGW_TUNE_T_KERNEL for the FP32 forms.  The state (fields,
  * buffer rotation and time) is copied before the timing runs and restored
  * after them, so the run continues as if the call had not been made; the copy
  * takes one more state's worth of device memory while it runs. */

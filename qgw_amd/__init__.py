@@ -31,6 +31,8 @@ PASS_NAMES = ["P1", "T1", "P2", "T2", "P3", "P4", "CFL", "DIAG"]
 TUNE_P4_TILE = 1
 TUNE_T_KERNEL = 2
 TUNE_P4_WAVES = 3
+TUNE_DST = 4
+TUNE_FFT_E = 5
 P4_TILES = ["64x8", "64x16", "64x32", "128x16", "stream"]
 _TUNE_CACHE = {}
 
@@ -428,7 +430,8 @@ class Model:
             self._chk(self.lib.qgw_autotune(self.ctx, reps, ctypes.byref(t), ctypes.byref(ms)), "autotune")
             _TUNE_CACHE[key] = {TUNE_P4_TILE: t.value, "ms_per_step": ms.value,
                                 TUNE_P4_WAVES: self.tune_get(TUNE_P4_WAVES),
-                                TUNE_T_KERNEL: self.tune_get(TUNE_T_KERNEL)}
+                                TUNE_T_KERNEL: self.tune_get(TUNE_T_KERNEL),
+                                TUNE_DST: self.tune_get(TUNE_DST), TUNE_FFT_E: self.tune_get(TUNE_FFT_E)}
         for knob, v in _TUNE_CACHE[key].items():
             if knob != "ms_per_step":
                 self.tune_set(knob, v)

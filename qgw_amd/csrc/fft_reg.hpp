@@ -20,8 +20,10 @@ namespace qgw {
 
 constexpr bool is_pow2(int L) { return L > 0 && (L & (L - 1)) == 0; }
 
-template <int L> struct RegPlan {
-  static constexpr int E = is_pow2(L) ? ((L >= 1024) ? 16 : 8) : 8;
+// EP: elements per thread (0: 16 for powers of two from 1024, else 8).  A
+// radix schedule of the transform: E = 16 runs radix-16 stages, E = 8 radix-8.
+template <int L, int EP = 0> struct RegPlan {
+  static constexpr int E = (EP > 0 && is_pow2(L)) ? EP : (is_pow2(L) ? ((L >= 1024) ? 16 : 8) : 8);
   static constexpr int NT = L / E;
   // radix of stage s (powers of two): E while the remaining length allows, then the rest
   static constexpr int radix(int s) {
@@ -140,9 +142,9 @@ __device__ __forceinline__ void twiddle(cx<T> *v, const cx<T> *__restrict__ tw, 
 // v[w*R + r] = X[j_w + r*L/R], j_w = t + w*NT.  On exit (S not last) v[] holds
 // the next stage's inputs; for the last stage v[w*R + r] is the output at
 // position t + (w + r*E/R)*NT.
-template <typename T, int L, int S>
+template <typename T, int L, int S, int EP = 0>
 __device__ __forceinline__ void reg_stage(cx<T> *v, cx<T> *buf, const cx<T> *__restrict__ tw, int t) {
-  using Pl = RegPlan<L>;
+  using Pl = RegPlan<L, EP>;
   constexpr int E = Pl::E, NT = Pl::NT;
   constexpr int R = Pl::radix(S);
   constexpr int Ns = Pl::ns(S);
@@ -172,36 +174,36 @@ __device__ __forceinline__ void reg_stage(cx<T> *v, cx<T> *buf, const cx<T> *__r
 #pragma unroll
       for (int r = 0; r < R2; r++) v[w * R2 + r] = buf[rslot(j + r * (L / R2))];
     }
-    reg_stage<T, L, S + 1>(v, buf, tw, t);
+    reg_stage<T, L, S + 1, EP>(v, buf, tw, t);
   }
 }
 
 // Home-position index helpers: element m of a thread sits at position
 // t + m*NT.  Stage inputs of the first stage and outputs of the last stage are
 // permutations of the home set; these map register slots to m.
-template <int L> constexpr int in_m(int slot) {
+template <int L, int EP = 0> constexpr int in_m(int slot) {
   // first stage radix R0: v[w*R0 + r] = X[t + w*NT + r*(E/R0)*NT]
-  constexpr int E = RegPlan<L>::E, R0 = RegPlan<L>::radix(0);
+  constexpr int E = RegPlan<L, EP>::E, R0 = RegPlan<L, EP>::radix(0);
   return (slot / R0) + (slot % R0) * (E / R0);
 }
-template <int L> constexpr int out_m(int slot) {
-  constexpr int E = RegPlan<L>::E, RL = RegPlan<L>::radix(RegPlan<L>::count() - 1);
+template <int L, int EP = 0> constexpr int out_m(int slot) {
+  constexpr int E = RegPlan<L, EP>::E, RL = RegPlan<L, EP>::radix(RegPlan<L, EP>::count() - 1);
   return (slot / RL) + (slot % RL) * (E / RL);
 }
 
 // Forward FFT of the sequence held at the home positions.  h[m] is the value
 // at position t + m*NT, in and out.  buf: RegPlan<L>::slots entries (powers of
 // two) or lds_slots(L) (other lengths).
-template <typename T, int L>
+template <typename T, int L, int EP = 0>
 __device__ __forceinline__ void fft_home(cx<T> *h, cx<T> *buf, const cx<T> *__restrict__ tw, int t) {
-  constexpr int E = RegPlan<L>::E, NT = RegPlan<L>::NT;
+  constexpr int E = RegPlan<L, EP>::E, NT = RegPlan<L, EP>::NT;
   if constexpr (is_pow2(L)) {
     cx<T> v[E];
 #pragma unroll
-    for (int s = 0; s < E; s++) v[s] = h[in_m<L>(s)];
-    reg_stage<T, L, 0>(v, buf, tw, t);
+    for (int s = 0; s < E; s++) v[s] = h[in_m<L, EP>(s)];
+    reg_stage<T, L, 0, EP>(v, buf, tw, t);
 #pragma unroll
-    for (int s = 0; s < E; s++) h[out_m<L>(s)] = v[s];
+    for (int s = 0; s < E; s++) h[out_m<L, EP>(s)] = v[s];
   } else {
     __syncthreads();
 #pragma unroll
@@ -213,6 +215,8 @@ __device__ __forceinline__ void fft_home(cx<T> *h, cx<T> *buf, const cx<T> *__re
   }
 }
 
-template <int L> constexpr int home_slots() { return is_pow2(L) ? RegPlan<L>::slots : lds_slots(L); }
+template <int L, int EP = 0> constexpr int home_slots() {
+  return is_pow2(L) ? RegPlan<L, EP>::slots : lds_slots(L);
+}
 
 }  // namespace qgw

@@ -29,6 +29,7 @@
 
 #include "fft.hpp"
 #include "fft_reg.hpp"
+#include "dst_half.hpp"
 
 namespace qgw {
 
@@ -329,6 +330,143 @@ k_p2(Geo g, T *__restrict__ whatT, const double *__restrict__ La2, const double 
     if (p >= 1 && p < L / 2) {
       row0[p - 1] = -h[m].y;
       if (has1) row1[p - 1] = h[m].x;
+    }
+  }
+}
+
+// ===========================================================================
+// Half-length forms of P1, P2 and P3 (dst_half.hpp): the same transforms and
+// outputs, through complex FFTs of length M = N + 1 instead of 2(N + 1).
+//   twh: e^{-2 pi i k / M} (length M)    tw2: e^{-2 pi i k / 2M} (length 2M)
+// EP: elements per thread of the FFT (its radix schedule; 0 = default).
+// ===========================================================================
+template <typename T, int M, int EP>
+__global__ void __launch_bounds__((DstHalf<M, EP>::NT))
+k_p1h(Geo g, const T *__restrict__ q, T *__restrict__ what, const cx<T> *__restrict__ twh,
+      const cx<T> *__restrict__ tw2) {
+  constexpr int E = DstHalf<M, EP>::E, NT = DstHalf<M, EP>::NT;
+  __shared__ cx<T> buf[DstHalf<M, EP>::slots];
+  const int t = threadIdx.x;
+  const int npair = (g.nl + 1) / 2;
+  const int lp = blockIdx.x % npair;
+  const int j = blockIdx.x / npair + 1;
+  const int l0 = 2 * lp, l1 = l0 + 1;
+  const bool has1 = l1 < g.nl;
+  const T *r0 = q + l0 * g.LS + (long)j * g.P;  // storage column x = 1..n holds x_1..x_N
+  const T *r1 = q + (has1 ? l1 : l0) * g.LS + (long)j * g.P;
+  cx<T> h[E];
+  dst_pre<T, M, EP>(h, t, tw2, [&](int x) { return mkc<T>(r0[x], has1 ? r1[x] : T(0)); });
+  fft_home<T, M, EP>(h, buf, twh, t);
+  dst_post<T, M, EP>(h, buf, t);
+  T *o0 = what + l0 * g.WS + (long)(j - 1) * g.Pn;
+  T *o1 = what + l1 * g.WS + (long)(j - 1) * g.Pn;
+  for (int i = t; i < M - 1; i += NT) {
+    const cx<T> v = buf[rslot(i)];
+    o0[i] = v.x;
+    if (has1) o1[i] = v.y;
+  }
+}
+
+template <typename T, int M, int EP>
+__global__ void __launch_bounds__((DstHalf<M, EP>::NT))
+k_p2h(Geo g, T *__restrict__ whatT, const double *__restrict__ La2, const double *__restrict__ Kb2,
+      const double *__restrict__ iRd2, double norm, const cx<T> *__restrict__ twh, const cx<T> *__restrict__ tw2) {
+  constexpr int E = DstHalf<M, EP>::E, NT = DstHalf<M, EP>::NT;
+  __shared__ cx<T> buf[DstHalf<M, EP>::slots];
+  const int t = threadIdx.x;
+  const int n = g.n;
+  const long rows = (long)g.nl * n;
+  const long q0 = 2L * blockIdx.x, q1 = q0 + 1;
+  const bool has1 = q1 < rows;
+  const int k0 = (int)(q0 / n), a0 = (int)(q0 % n) + 1;
+  const int k1 = has1 ? (int)(q1 / n) : k0, a1 = has1 ? (int)(q1 % n) + 1 : a0;
+  T *row0 = whatT + k0 * g.WTS + (long)(a0 - 1) * g.Pm;
+  T *row1 = whatT + k1 * g.WTS + (long)(a1 - 1) * g.Pm;
+  cx<T> h[E];
+  // forward transform along y (row element y-1 holds x_y)
+  dst_pre<T, M, EP>(h, t, tw2, [&](int y) { return mkc<T>(row0[y - 1], has1 ? row1[y - 1] : T(0)); });
+  fft_home<T, M, EP>(h, buf, twh, t);
+  dst_post<T, M, EP>(h, buf, t);
+  // Helmholtz division, f = 1 / (-(L_a^2 + K_b^2 + 1/Rd_k^2) * 4(n+1)(m+1)) (src/elliptic.h:120-138)
+  const double d0 = La2[a0], d1 = La2[a1];
+  const double ir0 = iRd2[k0], ir1 = iRd2[k1];
+  const bool lo0 = (k0 == 0 && a0 <= 16), lo1 = (k1 == 0 && a1 <= 16);
+  const T d0f = (T)(d0 + ir0), d1f = (T)(d1 + ir1), normf = (T)norm;
+  for (int i = t; i < M - 1; i += NT) {
+    const int b = i + 1;
+    const double kb = Kb2[b];
+    T f0, f1;
+    if constexpr (sizeof(T) == 8) {
+      f0 = (T)(1.0 / (-(d0 + kb + ir0) * norm));
+      f1 = (T)(1.0 / (-(d1 + kb + ir1) * norm));
+    } else {
+      // FP32 forms: FP64 only for the lowest barotropic wavenumbers (mode 0, a, b <= 16)
+      const T kbf = (T)kb;
+      f0 = T(1) / (-(d0f + kbf) * normf);
+      f1 = T(1) / (-(d1f + kbf) * normf);
+      if (b <= 16) {
+        if (lo0) f0 = (T)(1.0 / (-(d0 + kb + ir0) * norm));
+        if (lo1) f1 = (T)(1.0 / (-(d1 + kb + ir1) * norm));
+      }
+    }
+    const cx<T> v = buf[rslot(i)];
+    buf[rslot(i)] = mkc<T>(v.x * f0, v.y * f1);
+  }
+  __syncthreads();
+  // inverse transform (the same DST-I) of the divided spectra
+  dst_pre<T, M, EP>(h, t, tw2, [&](int y) { return buf[rslot(y - 1)]; });
+  fft_home<T, M, EP>(h, buf, twh, t);
+  dst_post<T, M, EP>(h, buf, t);
+  for (int i = t; i < M - 1; i += NT) {
+    const cx<T> v = buf[rslot(i)];
+    row0[i] = v.x;
+    if (has1) row1[i] = v.y;
+  }
+}
+
+template <typename T, int M, int EP>
+__global__ void __launch_bounds__((DstHalf<M, EP>::NT))
+k_p3h(Geo g, const T *__restrict__ what, T *__restrict__ psi, T *__restrict__ q, T ring_c,
+      const cx<T> *__restrict__ twh, const cx<T> *__restrict__ tw2) {
+  constexpr int E = DstHalf<M, EP>::E, NT = DstHalf<M, EP>::NT;
+  __shared__ cx<T> buf[DstHalf<M, EP>::slots];
+  const int t = threadIdx.x;
+  const int n = g.n, m_ = g.m;
+  const int npair = (g.nl + 1) / 2;
+  const int lp = blockIdx.x % npair;
+  const int j = blockIdx.x / npair + 1;
+  const int l0 = 2 * lp, l1 = l0 + 1;
+  const bool has1 = l1 < g.nl;
+  const T *i0 = what + l0 * g.WS + (long)(j - 1) * g.Pn;
+  const T *i1 = what + (has1 ? l1 : l0) * g.WS + (long)(j - 1) * g.Pn;
+  cx<T> h[E];
+  dst_pre<T, M, EP>(h, t, tw2, [&](int x) { return mkc<T>(i0[x - 1], has1 ? i1[x - 1] : T(0)); });
+  fft_home<T, M, EP>(h, buf, twh, t);
+  dst_post<T, M, EP>(h, buf, t);
+#pragma unroll
+  for (int side = 0; side < 2; side++) {
+    const int l = side ? l1 : l0;
+    if (side && !has1) break;
+    T *pr = psi + l * g.LS + (long)j * g.P;
+    T *qr = q + l * g.LS;
+    for (int i = t; i < n; i += NT) {
+      const int x = i + 1;
+      const cx<T> c = buf[rslot(i)];
+      const T v = side ? c.y : c.x;
+      pr[x] = v;
+      if (x == 1) qr[(long)j * g.P] = ring_c * v;
+      if (x == n) qr[(long)j * g.P + n + 1] = ring_c * v;
+      if (j == 1) qr[x] = ring_c * v;
+      if (j == m_) qr[(long)(m_ + 1) * g.P + x] = ring_c * v;
+    }
+    if (t == 0) {
+      pr[0] = T(0);
+      pr[n + 1] = T(0);
+    }
+    if (j == 1 || j == m_) {
+      const long rr = (long)(j == 1 ? 0 : m_ + 1) * g.P;
+      T *pw = psi + l * g.LS + rr;
+      for (int x = t; x < n + 2; x += NT) pw[x] = T(0);
     }
   }
 }

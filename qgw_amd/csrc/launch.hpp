@@ -26,6 +26,9 @@ struct LaunchArgs {
   hipStream_t stream = nullptr;
   void *q = nullptr, *qn = nullptr, *psi = nullptr, *what = nullptr, *whatT = nullptr;
   void *cl2m = nullptr, *cm2l = nullptr, *twx = nullptr, *twy = nullptr;
+  void *twxh = nullptr, *twyh = nullptr;  // length-(L/2) tables of the half-length transforms
+  int dst_half = 0;                       // 1: P1/P2/P3 through FFTs of length L/2
+  int fft_e = 16;                         // elements per thread of those FFTs (8 or 16, lengths >= 1024)
   void *fnew = nullptr, *f1 = nullptr, *f2 = nullptr, *comp = nullptr, *topo = nullptr, *wind = nullptr;
   const double *La2 = nullptr, *Kb2 = nullptr, *iRd2 = nullptr;
   double norm = 1.0, ring_c = 0.0;
@@ -49,6 +52,7 @@ template <typename T> int launch_diag_t(const LaunchArgs &a, double *part, int n
 template <typename T> int launch_from_f64_t(const LaunchArgs &a, int layers, const double *src, void *dst, void *resid);
 template <typename T> int launch_to_f64_t(const LaunchArgs &a, int layers, const void *src, const void *resid, double *dst);
 bool launch_has_length(int L);
+bool launch_has_half_length(int M);  // lengths M = L/2 of the half-length transforms
 
 // The specialisations live in the two instantiation units.
 #define QGW_DECLARE_SPECS(T)                                                                          \

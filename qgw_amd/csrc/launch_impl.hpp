@@ -16,8 +16,38 @@ using T = QGW_T;
   X(8) X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(2048) X(4096) X(8192) \
   X(24) X(48) X(96) X(192) X(384) X(768) X(1536) X(3072) X(6144)
 
+// Lengths M of the half-length transforms (M = L/2 >= 64).
+#define QGW_HLENGTHS(X) X(64) X(128) X(256) X(512) X(1024) X(2048) X(4096) X(96) X(192) X(384) X(768) X(1536) X(3072)
+// Radix schedule of a half-length transform: powers of two from 1024 run with
+// 16 (radix-16 stages) or 8 (radix-8) elements per thread, a.fft_e.
+#define QGW_HALF_EP(M)                                    \
+  if constexpr (is_pow2(M) && (M) >= 1024) {              \
+    if (a.fft_e == 8) {                                   \
+      QGW_GO(M, 8)                                        \
+    } else {                                              \
+      QGW_GO(M, 16)                                       \
+    }                                                     \
+  } else {                                                \
+    QGW_GO(M, 0)                                          \
+  }
+
 template <> int launch_p1_t<T>(const LaunchArgs &a) {
   const dim3 grid(a.g.m * ((a.g.nl + 1) / 2));
+  if (a.dst_half) {
+    switch (a.Lx / 2) {
+#define QGW_GO(M, EP)                                                                                  \
+  hipLaunchKernelGGL((k_p1h<T, M, EP>), grid, dim3(DstHalf<M, EP>::NT), 0, a.stream, a.g, (const T *)a.q, \
+                     (T *)a.what, (const cx<T> *)a.twxh, (const cx<T> *)a.twx);
+#define QGW_CASE(M) \
+  case M:           \
+    QGW_HALF_EP(M)  \
+    return 0;
+      QGW_HLENGTHS(QGW_CASE)
+#undef QGW_CASE
+#undef QGW_GO
+    }
+    return QGW_ERR_ARG;
+  }
   switch (a.Lx) {
 #define QGW_CASE(L)                                                                                 \
   case L:                                                                                           \
@@ -32,6 +62,21 @@ template <> int launch_p1_t<T>(const LaunchArgs &a) {
 
 template <> int launch_p3_t<T>(const LaunchArgs &a) {
   const dim3 grid(a.g.m * ((a.g.nl + 1) / 2));
+  if (a.dst_half) {
+    switch (a.Lx / 2) {
+#define QGW_GO(M, EP)                                                                                    \
+  hipLaunchKernelGGL((k_p3h<T, M, EP>), grid, dim3(DstHalf<M, EP>::NT), 0, a.stream, a.g, (const T *)a.what, \
+                     (T *)a.psi, (T *)a.q, (T)a.ring_c, (const cx<T> *)a.twxh, (const cx<T> *)a.twx);
+#define QGW_CASE(M) \
+  case M:           \
+    QGW_HALF_EP(M)  \
+    return 0;
+      QGW_HLENGTHS(QGW_CASE)
+#undef QGW_CASE
+#undef QGW_GO
+    }
+    return QGW_ERR_ARG;
+  }
   switch (a.Lx) {
 #define QGW_CASE(L)                                                                                    \
   case L:                                                                                              \
@@ -47,6 +92,21 @@ template <> int launch_p3_t<T>(const LaunchArgs &a) {
 template <> int launch_p2_t<T>(const LaunchArgs &a) {
   const long rows = (long)a.g.nl * a.g.n;
   const dim3 grid((unsigned)((rows + 1) / 2));
+  if (a.dst_half) {
+    switch (a.Ly / 2) {
+#define QGW_GO(M, EP)                                                                                \
+  hipLaunchKernelGGL((k_p2h<T, M, EP>), grid, dim3(DstHalf<M, EP>::NT), 0, a.stream, a.g, (T *)a.whatT, \
+                     a.La2, a.Kb2, a.iRd2, a.norm, (const cx<T> *)a.twyh, (const cx<T> *)a.twy);
+#define QGW_CASE(M) \
+  case M:           \
+    QGW_HALF_EP(M)  \
+    return 0;
+      QGW_HLENGTHS(QGW_CASE)
+#undef QGW_CASE
+#undef QGW_GO
+    }
+    return QGW_ERR_ARG;
+  }
   switch (a.Ly) {
 #define QGW_CASE(L)                                                                               \
   case L:                                                                                         \

@@ -53,13 +53,15 @@ struct qgw_ctx {
   hipStream_t stream;
   void *q[2];
   int qcur;
-  void *psi, *fa[3], *comp, *topo, *wind, *what, *whatT, *cl2m, *cm2l, *twx, *twy;
+  void *psi, *fa[3], *comp, *topo, *wind, *what, *whatT, *cl2m, *cm2l, *twx, *twy, *twxh, *twyh;
   double *iRd2_d, *La2, *Kb2, *diag_part, *diag_out, *stage;
   unsigned long long *cfl_max;
   int nblk_diag;
   int p4_tile;
   int t_kernel;  // projection kernel choice (QGW_TUNE_T_KERNEL)
   int p4_waves;  // streaming P4 wavefronts per SIMD (QGW_TUNE_P4_WAVES)
+  int dst_half;  // half-length sine transforms (QGW_TUNE_DST); -1: lengths not provided
+  int fft_e;     // elements per thread of the half-length FFTs (QGW_TUNE_FFT_E)
   bool modes_set;
   double t, dt, dt0, dt1, DT_max, t_out;
   int it;
@@ -242,6 +244,10 @@ int run_invert(qgw_ctx *c) {
   a.Kb2 = c->Kb2;
   a.iRd2 = c->iRd2_d;
   a.t_kernel = c->t_kernel;
+  a.twxh = c->twxh;
+  a.twyh = c->twyh;
+  a.dst_half = c->dst_half > 0;
+  a.fft_e = c->fft_e;
   a.norm = 4.0 * (double)(c->g.n + 1) * (double)(c->g.m + 1);
   a.ring_c = 2.0 * c->p.bc_fac / (c->grid.Delta * c->grid.Delta);
   int st;
@@ -623,9 +629,13 @@ qgw_ctx *create_group(const qgw_params *ps, int nb, int precision, hipStream_t e
     if (precision == QGW_FP64) {
       st = make_twiddles<double>(c, Lx, &c->twx);
       if (!st) st = make_twiddles<double>(c, Ly, &c->twy);
+      if (!st) st = make_twiddles<double>(c, Lx / 2, &c->twxh);
+      if (!st) st = make_twiddles<double>(c, Ly / 2, &c->twyh);
     } else {
       st = make_twiddles<float>(c, Lx, &c->twx);
       if (!st) st = make_twiddles<float>(c, Ly, &c->twy);
+      if (!st) st = make_twiddles<float>(c, Lx / 2, &c->twxh);
+      if (!st) st = make_twiddles<float>(c, Ly / 2, &c->twyh);
     }
   }
   if (st) {
@@ -674,6 +684,8 @@ qgw_ctx *create_group(const qgw_params *ps, int nb, int precision, hipStream_t e
   c->qcur = 0;
   c->p4_tile = qgw::kP4TileCount;  // the register-streaming form
   c->p4_waves = 32;
+  c->fft_e = (precision == QGW_FP64) ? 16 : 8;
+  c->dst_half = (qgw::launch_has_half_length(Lx / 2) && qgw::launch_has_half_length(Ly / 2)) ? 1 : -1;
   c->modes_set = false;
   // the zero fills above ran on the null stream; the step runs on c->stream
   if (hipDeviceSynchronize() != hipSuccess) {
@@ -792,6 +804,26 @@ int autotune_group(qgw_ctx *c, int reps, int *p4_tile, double *best_ms) {
       }
     }
     c->t_kernel = targ;
+  }
+  if (!st && c->dst_half >= 0) {
+    // sine transforms: odd-extension FFTs, or half-length FFTs with each radix schedule
+    double bt = 1e300;
+    int darg = c->dst_half, earg = c->fft_e;
+    const int forms[3][2] = {{0, 16}, {1, 16}, {1, 8}};
+    for (int v = 0; v < 3 && !st; v++) {
+      c->dst_half = forms[v][0];
+      c->fft_e = forms[v][1];
+      double ms = 0;
+      st = time_steps(c, 1, &ms);
+      if (!st) st = time_steps(c, reps, &ms);
+      if (!st && ms < bt) {
+        bt = ms;
+        darg = forms[v][0];
+        earg = forms[v][1];
+      }
+    }
+    c->dst_half = darg;
+    c->fft_e = earg;
   }
   int st2 = snapshot_restore(c, snap);
   if (st) return st;
@@ -1101,6 +1133,16 @@ int qgw_tune_set(qgw_ctx *c, int knob, int value) {
     c->p4_tile = value;
     return QGW_OK;
   }
+  if (knob == QGW_TUNE_FFT_E) {
+    if (value != 8 && value != 16) return QGW_ERR_ARG;
+    c->fft_e = value;
+    return QGW_OK;
+  }
+  if (knob == QGW_TUNE_DST) {
+    if (value < 0 || value > 1 || (value == 1 && c->dst_half < 0)) return QGW_ERR_ARG;
+    if (c->dst_half >= 0) c->dst_half = value;
+    return QGW_OK;
+  }
   if (knob == QGW_TUNE_P4_WAVES) {
     if (value != 2 && value != 4 && value != 8 && value != 16 && value != 32) return QGW_ERR_ARG;
     c->p4_waves = value;
@@ -1126,6 +1168,14 @@ int qgw_tune_get(qgw_ctx *c, int knob, int *value) {
   }
   if (knob == QGW_TUNE_P4_WAVES) {
     *value = c->p4_waves;
+    return QGW_OK;
+  }
+  if (knob == QGW_TUNE_DST) {
+    *value = c->dst_half > 0 ? 1 : 0;
+    return QGW_OK;
+  }
+  if (knob == QGW_TUNE_FFT_E) {
+    *value = c->fft_e;
     return QGW_OK;
   }
   return QGW_ERR_ARG;

@@ -12,4 +12,12 @@ bool launch_has_length(int L) {
   }
   return false;
 }
+bool launch_has_half_length(int M) {
+  switch (M) {
+#define QGW_HAS(Mv) case Mv: return true;
+    QGW_HLENGTHS(QGW_HAS)
+#undef QGW_HAS
+  }
+  return false;
+}
 }  // namespace qgw
