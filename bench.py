@@ -7,18 +7,23 @@ initial condition.  A step is one full AB3 step of the model (PV inversion by
 sine transforms and vertical modes, Arakawa Jacobian, dissipation, forcing,
 update).  The same initial state is stepped in two forms:
 
-  * FP64 plain        -- every field and pass in FP64
-  * FP32 compensated  -- FP32 fields and passes, Kahan-compensated AB3
+  * FP64 plain        -- every field and pass in FP64, one launch at a time
+                         through the per-call C-ABI (qgw_run)
+  * FP32 batched      -- FP32 fields and passes, Kahan-compensated AB3
                          increment (FP64 only for the Helmholtz denominators
-                         and the energy/enstrophy sums)
+                         and the energy/enstrophy sums), stepped through the
+                         ensemble C-ABI (qgw_batch_step: a captured hipGraph
+                         of six steps, replayed)
 
-`value` is the FP32 form's grid-point.layer.steps/s (NX*NY*nl*steps/s) with
-all inputs resident in HBM, timed with HIP events on the model's stream over
-exactly --steps steps after --warmup untimed steps.  The line also carries
-the FP64 figure and the ratio, the energy/enstrophy error of the FP32 form
-against the FP64 form after --accuracy-steps steps, a roofline entry for the
-dominant kernel and a CPU baseline (the FP64 oracle, a C restatement of the
-reference step, on the host cores, over a bounded sample).
+`value` is the FP32 batched form's grid-point.layer.steps/s (NX*NY*nl*steps/s)
+with all inputs resident in HBM, timed with HIP events on the stream the
+kernels run on, over exactly --steps steps after --warmup untimed steps.  The
+line also carries the FP64 figure and the ratio, the energy/enstrophy error of
+the FP32 form against the FP64 form after --accuracy-steps steps, a roofline
+entry for the dominant kernel, the 96-member ensemble (uneven sizes, FP32,
+dt 1200 s) as a second throughput figure, and a CPU baseline (the FP64 oracle,
+a C restatement of the reference step, on the host cores, over a bounded
+sample).
 """
 import argparse
 import json
@@ -26,6 +31,7 @@ import os
 import platform
 import sys
 import time
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 
@@ -38,6 +44,10 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
 # element size s (FP64: 8, FP32: 4).  P4 with Kahan adds a read and a write of
 # the compensation array.
 PASS_WORDS = {"P1": 2, "T1": 2, "P2": 2, "T2": 2, "P3": 2, "P4": 6}
+# Kernel name prefixes of each pass, in order of preference (the profile holds
+# whichever form ran).
+PASS_KERNELS = {"P1": ("k_p1h<", "k_p1<"), "T1": ("k_tproj_mfma<", "k_tproj<"), "P2": ("k_p2h<", "k_p2<"),
+                "T2": ("k_tproj_mfma<", "k_tproj<"), "P3": ("k_p3h<", "k_p3<"), "P4": ("k_p4s<", "k_p4<")}
 
 
 def log(*a):
@@ -55,34 +65,57 @@ def cpu_model():
     return platform.processor() or "unknown"
 
 
-def run_form(Q, p, modes, q0, prec, steps, warmup, prof_steps, tune=True):
-    m = Q.Model(p, prec, modes)
-    tuned = m.autotune() if tune else None
+def run_fp64_plain(Q, p, modes, q0, steps, warmup, prof_steps, tune):
+    m = Q.Model(p, Q.FP64, modes)
     m.set_q(q0)
+    if tune:
+        m.autotune()
+    tuning = m.tuning()
     m.run(warmup)
     m.synchronize()
     ms = m.time_run(steps)
-    # per-pass timing on a few more steps (events around every launch)
     m.profile(True)
     m.run(prof_steps)
     m.profile(False)
     passes = m.profile_read()
     m.close()
-    return ms, passes, tuned
+    return ms, passes, tuning
+
+
+def run_fp32_batched(Q, p, modes, q0, steps, warmup, prof_steps, tune):
+    b = Q.Batch([p], Q.FP32, [modes])
+    b.set_q(0, q0)
+    if tune:
+        b.autotune(reps=5)
+    tuning = b.tuning(0)
+    b.step(warmup)
+    b.synchronize()
+    g0 = b.graph_launches()
+    ms = b.time_step(steps)
+    replays = b.graph_launches() - g0
+    b.profile(True)  # per-pass events: these steps run launch by launch
+    b.step(prof_steps)
+    b.profile(False)
+    passes = b.profile_read()
+    b.close()
+    return ms, passes, tuning, replays
 
 
 def accuracy(Q, p, modes, q0, nsteps):
-    out = {}
-    for name, prec in (("fp64", Q.FP64), ("fp32", Q.FP32)):
-        m = Q.Model(p, prec, modes)
-        m.set_q(q0)
-        m.run(nsteps)
-        m.invert_pv()
-        El, Zl, E, Z = m.diagnostics()
-        out[name] = (E, Z, m.q, m.psi)
-        m.close()
-    E64, Z64, q64, p64 = out["fp64"]
-    E32, Z32, q32, p32 = out["fp32"]
+    m = Q.Model(p, Q.FP64, modes)
+    m.set_q(q0)
+    m.run(nsteps)
+    m.invert_pv()
+    E64, Z64 = m.diagnostics()[2:]
+    q64, p64 = m.q, m.psi
+    m.close()
+    b = Q.Batch([p], Q.FP32, [modes])
+    b.set_q(0, q0)
+    b.step(nsteps)
+    b.invert_pv()
+    E32, Z32 = b.diagnostics(0)[2:]
+    q32, p32 = b.q(0), b.psi(0)
+    b.close()
     return {
         "steps": nsteps,
         "energy_rel_err": abs(E32 - E64) / abs(E64),
@@ -92,6 +125,33 @@ def accuracy(Q, p, modes, q0, nsteps):
         "energy_fp64": E64,
         "enstrophy_fp64": Z64,
     }
+
+
+def ensemble(Q, configs, steps):
+    """The 96-member ensemble of uneven sizes in one batch (FP32 + Kahan)."""
+    ps = configs.c4()
+    t0 = time.perf_counter()
+    modes = [Q.eigmode(p) for p in ps]
+    with ThreadPoolExecutor(4) as ex:
+        ics = list(ex.map(lambda i: Q.make_ic(ps[i], modes[i])[1], range(len(ps))))
+    t_ic = time.perf_counter() - t0
+    b = Q.Batch(ps, Q.FP32, modes)
+    for i, q in enumerate(ics):
+        b.set_q(i, q)
+    del ics
+    ngroups = b.groups()[0]
+    b.autotune(reps=2)
+    b.step(6)
+    b.synchronize()
+    ms = b.time_step(steps)
+    work = sum(p.NX * p.NY * p.nl for p in ps)
+    out = {"members": len(ps), "groups": ngroups,
+           "classes": "32 x (512^2, 2), 32 x (1024^2, 3), 24 x (1536^2, 5), 8 x (2048^2, 4); FP32 + Kahan, dt 1200 s",
+           "value": work * steps / (ms / 1e3), "unit": "grid-pt*layer*steps/s", "ms_per_step": ms / steps,
+           "steps": steps, "graph_replays": b.graph_launches(), "device_GB": b.device_bytes() / 1e9,
+           "tuning": [b.tuning(g) for g in range(ngroups)], "initial_condition_s": t_ic}
+    b.close()
+    return out
 
 
 def cpu_baseline(p, modes, q0, seconds):
@@ -115,18 +175,34 @@ def cpu_baseline(p, modes, q0, seconds):
                       "(gcc -O3, OpenMP), %.1f s; CPU: %s" % (nsteps, el, cpu_model())}
 
 
+def kernel_traffic(dom, head):
+    """HBM bytes per launch of the pass's kernel from the committed counter profile."""
+    tpath = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(tpath):
+        return None, None
+    kernels = json.load(open(tpath))["kernels"]
+    ty = "double" if head == "fp64" else "float"
+    for prefix in PASS_KERNELS[dom]:
+        for kname, tv in kernels.items():
+            if kname.startswith(prefix + ty):
+                return tv["fetch_bytes"] + tv["write_bytes"], kname
+    return None, None
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=198)
+    ap.add_argument("--warmup", type=int, default=12)
     ap.add_argument("--nx", type=int, default=2048)
     ap.add_argument("--accuracy-steps", type=int, default=1000)
     ap.add_argument("--profile-steps", type=int, default=10)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--ensemble-steps", type=int, default=24)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--forms", default="fp64,fp32")
-    ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (default tile shapes)")
+    ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (default tuning)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the hot path is a single-GPU build (the ensemble and "
@@ -147,17 +223,21 @@ def main():
     work = p.NX * p.NY * p.nl
     g = Q.grid(p)[0]
 
-    res = {}
-    tunes = {}
+    res, tunes, replays = {}, {}, None
     for form in args.forms.split(","):
-        prec = {"fp64": Q.FP64, "fp32": Q.FP32, "fp32plain": Q.FP32_PLAIN}[form]
         log("timing form", form)
-        ms, passes, tuned = run_form(Q, p, modes, q0, prec, args.steps, args.warmup, args.profile_steps,
-                                     not args.no_tune)
+        if form == "fp64":
+            ms, passes, tuning = run_fp64_plain(Q, p, modes, q0, args.steps, args.warmup, args.profile_steps,
+                                                not args.no_tune)
+        elif form == "fp32":
+            ms, passes, tuning, replays = run_fp32_batched(Q, p, modes, q0, args.steps, args.warmup,
+                                                           args.profile_steps, not args.no_tune)
+        else:
+            raise SystemExit("unknown form " + form)
         res[form] = (ms, passes)
-        tunes[form] = tuned
-        log("  %s: %.3f ms/step" % (form, ms / args.steps), {k: round(v[0] / max(v[1], 1), 4)
-                                                          for k, v in passes.items() if v[1]})
+        tunes[form] = tuning
+        log("  %s: %.3f ms/step" % (form, ms / args.steps), tuning,
+            {k: round(v[0] / max(v[1], 1), 4) for k, v in passes.items() if v[1]})
 
     head = "fp32" if "fp32" in res else list(res)[0]
     ms, passes = res[head]
@@ -178,16 +258,7 @@ def main():
                   for k in per_launch}
     total_alg = sum(words.values()) * s * work
     step_gbs = total_alg / (ms_step * 1e-3) / 1e9
-
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic.json")
-    if os.path.exists(tpath):
-        prefix = "k_%s<%s" % (dom.lower(), "double" if head == "fp64" else "float")
-        if dom in ("T1", "T2"):
-            prefix = "k_tproj<%s" % ("double" if head == "fp64" else "float")
-        for kname, tv in json.load(open(tpath))["kernels"].items():
-            if kname.startswith(prefix):
-                traffic = tv["fetch_bytes"] + tv["write_bytes"]
+    traffic, tkernel = kernel_traffic(dom, head)
     out = {
         "metric": "grid-pt*layer*steps/s at 2048^2x10, FP32 batched vs FP64 plain, <=1e-4 energy err",
         "value": value,
@@ -202,11 +273,11 @@ def main():
         "dtype": "fp32" if head != "fp64" else "fp64",
         "data": "synthetic (seeded k^-3 initial condition, seeded stratification)",
         "config": {"workload": "2048^2 x 10-layer QG step (L=4000 km, dt=900 s, nu4=2e8, double-gyre wind)",
-                   "nx": p.NX, "ny": p.NY, "nl": p.nl, "form": head + ("+kahan" if head == "fp32" else ""),
-                   "interior_points": g.n * g.m,
-                   "p4_tile": {f: (Q.P4_TILES[t[Q.TUNE_P4_TILE]] if t else "stream") for f, t in tunes.items()}},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                   "nx": p.NX, "ny": p.NY, "nl": p.nl,
+                   "form": head + ("+kahan, batched (hipGraph replay)" if head == "fp32" else ""),
+                   "interior_points": g.n * g.m, "graph_replays": replays, "tuning": tunes},
+        "roofline": {"bound": "hbm", "kernel": dom, "kernel_name": tkernel, "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "algorithmic_bytes": alg_bytes,
                      "traffic_source": "rocprofv3 FETCH_SIZE + WRITE_SIZE, profiles/traffic.json"},
         "step_algorithmic_GBs": step_gbs,
@@ -221,6 +292,9 @@ def main():
     if args.accuracy_steps > 0 and "fp64" in res and "fp32" in res:
         log("accuracy leg: %d steps per form" % args.accuracy_steps)
         out["accuracy"] = accuracy(Q, p, modes, q0, args.accuracy_steps)
+    if not args.no_ensemble:
+        log("ensemble leg")
+        out["ensemble"] = ensemble(Q, configs, args.ensemble_steps)
     if not args.no_cpu:
         log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(p, modes, q0, args.cpu_seconds)

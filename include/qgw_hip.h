@@ -222,6 +222,7 @@ int qgw_batch_graph_launches(qgw_batch *b, int *count);
  * group k is returned in p4_tile_per_group[k] (may be NULL). */
 int qgw_batch_autotune(qgw_batch *b, int reps, int *p4_tile_per_group);
 int qgw_batch_tune_set(qgw_batch *b, int group, int knob, int value);
+int qgw_batch_tune_get(qgw_batch *b, int group, int knob, int *value);
 /* Per-pass timing summed over groups (steps then run without the graph). */
 int qgw_batch_profile_enable(qgw_batch *b, int on);
 int qgw_batch_profile_read(qgw_batch *b, double *ms, int *count, int npass);

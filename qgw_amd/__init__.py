@@ -175,7 +175,7 @@ def hip_lib():
             ("qgw_batch_diagnostics", [vp, ci, _dp, _dp, _dp, _dp]),
             ("qgw_batch_get_time", [vp, _dp, ip]), ("qgw_batch_use_graph", [vp, ci]),
             ("qgw_batch_graph_launches", [vp, ip]), ("qgw_batch_autotune", [vp, ci, ip]),
-            ("qgw_batch_tune_set", [vp, ci, ci, ci]),
+            ("qgw_batch_tune_set", [vp, ci, ci, ci]), ("qgw_batch_tune_get", [vp, ci, ci, ip]),
             ("qgw_batch_profile_enable", [vp, ci]), ("qgw_batch_profile_read", [vp, _dp, ip, ci]),
             ("qgw_batch_device_bytes", [vp, ctypes.POINTER(ctypes.c_size_t)]),
         ]:
@@ -420,6 +420,13 @@ class Model:
         self._chk(self.lib.qgw_tune_get(self.ctx, knob, ctypes.byref(v)), "tune_get")
         return v.value
 
+    def tuning(self):
+        """The tuning choices, by name."""
+        return {"p4": P4_TILES[self.tune_get(TUNE_P4_TILE)], "p4_waves": self.tune_get(TUNE_P4_WAVES),
+                "t_kernel": ["auto", "valu", "mfma32", "mfma16"][self.tune_get(TUNE_T_KERNEL)],
+                "dst": ["odd-extension", "half-length"][self.tune_get(TUNE_DST)],
+                "fft_e": self.tune_get(TUNE_FFT_E)}
+
     def autotune(self, reps=5):
         """Pick the tuning of this model by timing each variant on its own
         state (qgw_autotune copies the state first and restores it after).
@@ -562,6 +569,19 @@ class Batch:
 
     def tune_set(self, group, knob, value):
         self._chk(self.lib.qgw_batch_tune_set(self.b, group, knob, value), "tune_set")
+
+    def tune_get(self, group, knob):
+        v = ctypes.c_int()
+        self._chk(self.lib.qgw_batch_tune_get(self.b, group, knob, ctypes.byref(v)), "tune_get")
+        return v.value
+
+    def tuning(self, group=0):
+        """The tuning choices of one group, by name."""
+        return {"p4": P4_TILES[self.tune_get(group, TUNE_P4_TILE)],
+                "p4_waves": self.tune_get(group, TUNE_P4_WAVES),
+                "t_kernel": ["auto", "valu", "mfma32", "mfma16"][self.tune_get(group, TUNE_T_KERNEL)],
+                "dst": ["odd-extension", "half-length"][self.tune_get(group, TUNE_DST)],
+                "fft_e": self.tune_get(group, TUNE_FFT_E)}
 
     def profile(self, on=True):
         self._chk(self.lib.qgw_batch_profile_enable(self.b, 1 if on else 0), "profile_enable")

@@ -1372,6 +1372,11 @@ int qgw_batch_tune_set(qgw_batch *b, int group, int knob, int value) {
   return st;
 }
 
+int qgw_batch_tune_get(qgw_batch *b, int group, int knob, int *value) {
+  if (!b || group < 0 || group >= (int)b->groups.size()) return QGW_ERR_ARG;
+  return qgw_tune_get(b->groups[group], knob, value);
+}
+
 int qgw_batch_profile_enable(qgw_batch *b, int on) {
   if (!b) return QGW_ERR_ARG;
   for (auto *c : b->groups) {

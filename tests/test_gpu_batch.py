@@ -13,6 +13,9 @@ from conftest import rel
 
 pytestmark = pytest.mark.gpu
 
+# every knob the autotuner sets
+KNOBS = (Q.TUNE_P4_TILE, Q.TUNE_P4_WAVES, Q.TUNE_T_KERNEL, Q.TUNE_DST, Q.TUNE_FFT_E)
+
 
 def _announce(*a):
     print("[case]", *a, flush=True)
@@ -132,8 +135,9 @@ def test_batch_autotune_preserves_state():
     b2.step(4)
     choice = b1.autotune(reps=2)
     assert len(choice) == 3
-    for g, name in enumerate(choice):
-        b2.tune_set(g, Q.TUNE_P4_TILE, Q.P4_TILES.index(name))
+    for g in range(3):
+        for knob in KNOBS:
+            b2.tune_set(g, knob, b1.tune_get(g, knob))
     assert b1.get_time()["it"] == 4
     b1.step(8)
     b2.step(8)
@@ -155,7 +159,9 @@ def test_model_autotune_preserves_state():
     m2.run(3)
     t, ms = Q.ctypes.c_int(), Q.ctypes.c_double()
     m1._chk(m1.lib.qgw_autotune(m1.ctx, 2, Q.ctypes.byref(t), Q.ctypes.byref(ms)), "autotune")
-    m2.tune_set(Q.TUNE_P4_TILE, t.value)
+    assert m1.tune_get(Q.TUNE_P4_TILE) == t.value
+    for knob in KNOBS:
+        m2.tune_set(knob, m1.tune_get(knob))
     assert m1.get_time()["it"] == 3
     m1.run(5)
     m2.run(5)
