@@ -13,7 +13,10 @@
 // -- the sine-transform recurrence of Press et al., Numerical Recipes, sec.
 // 12.3 ("sinft"), applied to both halves of the complex FFT at once.  The
 // running sum is a block-wide prefix sum: each thread owns M/(2 NT)
-// consecutive k.
+// consecutive k.  It is accumulated in FP64 in every form: its partial sums
+// are of the size of the largest coefficient, so an FP32 sum would leave an
+// error of ~eps |S_1| sqrt(k) on the small high-wavenumber coefficients,
+// which the hyperviscous term amplifies.
 #pragma once
 #include "fft_reg.hpp"
 
@@ -85,26 +88,28 @@ template <typename T, int M, int EP> __device__ __forceinline__ void dst_post(cx
 #pragma unroll
   for (int m = 0; m < E; m++) buf[rslot(t + m * NT)] = h[m];
   __syncthreads();
-  cx<T> ev[KPT], cum[KPT];
-  cx<T> run = mkc<T>(T(0), T(0));
+  cx<T> ev[KPT];
+  cx<double> cum[KPT];
+  cx<double> run = mkc<double>(0.0, 0.0);
 #pragma unroll
   for (int i = 0; i < KPT; i++) {
     const int k = t * KPT + i;
     const cx<T> Yk = buf[rslot(k)];
     const cx<T> Ym = buf[rslot(k == 0 ? 0 : M - k)];
     ev[i] = mkc<T>(Ym.y - Yk.y, Yk.x - Ym.x);
-    cx<T> r = mkc<T>(Yk.x + Ym.x, Yk.y + Ym.y);
-    if (k == 0) r = mkc<T>(T(0.5) * r.x, T(0.5) * r.y);
-    run = cadd<T>(run, r);
+    cx<double> r = mkc<double>((double)Yk.x + (double)Ym.x, (double)Yk.y + (double)Ym.y);
+    if (k == 0) r = mkc<double>(0.5 * r.x, 0.5 * r.y);
+    run = cadd<double>(run, r);
     cum[i] = run;
   }
-  const cx<T> off = block_excl_scan<T, NT>(run, t);
+  const cx<double> off = block_excl_scan<double, NT>(run, t);
   __syncthreads();  // every read of Y is done
 #pragma unroll
   for (int i = 0; i < KPT; i++) {
     const int k = t * KPT + i;
     if (k >= 1) buf[rslot(2 * k - 1)] = ev[i];
-    buf[rslot(2 * k)] = cadd<T>(off, cum[i]);
+    const cx<double> o = cadd<double>(off, cum[i]);
+    buf[rslot(2 * k)] = mkc<T>((T)o.x, (T)o.y);
   }
   __syncthreads();
 }
