@@ -38,8 +38,9 @@ void qgwh_default_params(qgw_params *p);
  * this returns QGW_ERR_IO. */
 int qgwh_read_params(const char *path, qgw_params *p);
 
-/* Grid geometry of a bounded run (src/domain.h:64-176).  n, m are the
- * interior point counts (NX-1, NY-1); storage per layer is (n+2)*(m+2). */
+/* Grid geometry (src/domain.h:64-176).  n, m are the interior point counts:
+ * NX-1, NY-1 for bounded runs (bc_fac in [0, 1]), NX, NY for periodic runs
+ * (bc_fac = -1); storage per layer is (n+2)*(m+2) with a one-point ring. */
 typedef struct qgw_grid {
   int n, m;           /* interior points in x and y */
   int nxp2, nyp2;     /* storage extents (n+2, m+2) */
@@ -48,9 +49,14 @@ typedef struct qgw_grid {
 } qgw_grid;
 
 /* Compute the grid geometry; X (nxp2), Y (nyp2), Kw (nyp2) and Lw (nxp2) may
- * be NULL.  Kw[j] = pi*j/Lx for j=1..m and Lw[i] = pi*i/Lx for i=1..n, exactly
- * as src/domain.h:169-174 (including the use of Lx for K).  Returns QGW_ERR_ARG
- * for periodic (bc_fac = -1) or inconsistent inputs. */
+ * be NULL.  Bounded: X[i] = i*Delta, Kw[j] = pi*j/Lx for j=1..m and
+ * Lw[i] = pi*i/Lx for i=1..n, exactly as src/domain.h:169-174 (including the
+ * use of Lx for K).  Periodic (src/domain.h:163-167): X[i] = (i-1)*Delta,
+ * Kw[i] = 2 pi i/Lx for i = 0..NX/2 (the half-spectrum x wavenumbers) and
+ * Lw[j] = 2 pi/Lx * (((j + NX/2) mod NX) - NX/2) for j = 0..NX-1 (signed, y);
+ * a periodic grid must be square with NX even (the serial reference's r2c
+ * layout, src/elliptic.h:32-47).  Returns QGW_ERR_ARG for inconsistent
+ * inputs. */
 int qgwh_grid(const qgw_params *p, qgw_grid *g, double *X, double *Y, double *Kw, double *Lw);
 
 /* Bathtub topography on the full storage grid, (m+2)*(n+2) values, row-major
@@ -77,8 +83,9 @@ int qgwh_gamma(const qgw_params *p, double *gamma);
 int qgwh_wind_row(const qgw_params *p, double *forc_row);
 
 /* Seeded initial condition (no counterpart in the reference, which reads q
- * from restart.nc, src/qg.c:160-164).  Per layer, a sine series with
- * Gaussian coefficients of variance ~ E(kappa)/kappa^3,
+ * from restart.nc, src/qg.c:160-164).  Per layer, a sine series (bounded) or
+ * a Fourier series (periodic, zero mean, periodic rings) with Gaussian
+ * coefficients of variance ~ E(kappa)/kappa^3,
  * E(kappa) = kappa^3/(kappa0^6 + kappa^6), kappa0 = 2 pi/ic_lambda0, is
  * transformed to psi, rescaled so that the RMS of the forward-difference
  * velocity (src/timestep.h:77-78) equals ic_urms, and q is formed as the exact

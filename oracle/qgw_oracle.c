@@ -231,6 +231,8 @@ struct orc {
   double Delta, Lx, Ly, norm;
   double *X, *Y, *K, *L;
   double *psi, *q, *diff, *lapn, *topo, *f0, *f1, *f2, *wrk1;
+  int periodic; /* bc_fac == -1 */
+  cplx *cw;     /* periodic: the full 2-D spectrum, Ny x Nx */
   double *cl2m, *cm2l, *iRd2;
   double nu_hyper;
   double t, dt, dt0, dt1, DT_max, t_out;
@@ -241,10 +243,11 @@ struct orc {
 #define IDX(s, i, j, k) ((long)(k) * (s)->Nxp2 * (s)->Nyp2 + (long)(j) * (s)->Nxp2 + (i))
 
 void orc_check_timestep(const orc_params *p, double *dt_out, double *DT_max_out) {
-  /* init_domain: Delta from the cell count, Ly from NY; Nx = NX-1 (bounded) */
+  /* init_domain: Delta from the cell count, Ly from NY; Nx = NX-1 (bounded)
+   * or NX (periodic), src/domain.h:71-79 */
   double Delta = p->Lx / p->NX;
   double Ly = p->NY * Delta;
-  int Nx = p->NX - 1;
+  int Nx = (p->bc_fac == -1) ? p->NX : p->NX - 1;
   double nu_hyper = pow(-1, p->n_hyper / 2 - 1) * fabs(p->nu_hyper); /* src/domain.h:210 */
   double inv_t_nu_hyper = nu_hyper * 20. / pow(p->Lx / Nx, p->n_hyper);
   double inv_t_nu = p->nu * 20. / pow(p->Lx / Nx, 2);
@@ -344,18 +347,21 @@ static int compute_modes(orc *s) {
 
 orc *orc_create(const orc_params *p, int *status) {
   *status = 0;
-  if (p->bc_fac < 0 || p->NX < 3 || p->NY < 3 || p->nl < 1 || p->nl > ORC_NL_MAX) {
+  const int per = (p->bc_fac == -1);
+  if ((!per && p->bc_fac < 0) || p->NX < 3 || p->NY < 3 || p->nl < 1 || p->nl > ORC_NL_MAX ||
+      (per && (p->NX != p->NY || p->NX % 2))) {
     *status = -1;
     return NULL;
   }
   orc *s = (orc *)calloc(1, sizeof(orc));
   s->p = *p;
-  /* init_domain */
+  s->periodic = per;
+  /* init_domain (src/domain.h:64-176): bounded runs drop the wall point */
   s->Delta = p->Lx / p->NX;
   s->Lx = p->Lx;
   s->Ly = p->NY * s->Delta;
-  s->Nx = p->NX - 1;
-  s->Ny = p->NY - 1;
+  s->Nx = per ? p->NX : p->NX - 1;
+  s->Ny = per ? p->NY : p->NY - 1;
   s->Nxp1 = s->Nx + 1;
   s->Nyp1 = s->Ny + 1;
   s->Nxp2 = s->Nx + 2;
@@ -364,15 +370,28 @@ orc *orc_create(const orc_params *p, int *status) {
   s->Y = (double *)calloc(s->Nyp2, sizeof(double));
   s->L = (double *)calloc(s->Nxp2, sizeof(double));
   s->K = (double *)calloc(s->Nyp2, sizeof(double));
-  for (int i = 0; i < s->Nxp2; i++) s->X[i] = i * s->Delta;
-  for (int j = 0; j < s->Nyp2; j++) s->Y[j] = j * s->Delta;
-  for (int j = 1; j < s->Nyp1; j++) s->K[j] = PI_REF * j / p->Lx;
-  for (int i = 1; i < s->Nxp1; i++) s->L[i] = PI_REF * i / p->Lx;
-  /* init_elliptic (bounded) */
-  s->norm = 4.0 * s->Nxp1 * s->Nyp1;
+  const int I0 = per ? -1 : 0; /* periodic: the zero of X and Y at the second point */
+  for (int i = 0; i < s->Nxp2; i++) s->X[i] = (i + I0) * s->Delta;
+  for (int j = 0; j < s->Nyp2; j++) s->Y[j] = (j + I0) * s->Delta;
+  if (per) {
+    for (int j = 0; j < s->Nx; j++) s->L[j] = 2 * PI_REF / p->Lx * (((j + s->Nx / 2) % s->Nx) - s->Nx / 2);
+    for (int i = 0; i < s->Nx / 2 + 1; i++) s->K[i] = 2 * PI_REF * i / p->Lx;
+  } else {
+    for (int j = 1; j < s->Nyp1; j++) s->K[j] = PI_REF * j / p->Lx;
+    for (int i = 1; i < s->Nxp1; i++) s->L[i] = PI_REF * i / p->Lx;
+  }
+  /* init_elliptic (src/elliptic.h:29-71) */
+  if (per) {
+    s->norm = (double)p->NX * p->NY;
+    s->cw = (cplx *)calloc((size_t)s->Nx * s->Ny, sizeof(cplx));
+    plan_init(&s->px, s->Nx);
+    plan_init(&s->py, s->Ny);
+  } else {
+    s->norm = 4.0 * s->Nxp1 * s->Nyp1;
+    plan_init(&s->px, 2 * (s->Nx + 1));
+    plan_init(&s->py, 2 * (s->Ny + 1));
+  }
   s->wrk1 = (double *)calloc((size_t)s->Nx * s->Ny, sizeof(double));
-  plan_init(&s->px, 2 * (s->Nx + 1));
-  plan_init(&s->py, 2 * (s->Ny + 1));
   s->cl2m = (double *)calloc((size_t)p->nl * p->nl, sizeof(double));
   s->cm2l = (double *)calloc((size_t)p->nl * p->nl, sizeof(double));
   s->iRd2 = (double *)calloc(p->nl, sizeof(double));
@@ -411,7 +430,7 @@ void orc_destroy(orc *s) {
   if (!s) return;
   free(s->X); free(s->Y); free(s->K); free(s->L);
   free(s->psi); free(s->q); free(s->diff); free(s->lapn); free(s->topo);
-  free(s->f0); free(s->f1); free(s->f2); free(s->wrk1);
+  free(s->f0); free(s->f1); free(s->f2); free(s->wrk1); free(s->cw);
   free(s->cl2m); free(s->cm2l); free(s->iRd2);
   free(s->px.W); free(s->py.W);
   free(s);
@@ -454,9 +473,18 @@ void orc_sizes(orc *s, int *n, int *m, int *nl) {
 
 double orc_nu_hyper(orc *s) { return s->nu_hyper; }
 
-/* adjust_bc, bounded walls only (src/domain.h:308-348): S, N, W, E in that
- * order, so W/E own the corners. */
+/* adjust_bc (src/domain.h:221-349): periodic copies, or the bounded wall
+ * values; S, N, W, E in that order, so W/E own the corners. */
 void orc_adjust_bc(orc *s, double *field, int field_type) {
+  if (s->periodic) {
+    for (int k = 0; k < s->p.nl; k++) {
+      for (int i = 0; i < s->Nxp2; i++) field[IDX(s, i, 0, k)] = field[IDX(s, i, s->Ny, k)];
+      for (int i = 0; i < s->Nxp2; i++) field[IDX(s, i, s->Nyp1, k)] = field[IDX(s, i, 1, k)];
+      for (int j = 0; j < s->Nyp2; j++) field[IDX(s, 0, j, k)] = field[IDX(s, s->Nx, j, k)];
+      for (int j = 0; j < s->Nyp2; j++) field[IDX(s, s->Nxp1, j, k)] = field[IDX(s, 1, j, k)];
+    }
+    return;
+  }
   double psi_bc = 0.;
   double *psi = s->psi;
   double c = 2 * s->p.bc_fac / (s->Delta * s->Delta);
@@ -480,7 +508,75 @@ void orc_adjust_bc(orc *s, double *field, int field_type) {
   }
 }
 
+/* 2-D complex DFT of an Ny x Nx array (rows of Nx), forward or inverse
+ * (unnormalised; inverse by conjugation). */
+static void fft2_plans(cplx *a, int Nx, int Ny, const fftplan *px, const fftplan *py, int inverse) {
+#pragma omp parallel
+  {
+    int Lm = Nx > Ny ? Nx : Ny;
+    cplx *b1 = (cplx *)malloc(sizeof(cplx) * Lm), *b2 = (cplx *)malloc(sizeof(cplx) * Lm);
+#pragma omp for schedule(static)
+    for (int j = 0; j < Ny; j++) {
+      for (int i = 0; i < Nx; i++) b1[i] = inverse ? conj(a[(long)j * Nx + i]) : a[(long)j * Nx + i];
+      fft_rec(px, b2, b1, Nx, 1, 1);
+      for (int i = 0; i < Nx; i++) a[(long)j * Nx + i] = inverse ? conj(b2[i]) : b2[i];
+    }
+#pragma omp for schedule(static)
+    for (int i = 0; i < Nx; i++) {
+      for (int j = 0; j < Ny; j++) b1[j] = inverse ? conj(a[(long)j * Nx + i]) : a[(long)j * Nx + i];
+      fft_rec(py, b2, b1, Ny, 1, 1);
+      for (int j = 0; j < Ny; j++) a[(long)j * Nx + i] = inverse ? conj(b2[j]) : b2[j];
+    }
+    free(b1);
+    free(b2);
+  }
+}
+
+/* invert_pv, periodic branch (src/elliptic.h:76-155 with bc_fac = -1): the
+ * reference's r2c/c2r pair is restated on the full complex spectrum, which is
+ * Hermitian (the divisor is real and even in both wavenumbers), so its real
+ * part is the c2r result.  Division where the divisor is non-zero; the (0,0)
+ * coefficient is zeroed for every mode; normalisation by NX*NY. */
+static void invert_pv_periodic(orc *s) {
+  int nl = s->p.nl, Nx = s->Nx, Ny = s->Ny;
+  double *psi = s->psi, *q = s->q;
+  cplx *cw = s->cw;
+  size_t S = (size_t)s->Nxp2 * s->Nyp2 * nl;
+  for (size_t t = 0; t < S; t++) psi[t] = 0.;
+  for (int k = 0; k < nl; k++) {
+#pragma omp parallel for schedule(static)
+    for (int j = 1; j < s->Nyp1; j++)
+      for (int i = 1; i < s->Nxp1; i++) {
+        double acc = 0;
+        for (int l = 0; l < nl; l++) acc += s->cl2m[k * nl + l] * q[IDX(s, i, j, l)];
+        cw[(long)(j - 1) * Nx + (i - 1)] = acc;
+      }
+    fft2_plans(cw, Nx, Ny, &s->px, &s->py, 0);
+#pragma omp parallel for schedule(static)
+    for (int j = 0; j < Ny; j++)
+      for (int i = 0; i < Nx; i++) {
+        int ih = i <= Nx / 2 ? i : Nx - i; /* K of the Hermitian partner */
+        double fact = -(s->L[j] * s->L[j] + s->K[ih] * s->K[ih] + s->iRd2[k]);
+        if (fact != 0) cw[(long)j * Nx + i] = cw[(long)j * Nx + i] / fact;
+      }
+    cw[0] = 0;
+    fft2_plans(cw, Nx, Ny, &s->px, &s->py, 1);
+#pragma omp parallel for schedule(static)
+    for (int j = 1; j < s->Nyp1; j++)
+      for (int i = 1; i < s->Nxp1; i++) {
+        double w = creal(cw[(long)(j - 1) * Nx + (i - 1)]) / s->norm;
+        for (int l = 0; l < nl; l++) psi[IDX(s, i, j, l)] += s->cm2l[l * nl + k] * w;
+      }
+  }
+  orc_adjust_bc(s, psi, 0);
+  orc_adjust_bc(s, q, 1);
+}
+
 void orc_invert_pv(orc *s) {
+  if (s->periodic) {
+    invert_pv_periodic(s);
+    return;
+  }
   int nl = s->p.nl, Nx = s->Nx, Ny = s->Ny;
   double *psi = s->psi, *q = s->q, *wrk1 = s->wrk1;
   size_t S = (size_t)s->Nxp2 * s->Nyp2 * nl;

@@ -209,13 +209,15 @@ def read_params(path, p=None):
 
 
 def grid(p):
+    """Grid geometry and the X, Y, K, L arrays of src/domain.h (storage extents
+    n+2 and m+2: NX+1 bounded, NX+2 periodic)."""
     g = Grid()
-    n1, m1 = p.NX + 1, p.NY + 1
-    X, Y, K, L = np.zeros(n1), np.zeros(m1), np.zeros(m1), np.zeros(n1)
+    n2, m2 = p.NX + 2, p.NY + 2  # room for either layout
+    X, Y, K, L = np.zeros(n2), np.zeros(m2), np.zeros(m2), np.zeros(n2)
     st = host_lib().qgwh_grid(ctypes.byref(p), ctypes.byref(g), _ptr(X), _ptr(Y), _ptr(K), _ptr(L))
     if st != QGW_OK:
         raise QGWError("grid: unsupported parameters")
-    return g, X, Y, K, L
+    return g, X[:g.nxp2], Y[:g.nyp2], K[:g.nyp2], L[:g.nxp2]
 
 
 def eigmode(p):

@@ -151,8 +151,38 @@ int qgwh_read_params(const char *path, qgw_params *p) {
 
 int qgwh_grid(const qgw_params *p, qgw_grid *g, double *X, double *Y, double *Kw, double *Lw) {
   if (!p || !g) return QGW_ERR_ARG;
-  if (p->bc_fac < 0.0) return QGW_ERR_ARG; /* periodic branch not provided */
   if (p->NX < 3 || p->NY < 3 || !(p->Lx > 0.0)) return QGW_ERR_ARG;
+  if (p->bc_fac == -1.0) {
+    /* periodic (src/domain.h:71-75,144-147,163-167): NX x NY points, the zero
+     * of X and Y at the second grid point, Fourier wavenumbers.  The serial
+     * reference transforms an in-place r2c array whose row length is NX + 2
+     * with dimensions (NX, NY) (src/elliptic.h:32-47), which is consistent
+     * only for a square grid, and its half-spectrum wavenumbers need an even
+     * NX. */
+    if (p->NX != p->NY || (p->NX % 2) != 0) return QGW_ERR_ARG;
+    const int N = p->NX;
+    g->Delta = p->Lx / N;
+    g->Lx = p->Lx;
+    g->Ly = p->NY * g->Delta;
+    g->n = N;
+    g->m = p->NY;
+    g->nxp2 = g->n + 2;
+    g->nyp2 = g->m + 2;
+    if (X)
+      for (int i = 0; i < g->nxp2; i++) X[i] = (i - 1) * g->Delta;
+    if (Y)
+      for (int j = 0; j < g->nyp2; j++) Y[j] = (j - 1) * g->Delta;
+    if (Kw) { /* K[i], i < N/2 + 1: the half-spectrum (x) wavenumbers */
+      for (int j = 0; j < g->nyp2; j++) Kw[j] = 0.0;
+      for (int i = 0; i < N / 2 + 1; i++) Kw[i] = 2.0 * QGW_PI * i / p->Lx;
+    }
+    if (Lw) { /* L[j], j < N: the full (y) dimension, signed */
+      for (int i = 0; i < g->nxp2; i++) Lw[i] = 0.0;
+      for (int j = 0; j < N; j++) Lw[j] = 2.0 * QGW_PI / p->Lx * (((j + N / 2) % N) - N / 2);
+    }
+    return QGW_OK;
+  }
+  if (p->bc_fac < 0.0) return QGW_ERR_ARG; /* bc_fac is -1 (periodic) or in [0, 1] */
   g->Delta = p->Lx / p->NX;
   g->Lx = p->Lx;
   g->Ly = p->NY * g->Delta;
@@ -181,10 +211,11 @@ int qgwh_topo(const qgw_params *p, double *topo) {
   qgw_grid g;
   int st = qgwh_grid(p, &g, NULL, NULL, NULL, NULL);
   if (st) return st;
+  const double off = (p->bc_fac == -1.0) ? -1.0 : 0.0; /* X[i] = (i + I0) Delta */
   for (int j = 0; j < g.nyp2; j++) {
-    double y = j * g.Delta;
+    double y = (j + off) * g.Delta;
     for (int i = 0; i < g.nxp2; i++) {
-      double x = i * g.Delta;
+      double x = (i + off) * g.Delta;
       double s = shelf(x, p->w_topo) * shelf(p->Lx - x, p->w_topo) * shelf(y, p->w_topo) *
                  shelf(g.Ly - y, p->w_topo);
       topo[(size_t)j * g.nxp2 + i] = p->h_topo * (1.0 - s);
@@ -197,8 +228,9 @@ int qgwh_wind_row(const qgw_params *p, double *forc_row) {
   qgw_grid g;
   int st = qgwh_grid(p, &g, NULL, NULL, NULL, NULL);
   if (st) return st;
+  const double off = (p->bc_fac == -1.0) ? -1.0 : 0.0; /* Y[j] = (j + J0) Delta */
   for (int j = 0; j < g.nyp2; j++) {
-    double y = j * g.Delta;
+    double y = (j + off) * g.Delta;
     forc_row[j] = -p->tau0 / p->dh[0] * p->forc_mode * QGW_PI / g.Ly *
                   sin(p->forc_mode * QGW_PI * y / g.Ly);
   }
@@ -217,7 +249,7 @@ void qgwh_check_timestep(const qgw_params *p, double nu_hyper_signed, double *dt
     *DT_max_out = 0.0;
     return;
   }
-  /* Lx/Nx uses the interior count (Nx = NX-1 after init_domain). */
+  /* Lx/Nx uses the interior count (Nx = NX-1 bounded, NX periodic, after init_domain). */
   double h = p->Lx / g.n;
   double a = nu_hyper_signed * 20. / pow(h, p->n_hyper);
   double b = p->nu * 20. / pow(h, 2);
@@ -581,11 +613,147 @@ static uint64_t splitmix64(uint64_t *s) {
 /* uniform in (0, 1] */
 static double unif(uint64_t *s) { return ((double)(splitmix64(s) >> 11) + 1.0) * 0x1.0p-53; }
 
+/* In-place 2-D complex FFT of an m x n array (rows of n), forward or inverse
+ * (unnormalised), rows and columns in parallel. */
+static int fft2_complex(cplx *a, int n, int m, int inverse) {
+  hplan px, py;
+  int st = hplan_make(&px, n);
+  if (st) return st;
+  st = hplan_make(&py, m);
+  if (st) {
+    free(px.tw);
+    return st;
+  }
+  int err = 0;
+#pragma omp parallel
+  {
+    int L = n > m ? n : m;
+    cplx *w1 = (cplx *)malloc(sizeof(cplx) * (size_t)L);
+    cplx *w2 = (cplx *)malloc(sizeof(cplx) * (size_t)L);
+    if (!w1 || !w2) {
+#pragma omp atomic write
+      err = 1;
+    } else {
+#pragma omp for schedule(static)
+      for (int j = 0; j < m; j++) {
+        cplx *r = a + (size_t)j * n;
+        for (int i = 0; i < n; i++) w1[i] = inverse ? conj(r[i]) : r[i];
+        cplx *o = hfft(&px, w1, w2);
+        for (int i = 0; i < n; i++) r[i] = inverse ? conj(o[i]) : o[i];
+      }
+#pragma omp for schedule(static)
+      for (int i = 0; i < n; i++) {
+        for (int j = 0; j < m; j++) w1[j] = inverse ? conj(a[(size_t)j * n + i]) : a[(size_t)j * n + i];
+        cplx *o = hfft(&py, w1, w2);
+        for (int j = 0; j < m; j++) a[(size_t)j * n + i] = inverse ? conj(o[j]) : o[j];
+      }
+    }
+    free(w1);
+    free(w2);
+  }
+  free(px.tw);
+  free(py.tw);
+  return err ? QGW_ERR_NOMEM : QGW_OK;
+}
+
+/* Periodic form of the seeded initial condition: the same k^-3 spectrum on
+ * the Fourier wavenumbers (2 pi k / L), psi the real part of the inverse 2-D
+ * transform of seeded complex Gaussian coefficients (zero mean), q the modal
+ * Helmholtz operator applied in Fourier space, so that the periodic inversion
+ * (src/elliptic.h:103-118) reproduces psi.  The rings are periodic copies
+ * (src/domain.h:225-256). */
+static int make_ic_periodic(const qgw_params *p, const qgw_grid *g, const double *cl2m, const double *cm2l,
+                            const double *iRd2, uint64_t seed, double *psi, double *q) {
+  const int n = g->n, m = g->m, nl = p->nl;
+  const size_t NI = (size_t)n * m, S = (size_t)g->nxp2 * g->nyp2;
+  cplx *C = (cplx *)malloc(sizeof(cplx) * NI * nl);
+  cplx *Q = (cplx *)malloc(sizeof(cplx) * NI * nl);
+  double *Xm = (double *)malloc(sizeof(double) * 2 * nl);
+  if (!C || !Q || !Xm) {
+    free(C);
+    free(Q);
+    free(Xm);
+    return QGW_ERR_NOMEM;
+  }
+  const double kappa0 = 2.0 * QGW_PI / p->ic_lambda0, k06 = pow(kappa0, 6);
+  uint64_t s = seed;
+  for (int l = 0; l < nl; l++)
+    for (int b = 0; b < m; b++)
+      for (int a = 0; a < n; a++) {
+        double u1 = unif(&s), u2 = unif(&s);
+        double r = sqrt(-2.0 * log(u1));
+        double ka = 2.0 * QGW_PI / p->Lx * (((a + n / 2) % n) - n / 2);
+        double kb = 2.0 * QGW_PI / p->Lx * (((b + m / 2) % m) - m / 2);
+        double kap = sqrt(ka * ka + kb * kb);
+        double amp = (a == 0 && b == 0) ? 0.0 : 1.0 / sqrt(k06 + pow(kap, 6));
+        C[(size_t)l * NI + (size_t)b * n + a] = amp * r * (cos(2.0 * QGW_PI * u2) + I * sin(2.0 * QGW_PI * u2));
+      }
+  int st = QGW_OK;
+  for (int l = 0; l < nl && !st; l++) st = fft2_complex(C + (size_t)l * NI, n, m, 1);
+  if (st) goto out;
+  for (size_t t = 0; t < NI * nl; t++) C[t] = creal(C[t]); /* the real field psi */
+  /* RMS of the forward-difference velocity, periodic */
+  double sum = 0.0;
+  for (int l = 0; l < nl; l++)
+    for (int j = 0; j < m; j++)
+      for (int i = 0; i < n; i++) {
+        const cplx *f = C + (size_t)l * NI;
+        double c = creal(f[(size_t)j * n + i]);
+        double e = creal(f[(size_t)j * n + (i + 1) % n]);
+        double nn = creal(f[(size_t)((j + 1) % m) * n + i]);
+        double u = -(nn - c) / g->Delta, v = (e - c) / g->Delta;
+        sum += u * u + v * v;
+      }
+  double rms = sqrt(sum / ((double)NI * nl));
+  double scale = (rms > 0.0 && p->ic_urms > 0.0) ? p->ic_urms / rms : 1.0;
+  for (size_t t = 0; t < NI * nl; t++) {
+    C[t] *= scale;
+    Q[t] = C[t];
+  }
+  for (int l = 0; l < nl && !st; l++) st = fft2_complex(Q + (size_t)l * NI, n, m, 0);
+  if (st) goto out;
+  for (int b = 0; b < m; b++)
+    for (int a = 0; a < n; a++) {
+      size_t o = (size_t)b * n + a;
+      double ka = 2.0 * QGW_PI / p->Lx * (((a + n / 2) % n) - n / 2);
+      double kb = 2.0 * QGW_PI / p->Lx * (((b + m / 2) % m) - m / 2);
+      for (int k = 0; k < nl; k++) {
+        cplx acc = 0.0;
+        for (int l = 0; l < nl; l++) acc += cl2m[k * nl + l] * Q[(size_t)l * NI + o];
+        acc *= -(ka * ka + kb * kb + iRd2[k]);
+        Xm[2 * k] = creal(acc);
+        Xm[2 * k + 1] = cimag(acc);
+      }
+      for (int l = 0; l < nl; l++) {
+        cplx acc = 0.0;
+        for (int k = 0; k < nl; k++) acc += cm2l[l * nl + k] * (Xm[2 * k] + I * Xm[2 * k + 1]);
+        Q[(size_t)l * NI + o] = acc;
+      }
+    }
+  for (int l = 0; l < nl && !st; l++) st = fft2_complex(Q + (size_t)l * NI, n, m, 1);
+  if (st) goto out;
+  for (int l = 0; l < nl; l++)
+    for (int j = 0; j < g->nyp2; j++)
+      for (int i = 0; i < g->nxp2; i++) {
+        const int jj = (j == 0) ? m - 1 : (j == m + 1) ? 0 : j - 1; /* periodic copies on the ring */
+        const int ii = (i == 0) ? n - 1 : (i == n + 1) ? 0 : i - 1;
+        const size_t o = (size_t)l * NI + (size_t)jj * n + ii, d = (size_t)l * S + (size_t)j * g->nxp2 + i;
+        psi[d] = creal(C[o]);
+        q[d] = creal(Q[o]) / (double)NI;
+      }
+out:
+  free(C);
+  free(Q);
+  free(Xm);
+  return st;
+}
+
 int qgwh_make_ic(const qgw_params *p, const double *cl2m, const double *cm2l, const double *iRd2,
                  uint64_t seed, double *psi, double *q) {
   qgw_grid g;
   int st = qgwh_grid(p, &g, NULL, NULL, NULL, NULL);
   if (st) return st;
+  if (p->bc_fac == -1.0) return make_ic_periodic(p, &g, cl2m, cm2l, iRd2, seed, psi, q);
   int n = g.n, m = g.m, nl = p->nl;
   size_t NI = (size_t)n * m;
   size_t S = (size_t)g.nxp2 * g.nyp2;

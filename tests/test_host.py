@@ -92,8 +92,20 @@ def test_grid_and_wavenumbers():
     assert g.Delta == 1000e3 / 64 and g.Ly == 48 * g.Delta
     assert L[1] == np.pi / 1000e3 and L[63] == np.pi * 63 / 1000e3
     assert K[47] == np.pi * 47 / 1000e3  # K uses Lx (src/domain.h:170-171)
+
+    # periodic (src/domain.h:71-75,144-147,163-167): NX points, zero of X at
+    # the second grid point, K[i] = 2 pi i/Lx (i <= NX/2), L[j] signed
+    g, X, Y, K, L = Q.grid(Q.default_params(NX=64, NY=64, Lx=2.0, bc_fac=-1.0))
+    assert (g.n, g.m, g.nxp2, g.nyp2) == (64, 64, 66, 66)
+    assert X[0] == -2.0 / 64 and X[1] == 0.0 and Y[65] == 64 * 2.0 / 64
+    assert K[0] == 0.0 and K[32] == 2 * np.pi * 32 / 2.0
+    assert L[1] == 2 * np.pi / 2.0 and L[31] == 2 * np.pi * 31 / 2.0
+    assert L[32] == -2 * np.pi * 32 / 2.0 and L[63] == -2 * np.pi / 2.0
+    for bad in (dict(NX=64, NY=48), dict(NX=63, NY=63)):  # square, even NX only
+        with pytest.raises(Q.QGWError):
+            Q.grid(Q.default_params(Lx=1.0, bc_fac=-1.0, **bad))
     with pytest.raises(Q.QGWError):
-        Q.grid(Q.default_params(NX=64, NY=64, Lx=1.0, bc_fac=-1.0))
+        Q.grid(Q.default_params(NX=64, NY=64, Lx=1.0, bc_fac=-0.5))
 
 
 def test_topo_and_wind_match_oracle():
