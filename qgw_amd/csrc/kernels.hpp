@@ -472,6 +472,170 @@ k_p3h(Geo g, const T *__restrict__ what, T *__restrict__ psi, T *__restrict__ q,
 }
 
 // ===========================================================================
+// Periodic branch (bc_fac = -1, src/elliptic.h:32-47,103-118): the reference's
+// in-place r2c / c2r 2-D transform pair, as
+//   P1 (k_pf1): per (row, layer pair): FFT_N of a + i b, split into the two
+//       half spectra A_k = (Z_k + conj Z_{N-k})/2, B_k = (Z_k - conj Z_{N-k})/(2i),
+//       k = 0..N/2, stored interleaved (re, im) in what rows of N + 2 reals;
+//   T1: the transpose puts the (re, im) pair of each x wavenumber in two
+//       adjacent whatT rows;
+//   P2 (k_pf2): per (mode, x wavenumber): FFT_N along y of re + i im, division
+//       by -(L_j^2 + K_i^2 + 1/Rd_k^2) * N^2 where non-zero, the (0,0)
+//       coefficient zeroed, inverse FFT;
+//   P3 (k_pf3): per (row, layer pair): Z_p = A_p + i B_p (p <= N/2) and
+//       conj A_{N-p} + i conj B_{N-p} (p > N/2), inverse FFT_N, a = Re, b = Im;
+//       psi row, and the periodic rings of psi and q (src/domain.h:225-256).
+// The imaginary parts of the k = 0 and k = N/2 coefficients are ignored in
+// P3, as a c2r transform does.  Twiddles: length N.
+// ===========================================================================
+template <typename T, int L>
+__global__ void __launch_bounds__(RegPlan<L>::NT)
+k_pf1(Geo g, const T *__restrict__ q, T *__restrict__ what, const cx<T> *__restrict__ tw) {
+  constexpr int E = RegPlan<L>::E, NT = RegPlan<L>::NT;
+  __shared__ cx<T> buf[DstHalf<L>::slots];
+  const int t = threadIdx.x;
+  const int npair = (g.nl + 1) / 2;
+  const int lp = blockIdx.x % npair;
+  const int j = blockIdx.x / npair + 1;
+  const int l0 = 2 * lp, l1 = l0 + 1;
+  const bool has1 = l1 < g.nl;
+  const T *r0 = q + l0 * g.LS + (long)j * g.P + 1;  // x = p + 1
+  const T *r1 = q + (has1 ? l1 : l0) * g.LS + (long)j * g.P + 1;
+  cx<T> h[E];
+#pragma unroll
+  for (int m = 0; m < E; m++) {
+    const int p = t + m * NT;
+    h[m] = mkc<T>(r0[p], has1 ? r1[p] : T(0));
+  }
+  fft_home<T, L>(h, buf, tw, t);
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < E; m++) buf[rslot(t + m * NT)] = h[m];
+  __syncthreads();
+  T *o0 = what + l0 * g.WS + (long)(j - 1) * g.Pn;
+  T *o1 = what + l1 * g.WS + (long)(j - 1) * g.Pn;
+  for (int k = t; k <= L / 2; k += NT) {
+    const cx<T> Z = buf[rslot(k)], Zm = buf[rslot(k == 0 ? 0 : L - k)];
+    o0[2 * k] = T(0.5) * (Z.x + Zm.x);
+    o0[2 * k + 1] = T(0.5) * (Z.y - Zm.y);
+    if (has1) {
+      o1[2 * k] = T(0.5) * (Z.y + Zm.y);
+      o1[2 * k + 1] = T(0.5) * (Zm.x - Z.x);
+    }
+  }
+}
+
+template <typename T, int L>
+__global__ void __launch_bounds__(RegPlan<L>::NT)
+k_pf2(Geo g, T *__restrict__ whatT, const double *__restrict__ Kx2, const double *__restrict__ Ly2,
+      const double *__restrict__ iRd2, double norm, const cx<T> *__restrict__ tw) {
+  constexpr int E = RegPlan<L>::E, NT = RegPlan<L>::NT;
+  __shared__ cx<T> buf[home_slots<L>()];
+  const int t = threadIdx.x;
+  const int nk = g.n / 2 + 1;  // x wavenumbers
+  const int k = blockIdx.x / nk, a = blockIdx.x % nk;
+  T *re = whatT + k * g.WTS + (long)(2 * a) * g.Pm;
+  T *im = re + g.Pm;
+  cx<T> h[E];
+#pragma unroll
+  for (int m = 0; m < E; m++) {
+    const int p = t + m * NT;
+    h[m] = mkc<T>(re[p], im[p]);
+  }
+  fft_home<T, L>(h, buf, tw, t);
+  const double kx2 = Kx2[a], ir = iRd2[k];
+#pragma unroll
+  for (int m = 0; m < E; m++) {
+    const int p = t + m * NT;
+    const double fact = -(Ly2[p] + kx2 + ir);
+    if (a == 0 && p == 0) {
+      h[m] = mkc<T>(T(0), T(0));
+    } else if (fact != 0.0) {
+      const T f = (T)(1.0 / (fact * norm));
+      h[m] = mkc<T>(h[m].x * f, h[m].y * f);
+    }
+    h[m] = mkc<T>(h[m].x, -h[m].y);  // inverse by conjugation
+  }
+  fft_home<T, L>(h, buf, tw, t);
+#pragma unroll
+  for (int m = 0; m < E; m++) {
+    const int p = t + m * NT;
+    re[p] = h[m].x;
+    im[p] = -h[m].y;
+  }
+}
+
+template <typename T, int L>
+__global__ void __launch_bounds__(RegPlan<L>::NT)
+k_pf3(Geo g, const T *__restrict__ what, T *__restrict__ psi, T *__restrict__ q, const cx<T> *__restrict__ tw) {
+  constexpr int E = RegPlan<L>::E, NT = RegPlan<L>::NT;
+  __shared__ cx<T> buf[home_slots<L>()];
+  const int t = threadIdx.x;
+  const int n = g.n, m_ = g.m;
+  const int npair = (g.nl + 1) / 2;
+  const int lp = blockIdx.x % npair;
+  const int j = blockIdx.x / npair + 1;
+  const int l0 = 2 * lp, l1 = l0 + 1;
+  const bool has1 = l1 < g.nl;
+  const T *i0 = what + l0 * g.WS + (long)(j - 1) * g.Pn;
+  const T *i1 = what + (has1 ? l1 : l0) * g.WS + (long)(j - 1) * g.Pn;
+  cx<T> h[E];
+#pragma unroll
+  for (int m = 0; m < E; m++) {
+    const int p = t + m * NT;
+    const int kk = (p <= L / 2) ? p : L - p;
+    const bool edge = (kk == 0 || kk == L / 2);
+    const T ar = i0[2 * kk], ai = edge ? T(0) : i0[2 * kk + 1];
+    const T br = has1 ? i1[2 * kk] : T(0), bi = (has1 && !edge) ? i1[2 * kk + 1] : T(0);
+    // Z = A + i B (p <= N/2) or conj A + i conj B (p > N/2); conjugated for the inverse
+    const T sa = (p <= L / 2) ? T(1) : T(-1);
+    const cx<T> z = mkc<T>(ar - sa * bi, sa * ai + br);
+    h[m] = mkc<T>(z.x, -z.y);
+  }
+  fft_home<T, L>(h, buf, tw, t);
+#pragma unroll
+  for (int side = 0; side < 2; side++) {
+    const int l = side ? l1 : l0;
+    if (side && !has1) break;
+    T *pr = psi + l * g.LS;
+    T *qr = q + l * g.LS;
+    const long rj = (long)j * g.P;
+#pragma unroll
+    for (int m = 0; m < E; m++) {
+      const int p = t + m * NT;
+      const T v = side ? -h[m].y : h[m].x;  // conj(FFT(conj Z)): a = Re, b = Im
+      const int x = p + 1;
+      pr[rj + x] = v;
+      if (x == n) pr[rj] = v;
+      if (x == 1) pr[rj + n + 1] = v;
+      if (j == m_) {  // row 0 is the copy of row N
+        pr[x] = v;
+        if (x == n) pr[0] = v;
+        if (x == 1) pr[n + 1] = v;
+      }
+      if (j == 1) {  // row N+1 is the copy of row 1
+        const long rt = (long)(m_ + 1) * g.P;
+        pr[rt + x] = v;
+        if (x == n) pr[rt] = v;
+        if (x == 1) pr[rt + n + 1] = v;
+      }
+    }
+    // q: periodic copies of its current interior
+    if (t == 0) {
+      qr[rj] = qr[rj + n];
+      qr[rj + n + 1] = qr[rj + 1];
+    }
+    if (j == m_ || j == 1) {
+      const long rs = rj, rt = (j == m_) ? 0 : (long)(m_ + 1) * g.P;
+      for (int x = t; x < n + 2; x += NT) {
+        const int xs = (x == 0) ? n : (x == n + 1) ? 1 : x;
+        qr[rt + x] = qr[rs + xs];
+      }
+    }
+  }
+}
+
+// ===========================================================================
 // P3: inverse transform along x of two layers of one grid row, the psi row
 // with its zero wall values, and the wall ring of q (src/domain.h:308-348:
 // q ring = 2 bc_fac/Delta^2 * adjacent psi; corners are zero because psi's
@@ -620,7 +784,15 @@ __device__ __forceinline__ void p4_level(const T *src, T *dst, const T *sP, int 
   });
 }
 
-template <typename T, int NLAP, int KAHAN, int TX, int TY, bool FAST>
+// PER: periodic domain (bc_fac = -1): the psi and q tiles are loaded with
+// wrapped indices (the stored rings are periodic copies, src/domain.h:225-256,
+// and so is every Laplacian level), the levels use the plain stencil
+// everywhere, and the whole q ring (corners included) is carried.
+// Index v of any sign mapped to 1..N (a tile with its halo may span more
+// than one period on small grids).
+__device__ __forceinline__ int wrap1(int v, int N) { return ((v - 1) % N + N) % N + 1; }
+
+template <typename T, int NLAP, int KAHAN, int TX, int TY, bool FAST, bool PER = false>
 __device__ __forceinline__ void p4_body(P4Smem<T, TX, TY, NLAP> &sm, const Geo &g, const T *__restrict__ P,
                                         const T *__restrict__ Q, const T *__restrict__ topo, int k, int x0, int y0,
                                         int tx, int ty, const T (&hf1)[TY / 4], const T (&hf2)[TY / 4],
@@ -634,15 +806,21 @@ __device__ __forceinline__ void p4_body(P4Smem<T, TX, TY, NLAP> &sm, const Geo &
   // ---- psi (halo H) and q (halo 1) tiles; zero outside the storage grid
   p4_region<TX, H, PH>(tx, ty, [&](int ly, int lx) {
     const int gx = x0 - H + lx, gy = y0 - H + ly;
-    if constexpr (FAST)
+    if constexpr (PER)
+      sm.P[ly][lx] = P[wrap1(gy, m) * g.P + wrap1(gx, n)];
+    else if constexpr (FAST)
       sm.P[ly][lx] = P[gy * g.P + gx];
     else
       sm.P[ly][lx] = (gx >= 0 && gx <= n + 1 && gy >= 0 && gy <= m + 1) ? P[gy * g.P + gx] : T(0);
   });
   p4_region<TX, 1, TY + 2>(tx, ty, [&](int ly, int lx) {
     const int gx = x0 - 1 + lx, gy = y0 - 1 + ly;
-    const bool in = FAST || (gx <= n + 1 && gy <= m + 1);
-    sm.Q[ly][lx] = in ? Q[gy * g.P + gx] : T(0);
+    if constexpr (PER) {
+      sm.Q[ly][lx] = Q[wrap1(gy, m) * g.P + wrap1(gx, n)];
+    } else {
+      const bool in = FAST || (gx <= n + 1 && gy <= m + 1);
+      sm.Q[ly][lx] = in ? Q[gy * g.P + gx] : T(0);
+    }
   });
   __syncthreads();
   const T inv_d2 = (T)cf.inv_d2;
@@ -650,21 +828,21 @@ __device__ __forceinline__ void p4_body(P4Smem<T, TX, TY, NLAP> &sm, const Geo &
   const T *sPp = &sm.P[0][0];
   // ---- nested Laplacians of psi down to radius 1 (calc_diff, src/dynamics.h:25-43)
   if constexpr (NLAP >= 2) {
-    p4_level<T, TX, TY, H, FAST, H - 1, PW, SM::AW>(sPp, &sm.A[0][0], sPp, x0, y0, n, m, inv_d2, ring_c, tx, ty);
+    p4_level<T, TX, TY, H, FAST || PER, H - 1, PW, SM::AW>(sPp, &sm.A[0][0], sPp, x0, y0, n, m, inv_d2, ring_c, tx, ty);
     __syncthreads();
   }
   if constexpr (NLAP >= 3) {
-    p4_level<T, TX, TY, H, FAST, H - 2, SM::AW, SM::BW>(&sm.A[0][0], &sm.B[0][0], sPp, x0, y0, n, m, inv_d2,
+    p4_level<T, TX, TY, H, FAST || PER, H - 2, SM::AW, SM::BW>(&sm.A[0][0], &sm.B[0][0], sPp, x0, y0, n, m, inv_d2,
                                                          ring_c, tx, ty);
     __syncthreads();
   }
   if constexpr (NLAP >= 4) {
-    p4_level<T, TX, TY, H, FAST, H - 3, SM::BW, SM::AW>(&sm.B[0][0], &sm.A[0][0], sPp, x0, y0, n, m, inv_d2,
+    p4_level<T, TX, TY, H, FAST || PER, H - 3, SM::BW, SM::AW>(&sm.B[0][0], &sm.A[0][0], sPp, x0, y0, n, m, inv_d2,
                                                          ring_c, tx, ty);
     __syncthreads();
   }
   if constexpr (NLAP >= 5) {
-    p4_level<T, TX, TY, H, FAST, H - 4, SM::AW, SM::BW>(&sm.A[0][0], &sm.B[0][0], sPp, x0, y0, n, m, inv_d2,
+    p4_level<T, TX, TY, H, FAST || PER, H - 4, SM::AW, SM::BW>(&sm.A[0][0], &sm.B[0][0], sPp, x0, y0, n, m, inv_d2,
                                                          ring_c, tx, ty);
     __syncthreads();
   }
@@ -732,11 +910,17 @@ __device__ __forceinline__ void p4_body(P4Smem<T, TX, TY, NLAP> &sm, const Geo &
     } else {
       qn[o] = qo + inc;
     }
-    // carry q's wall ring into the next buffer (corners stay zero)
+    // carry q's wall ring into the next buffer (bounded: corners stay zero)
     if (gy == 1) qn[gx] = sm.Q[qy - 1][qx];
     if (gy == m) qn[(m + 1) * g.P + gx] = sm.Q[qy + 1][qx];
     if (gx == 1) qn[gy * g.P] = sm.Q[qy][qx - 1];
     if (gx == n) qn[gy * g.P + n + 1] = sm.Q[qy][qx + 1];
+    if constexpr (PER) {
+      if (gy == 1 && gx == 1) qn[0] = sm.Q[qy - 1][qx - 1];
+      if (gy == 1 && gx == n) qn[n + 1] = sm.Q[qy - 1][qx + 1];
+      if (gy == m && gx == 1) qn[(m + 1) * g.P] = sm.Q[qy + 1][qx - 1];
+      if (gy == m && gx == n) qn[(m + 1) * g.P + n + 1] = sm.Q[qy + 1][qx + 1];
+    }
   }
 }
 
@@ -751,7 +935,7 @@ __device__ __forceinline__ void p4_body(P4Smem<T, TX, TY, NLAP> &sm, const Geo &
 // term (nu_hyper = 0).  Tiles whose psi halo lies inside the storage grid and
 // whose Laplacian levels lie inside the interior take a branch-free path.
 // ===========================================================================
-template <typename T, int NLAP, int KAHAN, int TX, int TY>
+template <typename T, int NLAP, int KAHAN, int TX, int TY, bool PER = false>
 __global__ void __launch_bounds__(TX * 4)
 k_p4(Geo g, const T *__restrict__ psi, const T *__restrict__ q, T *__restrict__ qn, T *__restrict__ fnew,
      const T *__restrict__ f1, const T *__restrict__ f2, T *__restrict__ comp, const T *__restrict__ topo,
@@ -792,7 +976,10 @@ k_p4(Geo g, const T *__restrict__ psi, const T *__restrict__ q, T *__restrict__ 
   topo += mb * g.LS;
   wind += (long)mb * (m + 2);
   const bool fast = (x0 - H >= 0) && (x0 + TX - 1 + H <= n + 1) && (y0 - H >= 0) && (y0 + TY - 1 + H <= m + 1);
-  if (fast)
+  if constexpr (PER)
+    p4_body<T, NLAP, KAHAN, TX, TY, false, true>(sm, g, P, Q, topo, km, x0, y0, tx, ty, hf1, hf2, hcp, qn, fnew,
+                                                 comp, wind, cf);
+  else if (fast)
     p4_body<T, NLAP, KAHAN, TX, TY, true>(sm, g, P, Q, topo, km, x0, y0, tx, ty, hf1, hf2, hcp, qn, fnew, comp,
                                           wind, cf);
   else

@@ -33,6 +33,18 @@ using T = QGW_T;
 
 template <> int launch_p1_t<T>(const LaunchArgs &a) {
   const dim3 grid(a.g.m * ((a.g.nl + 1) / 2));
+  if (a.per) {
+    switch (a.Lx) {
+#define QGW_CASE(L)                                                                                  \
+  case L:                                                                                            \
+    hipLaunchKernelGGL((k_pf1<T, L>), grid, dim3(RegPlan<L>::NT), 0, a.stream, a.g, (const T *)a.q, \
+                       (T *)a.what, (const cx<T> *)a.twx);                                           \
+    return 0;
+      QGW_LENGTHS(QGW_CASE)
+#undef QGW_CASE
+    }
+    return QGW_ERR_ARG;
+  }
   if (a.dst_half) {
     switch (a.Lx / 2) {
 #define QGW_GO(M, EP)                                                                                  \
@@ -62,6 +74,18 @@ template <> int launch_p1_t<T>(const LaunchArgs &a) {
 
 template <> int launch_p3_t<T>(const LaunchArgs &a) {
   const dim3 grid(a.g.m * ((a.g.nl + 1) / 2));
+  if (a.per) {
+    switch (a.Lx) {
+#define QGW_CASE(L)                                                                                     \
+  case L:                                                                                               \
+    hipLaunchKernelGGL((k_pf3<T, L>), grid, dim3(RegPlan<L>::NT), 0, a.stream, a.g, (const T *)a.what, \
+                       (T *)a.psi, (T *)a.q, (const cx<T> *)a.twx);                                     \
+    return 0;
+      QGW_LENGTHS(QGW_CASE)
+#undef QGW_CASE
+    }
+    return QGW_ERR_ARG;
+  }
   if (a.dst_half) {
     switch (a.Lx / 2) {
 #define QGW_GO(M, EP)                                                                                    \
@@ -92,6 +116,19 @@ template <> int launch_p3_t<T>(const LaunchArgs &a) {
 template <> int launch_p2_t<T>(const LaunchArgs &a) {
   const long rows = (long)a.g.nl * a.g.n;
   const dim3 grid((unsigned)((rows + 1) / 2));
+  if (a.per) {
+    const dim3 gp((unsigned)((long)a.g.nl * (a.g.n / 2 + 1)));
+    switch (a.Ly) {
+#define QGW_CASE(L)                                                                                 \
+  case L:                                                                                           \
+    hipLaunchKernelGGL((k_pf2<T, L>), gp, dim3(RegPlan<L>::NT), 0, a.stream, a.g, (T *)a.whatT,    \
+                       a.La2, a.Kb2, a.iRd2, a.norm, (const cx<T> *)a.twy);                         \
+    return 0;
+      QGW_LENGTHS(QGW_CASE)
+#undef QGW_CASE
+    }
+    return QGW_ERR_ARG;
+  }
   if (a.dst_half) {
     switch (a.Ly / 2) {
 #define QGW_GO(M, EP)                                                                                \
@@ -158,19 +195,23 @@ static int tproj(const LaunchArgs &a, const void *src, void *dst, const void *M,
   return 0;
 }
 
+// Columns of a what row: n spectral values, or n + 2 reals (N/2 + 1 complex,
+// interleaved) in the periodic form.
+static int what_cols(const LaunchArgs &a) { return a.per ? a.g.n + 2 : a.g.n; }
+
 template <> int launch_t1_t<T>(const LaunchArgs &a) {
   // what (layers) [l][j][a] -> whatT (modes) [k][a][j], w_k = sum_l cl2m[k,l] w_l
-  return tproj(a, a.what, a.whatT, a.cl2m, a.g.m, a.g.n, a.g.Pn, a.g.WS, a.g.Pm, a.g.WTS);
+  return tproj(a, a.what, a.whatT, a.cl2m, a.g.m, what_cols(a), a.g.Pn, a.g.WS, a.g.Pm, a.g.WTS);
 }
 
 template <> int launch_t2_t<T>(const LaunchArgs &a) {
   // whatT (modes) [k][a][j] -> what (layers) [l][j][a], w_l = sum_k cm2l[l,k] w_k
-  return tproj(a, a.whatT, a.what, a.cm2l, a.g.n, a.g.m, a.g.Pm, a.g.WTS, a.g.Pn, a.g.WS);
+  return tproj(a, a.whatT, a.what, a.cm2l, what_cols(a), a.g.m, a.g.Pm, a.g.WTS, a.g.Pn, a.g.WS);
 }
 
-template <int NLAP, int KAHAN, int TX, int TY> static void p4_go(const LaunchArgs &a) {
+template <int NLAP, int KAHAN, int TX, int TY, bool PER = false> static void p4_go(const LaunchArgs &a) {
   const dim3 grid((a.g.n + TX - 1) / TX, (a.g.m + TY - 1) / TY, a.g.nl);
-  hipLaunchKernelGGL((k_p4<T, NLAP, KAHAN, TX, TY>), grid, dim3(TX * 4), 0, a.stream, a.g, (const T *)a.psi,
+  hipLaunchKernelGGL((k_p4<T, NLAP, KAHAN, TX, TY, PER>), grid, dim3(TX * 4), 0, a.stream, a.g, (const T *)a.psi,
                      (const T *)a.q, (T *)a.qn, (T *)a.fnew, (const T *)a.f1, (const T *)a.f2, (T *)a.comp,
                      (const T *)a.topo, (const T *)a.wind, a.cf);
 }
@@ -194,6 +235,10 @@ template <int NLAP, int KAHAN> static void p4s_go(const LaunchArgs &a) {
 
 template <int NLAP, int KAHAN> static int p4_tile(const LaunchArgs &a) {
   static_assert(kP4TileCount == 4, "tile table");
+  if (a.per) {  // periodic: the 64x16 tile with wrapped halo loads
+    p4_go<NLAP, KAHAN, kP4TileW[1], kP4TileH[1], true>(a);
+    return 0;
+  }
   switch (a.p4_tile) {
     case kP4TileCount: p4s_go<NLAP, KAHAN>(a); return 0;
     case 0: p4_go<NLAP, KAHAN, kP4TileW[0], kP4TileH[0]>(a); return 0;

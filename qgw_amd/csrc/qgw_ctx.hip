@@ -62,6 +62,7 @@ struct qgw_ctx {
   int p4_waves;  // streaming P4 wavefronts per SIMD (QGW_TUNE_P4_WAVES)
   int dst_half;  // half-length sine transforms (QGW_TUNE_DST); -1: lengths not provided
   int fft_e;     // elements per thread of the half-length FFTs (QGW_TUNE_FFT_E)
+  int per;       // periodic domain (bc_fac = -1)
   bool modes_set;
   double t, dt, dt0, dt1, DT_max, t_out;
   int it;
@@ -248,7 +249,9 @@ int run_invert(qgw_ctx *c) {
   a.twyh = c->twyh;
   a.dst_half = c->dst_half > 0;
   a.fft_e = c->fft_e;
-  a.norm = 4.0 * (double)(c->g.n + 1) * (double)(c->g.m + 1);
+  a.per = c->per;
+  // normalisation of the transform pair (src/elliptic.h:36,51)
+  a.norm = c->per ? (double)c->g.n * (double)c->g.m : 4.0 * (double)(c->g.n + 1) * (double)(c->g.m + 1);
   a.ring_c = 2.0 * c->p.bc_fac / (c->grid.Delta * c->grid.Delta);
   int st;
   {
@@ -312,6 +315,7 @@ int run_p4(qgw_ctx *c, const double *cab3, int update) {
   a.nlap = (c->nu_hyper != 0.0) ? (c->p.n_hyper / 2 + 1) : 0;
   a.p4_tile = c->p4_tile;
   a.p4_waves = c->p4_waves;
+  a.per = c->per;
   int st;
   {
     PassScope ps(c, QGW_PASS_P4);
@@ -421,8 +425,10 @@ int check_params(const qgw_params *p, qgw_grid *grid, int *Lx, int *Ly) {
   if (p->n_hyper < 0 || (p->n_hyper % 2) != 0 || p->n_hyper > 8) return QGW_ERR_ARG;
   if (p->k_f != 0 || p->dt_forc_period != 0) return QGW_ERR_ARG;  // stochastic / file forcing not provided
   if (qgwh_grid(p, grid, nullptr, nullptr, nullptr, nullptr) != QGW_OK) return QGW_ERR_ARG;
-  *Lx = 2 * (grid->n + 1);
-  *Ly = 2 * (grid->m + 1);
+  // transform lengths: the odd extension 2(n+1) (bounded) or the period N (periodic)
+  const bool per = (p->bc_fac == -1.0);
+  *Lx = per ? grid->n : 2 * (grid->n + 1);
+  *Ly = per ? grid->m : 2 * (grid->m + 1);
   if (!supported_length(*Lx) || !supported_length(*Ly)) return QGW_ERR_ARG;
   return QGW_OK;
 }
@@ -587,12 +593,15 @@ qgw_ctx *create_group(const qgw_params *ps, int nb, int precision, hipStream_t e
   g.m = m;
   g.nl = NL;
   g.nlm = nl;
+  const bool per = (p->bc_fac == -1.0);
+  // spectral rows: n values (bounded), or N/2 + 1 complex = n + 2 reals (periodic)
+  const int wc = per ? n + 2 : n;
   g.P = pitch_of(n + 2);
   g.LS = (long)(m + 2) * g.P;
-  g.Pn = pitch_of(n);
+  g.Pn = pitch_of(wc);
   g.WS = (long)m * g.Pn;
   g.Pm = pitch_of(m);
-  g.WTS = (long)n * g.Pm;
+  g.WTS = (long)wc * g.Pm;
   *status = QGW_ERR_HIP;
   if (ext) {
     c->stream = ext;
@@ -643,12 +652,19 @@ qgw_ctx *create_group(const qgw_params *ps, int nb, int precision, hipStream_t e
     release(c);
     return nullptr;
   }
-  // wavenumbers squared (src/domain.h:169-174; K uses Lx as the reference does)
+  // wavenumbers squared, La2 along x and Kb2 along y: bounded K[j] and L[i]
+  // (src/domain.h:169-174; K uses Lx as the reference does), periodic K[i]
+  // (x, half spectrum) and L[j] (y, signed) (src/domain.h:163-167)
   {
     std::vector<double> Lw(n + 2), Kw(m + 2), La2(n + 2, 0.0), Kb2(m + 2, 0.0);
     qgwh_grid(p, &grid, nullptr, nullptr, Kw.data(), Lw.data());
-    for (int i = 0; i < n + 2; i++) La2[i] = Lw[i] * Lw[i];
-    for (int j = 0; j < m + 2; j++) Kb2[j] = Kw[j] * Kw[j];
+    if (per) {
+      for (int i = 0; i < n / 2 + 1; i++) La2[i] = Kw[i] * Kw[i];
+      for (int j = 0; j < m; j++) Kb2[j] = Lw[j] * Lw[j];
+    } else {
+      for (int i = 0; i < n + 2; i++) La2[i] = Lw[i] * Lw[i];
+      for (int j = 0; j < m + 2; j++) Kb2[j] = Kw[j] * Kw[j];
+    }
     if (hipMemcpy(c->La2, La2.data(), sizeof(double) * (n + 2), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(c->Kb2, Kb2.data(), sizeof(double) * (m + 2), hipMemcpyHostToDevice) != hipSuccess) {
       release(c);
@@ -685,7 +701,8 @@ qgw_ctx *create_group(const qgw_params *ps, int nb, int precision, hipStream_t e
   c->p4_tile = qgw::kP4TileCount;  // the register-streaming form
   c->p4_waves = 32;
   c->fft_e = (precision == QGW_FP64) ? 16 : 8;
-  c->dst_half = (qgw::launch_has_half_length(Lx / 2) && qgw::launch_has_half_length(Ly / 2)) ? 1 : -1;
+  c->per = per ? 1 : 0;
+  c->dst_half = (!per && qgw::launch_has_half_length(Lx / 2) && qgw::launch_has_half_length(Ly / 2)) ? 1 : -1;
   c->modes_set = false;
   // the zero fills above ran on the null stream; the step runs on c->stream
   if (hipDeviceSynchronize() != hipSuccess) {
@@ -786,7 +803,7 @@ int autotune_group(qgw_ctx *c, int reps, int *p4_tile, double *best_ms) {
   double best = 1e300;
   int arg = c->p4_tile, warg = c->p4_waves;
   static const int kWaves[] = {4, 8, 16, 32};
-  for (int v = 0; v < qgw::kP4Variants && !st; v++) {
+  for (int v = 0; v < (c->per ? 0 : qgw::kP4Variants) && !st; v++) {  // periodic: one P4 kernel
     const bool stream = (v == qgw::kP4TileCount);
     for (int wi = 0; wi < (stream ? 4 : 1) && !st; wi++) {
       c->p4_tile = v;
@@ -837,6 +854,8 @@ int autotune_group(qgw_ctx *c, int reps, int *p4_tile, double *best_ms) {
     c->dst_half = darg;
     c->fft_e = earg;
   }
+  // the chosen configuration, timed once more (the figure returned)
+  if (!st) st = time_candidate(c, reps, &best);
   int st2 = snapshot_restore(c, snap);
   if (st) return st;
   if (st2) return st2;
