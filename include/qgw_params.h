@@ -53,7 +53,7 @@ typedef struct qgw_params {
   /* ---- extensions ---- */
   double dt_fixed;        /* > 0: fixed time step [s]; 0: the reference's adaptive step */
   int nsteps;             /* > 0: number of steps to run (driver); 0: run to tend */
-  int ic_kind;            /* 0: read nothing (start from rest), 1: seeded k^-3 random field */
+  int ic_kind;            /* 0: q and topo from restart.nc when present, else rest; 1: seeded k^-3 field */
   uint64_t seed;          /* RNG seed of the seeded initial condition */
   double ic_urms;         /* target RMS velocity of the initial condition [m/s] */
   double ic_lambda0;      /* peak wavelength of the initial spectrum [m] */

@@ -120,6 +120,13 @@ def host_lib():
         lib.qgwh_strerror.argtypes = [ctypes.c_int]
         lib.qgwh_strerror.restype = ctypes.c_char_p
         lib.qgwh_params_size.restype = ctypes.c_size_t
+        vp = ctypes.c_void_p
+        lib.qgwh_nc_create.argtypes = [ctypes.c_char_p, P, ctypes.POINTER(ctypes.c_int)]
+        lib.qgwh_nc_create.restype = vp
+        lib.qgwh_nc_write.argtypes = [vp, ctypes.c_double, _dp, _dp]
+        lib.qgwh_nc_records.argtypes = [vp]
+        lib.qgwh_nc_close.argtypes = [vp]
+        lib.qgwh_nc_read.argtypes = [ctypes.c_char_p, P, ctypes.c_int, _dp, _dp, ctypes.POINTER(ctypes.c_int)]
         if lib.qgwh_params_size() != ctypes.sizeof(Params):
             raise QGWError("qgw_params layout mismatch between C and Python")
         _host = lib
@@ -291,6 +298,78 @@ def check_timestep(p):
     h.qgwh_check_timestep(ctypes.byref(p), h.qgwh_nu_hyper_signed(ctypes.byref(p)), ctypes.byref(dt),
                           ctypes.byref(DT))
     return dt.value, DT.value
+
+
+class NcWriter:
+    """netCDF classic output file of a run (create_nc / write_nc,
+    src/netcdf_io.h:38-277): float record variables psi and q over
+    (t, level, y, x); y and x cover the storage grid with its ring (bounded)
+    or the NX x NY interior (periodic)."""
+
+    def __init__(self, path, p):
+        self.p = p
+        self.g = grid(p)[0]
+        st = ctypes.c_int(0)
+        self._h = host_lib().qgwh_nc_create(os.fsencode(path), ctypes.byref(p), ctypes.byref(st))
+        if not self._h:
+            raise QGWError("nc_create(%s): %s" % (path, host_lib().qgwh_strerror(st.value).decode()))
+
+    def write(self, t, psi, q):
+        """Append one record: time t and the storage arrays psi, q
+        (shape (nl, m+2, n+2), float64)."""
+        shape = (self.p.nl, self.g.nyp2, self.g.nxp2)
+        psi = np.ascontiguousarray(psi, dtype=np.float64)
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        if psi.shape != shape or q.shape != shape:
+            raise QGWError("nc_write: fields must have shape %s" % (shape,))
+        st = host_lib().qgwh_nc_write(self._h, float(t), _ptr(psi), _ptr(q))
+        if st != QGW_OK:
+            raise QGWError("nc_write: %s" % host_lib().qgwh_strerror(st).decode())
+
+    @property
+    def records(self):
+        return host_lib().qgwh_nc_records(self._h)
+
+    def close(self):
+        if self._h:
+            h, self._h = self._h, None
+            st = host_lib().qgwh_nc_close(h)
+            if st != QGW_OK:
+                raise QGWError("nc_close: %s" % host_lib().qgwh_strerror(st).decode())
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            host_lib().qgwh_nc_close(self._h)
+
+
+def read_restart(path, p, rec=0, q=None, topo=None):
+    """read_nc of a restart file (src/netcdf_io.h:286-386): variables "q"
+    (nl levels) and "topo" (one level) at record rec, rounded to float as the
+    reference reads them, into storage arrays.  q and topo are updated in
+    place when given (a variable absent from the file leaves them untouched)
+    and allocated as zeros otherwise.  Returns (q, topo, found) with found a
+    set of the variable names read."""
+    g = grid(p)[0]
+    if q is None:
+        q = np.zeros((p.nl, g.nyp2, g.nxp2))
+    if topo is None:
+        topo = np.zeros((g.nyp2, g.nxp2))
+    for a, shape in ((q, (p.nl, g.nyp2, g.nxp2)), (topo, (g.nyp2, g.nxp2))):
+        if a.shape != shape or a.dtype != np.float64 or not a.flags.c_contiguous:
+            raise QGWError("read_restart: arrays must be contiguous float64 of shape %s" % (shape,))
+    found = ctypes.c_int(0)
+    st = host_lib().qgwh_nc_read(os.fsencode(path), ctypes.byref(p), int(rec), _ptr(q), _ptr(topo),
+                                 ctypes.byref(found))
+    if st != QGW_OK:
+        raise QGWError("read_nc(%s): %s" % (path, host_lib().qgwh_strerror(st).decode()))
+    names = {n for b, n in ((1, "q"), (2, "topo")) if found.value & b}
+    return q, topo, names
 
 
 # ---------------------------------------------------------------------------

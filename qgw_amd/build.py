@@ -1,6 +1,7 @@
 """Build the native libraries in-tree.
 
-  libqgw_host.so   host C (gcc): params reader, grid, modes, initial condition
+  libqgw_host.so   host C (gcc): params reader, grid, modes, initial condition,
+                   netCDF classic output and restart input
   libqgw_hip.so    HIP C++ for gfx950 (hipcc): the step kernels and the C-ABI
   qg_hip           the C driver (restated main loop of the reference)
 
@@ -44,7 +45,7 @@ def build(verbose=False):
     os.makedirs(BUILD, exist_ok=True)
     host_so = os.path.join(HERE, "libqgw_host.so")
     _run(["gcc"] + CC_FLAGS + ["-shared", "-o", host_so,
-                               os.path.join(HOST, "qgw_host.c"), "-lm"])
+                               os.path.join(HOST, "qgw_host.c"), os.path.join(HOST, "qgw_nc.c"), "-lm"])
 
     def comp(src):
         obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
