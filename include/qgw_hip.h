@@ -56,9 +56,11 @@ typedef struct qgw_ctx qgw_ctx;
  * QGW_FP32_PLAIN.  q, psi and the tendency history start at zero, topography
  * from the bathtub formula.  The vertical modes must be supplied with
  * qgw_set_modes before the first inversion.  Returns NULL and sets *status
- * on failure (bounded domains only: bc_fac >= 0; 1 <= nl <= QGW_DEV_NL_MAX;
- * 2(n+1) and 2(m+1) in {2^a: 8..8192} or {3*2^a: 24..6144}; n_hyper even
- * and <= 8; no stochastic (k_f) or file (dt_forc_period) forcing). */
+ * on failure (1 <= nl <= QGW_DEV_NL_MAX; bounded (bc_fac >= 0): 2(n+1) and
+ * 2(m+1) in {2^a: 8..8192} or {3*2^a: 24..6144}; periodic (bc_fac = -1):
+ * NX = NY even, in the same length sets; n_hyper even and <= 8; no
+ * stochastic (k_f) forcing; with file forcing, dt_forc_period a positive
+ * multiple of dt_forc > 0). */
 qgw_ctx *qgw_create(const qgw_params *p, int precision, int *status);
 
 /* Free the run (clean_fft, clean_timestep, clean_eigmode: src/elliptic.h:157,
@@ -96,9 +98,10 @@ int qgw_adjust_timestep(qgw_ctx *ctx, double *dt_out);
  * of the interior of q, t += dt (timestep, src/timestep.h:114-167). */
 int qgw_timestep(qgw_ctx *ctx);
 
-/* nsteps consecutive qgw_timestep calls.  With a fixed dt the whole sequence
- * is enqueued without host synchronisation (the tendency buffers rotate by
- * pointer), and is replayed from a captured HIP graph when nsteps allows. */
+/* nsteps consecutive qgw_timestep calls.  With a fixed dt and no 4-D forcing
+ * record change the whole sequence is enqueued without host synchronisation
+ * (the tendency buffers rotate by pointer).  The graph-replayed step is the
+ * batch form (qgw_batch_run). */
 int qgw_run(qgw_ctx *ctx, int nsteps);
 
 /* Energy and enstrophy of the current (psi, q) pair, accumulated in FP64:
@@ -119,6 +122,18 @@ int qgw_set_time(qgw_ctx *ctx, double t, double dt, double dt0, double dt1, int 
  * driver at each output, src/qg.c:185-188). */
 int qgw_set_next_output(qgw_ctx *ctx, double t_out);
 
+/* File of the 4-D forcing (init_4d_forcing/calc_4d_forcing,
+ * src/forcing.h:226-289), used when dt_forc_period != 0: variable
+ * "q_forcing_3d" (t, level, y, x) with dt_forc_period/dt_forc records, read
+ * by qgwh_nc_read_var.  Default "input_vars.nc" in the working directory, as
+ * in the reference.  Each step adds (w1 F[r] + w2 F[r+1 mod nrec]) dt to the
+ * interior of q after the AB3 update, r = floor(t/dt_forc) mod nrec and
+ * w2 = t/dt_forc - floor(t/dt_forc) at the time t before the step; the two
+ * records are read when r changes.  A record that cannot be read fails the
+ * step with QGW_ERR_IO before the state changes (the reference prints the
+ * error and carries on with the previous records). */
+int qgw_set_forcing_file(qgw_ctx *ctx, const char *path);
+
 /* Wait for all work queued on the context's stream. */
 int qgw_synchronize(qgw_ctx *ctx);
 
@@ -138,7 +153,8 @@ int qgw_time_run(qgw_ctx *ctx, int nsteps, double *ms);
 #define QGW_PASS_P4 5      /* tendency + AB3 update */
 #define QGW_PASS_CFL 6     /* CFL reduction (adaptive dt only) */
 #define QGW_PASS_DIAG 7    /* diagnostics reduction */
-#define QGW_NPASS 8
+#define QGW_PASS_FORC 8    /* forcing added after the AB3 update (4-D file forcing) */
+#define QGW_NPASS 9
 int qgw_profile_enable(qgw_ctx *ctx, int on);
 int qgw_profile_read(qgw_ctx *ctx, double *ms, int *count, int npass);
 

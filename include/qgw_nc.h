@@ -45,6 +45,14 @@ int qgwh_nc_close(qgw_nc *nc);
  * 1 for topo. */
 int qgwh_nc_read(const char *path, const qgw_params *p, int rec, double *q, double *topo, int *found);
 
+/* read_nc of one named variable with `layers` levels (src/netcdf_io.h:286-386;
+ * the 4-D forcing reads "q_forcing_3d" with nl levels this way,
+ * src/forcing.h:235-236,262-263) at record `rec` into the storage array out,
+ * same rules as qgwh_nc_read.  *found (may be NULL) is 1 when the variable
+ * exists. */
+int qgwh_nc_read_var(const char *path, const qgw_params *p, const char *name, int layers, int rec, double *out,
+                     int *found);
+
 #ifdef __cplusplus
 }
 #endif

@@ -62,6 +62,7 @@ def lib():
         L.orc_adjust_timestep.argtypes = [vp]
         L.orc_adjust_timestep.restype = ctypes.c_double
         L.orc_run.argtypes = [vp, ctypes.c_int]
+        L.orc_set_forcing4d.argtypes = [vp, ctypes.c_int, ctypes.c_double, _dp]
         L.orc_get_time.argtypes = [vp, _dp, _dp, _dp, _dp, ip, _dp]
         L.orc_set_time.argtypes = [vp, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                    ctypes.c_int, ctypes.c_double]
@@ -146,6 +147,17 @@ class Oracle:
 
     def invert_pv(self):
         self.L.orc_invert_pv(self.s)
+
+    def set_forcing4d(self, records, dt_forc):
+        """4-D forcing records, shape (n_rec, nl, m+2, n+2); None switches it off."""
+        if records is None:
+            self.L.orc_set_forcing4d(self.s, 0, 0.0, None)
+            return
+        r = np.ascontiguousarray(records, dtype=np.float64)
+        if r.shape[1:] != self.shape:
+            raise ValueError("forcing records must have shape (n_rec,) + %s" % (self.shape,))
+        if self.L.orc_set_forcing4d(self.s, r.shape[0], float(dt_forc), _p(r)) != 0:
+            raise ValueError("bad forcing records")
 
     def calc_diff(self):
         self.L.orc_calc_diff(self.s)

@@ -238,6 +238,10 @@ struct orc {
   double t, dt, dt0, dt1, DT_max, t_out;
   int it;
   fftplan px, py;
+  /* 4-D forcing: n_forc records of nl storage fields, spaced dt_forc */
+  int n_forc;
+  double dt_forc;
+  double *forc;
 };
 
 #define IDX(s, i, j, k) ((long)(k) * (s)->Nxp2 * (s)->Nyp2 + (long)(j) * (s)->Nxp2 + (i))
@@ -432,7 +436,7 @@ void orc_destroy(orc *s) {
   free(s->psi); free(s->q); free(s->diff); free(s->lapn); free(s->topo);
   free(s->f0); free(s->f1); free(s->f2); free(s->wrk1); free(s->cw);
   free(s->cl2m); free(s->cm2l); free(s->iRd2);
-  free(s->px.W); free(s->py.W);
+  free(s->px.W); free(s->py.W); free(s->forc);
   free(s);
 }
 
@@ -759,8 +763,41 @@ void orc_timestep(orc *s) {
         q[o] = q[o] + c[0] * f0[o] + c[1] * f1[o] + c[2] * f2[o];
       }
   }
+  if (s->n_forc > 0) {
+    /* calc_4d_forcing (src/forcing.h:246-289) at the time before the step,
+     * added with the new dt (src/timestep.h:152-163) */
+    const double t = s->t;
+    const int irec1 = ((int)floor(t / s->dt_forc)) % s->n_forc;
+    const int irec2 = (irec1 + 1) % s->n_forc;
+    const double w2 = t / s->dt_forc - (int)floor(t / s->dt_forc);
+    const double w1 = 1 - w2;
+    const long S = (long)s->Nxp2 * s->Nyp2 * s->p.nl;
+    const double *F1 = s->forc + irec1 * S, *F2 = s->forc + irec2 * S;
+    for (int k = 0; k < s->p.nl; k++)
+      for (int j = 1; j < s->Nyp1; j++)
+        for (int i = 1; i < s->Nxp1; i++) {
+          const long o = IDX(s, i, j, k);
+          const double qf = w1 * F1[o] + w2 * F2[o];
+          q[o] += qf * s->dt;
+        }
+  }
   s->t = s->t + s->dt;
   s->it++;
+}
+
+int orc_set_forcing4d(orc *s, int n_rec, double dt_forc, const double *records) {
+  free(s->forc);
+  s->forc = NULL;
+  s->n_forc = 0;
+  if (n_rec <= 0) return 0;
+  if (!(dt_forc > 0) || !records) return -1;
+  const size_t S = (size_t)s->Nxp2 * s->Nyp2 * s->p.nl * n_rec;
+  s->forc = (double *)malloc(S * sizeof(double));
+  if (!s->forc) return -1;
+  memcpy(s->forc, records, S * sizeof(double));
+  s->n_forc = n_rec;
+  s->dt_forc = dt_forc;
+  return 0;
 }
 
 void orc_run(orc *s, int nsteps) {

@@ -19,8 +19,10 @@
  *   orc_rhs ................ rhs src/dynamics.h:47-73 with the stencil macros
  *                            src/domain.h:12-36
  *   orc_adjust_timestep .... adjust_timestep src/timestep.h:69-111
- *   orc_timestep ........... timestep src/timestep.h:114-167 (no stochastic or
- *                            4-D forcing), AB3 coefficients doc/timestep.org:95-97
+ *   orc_timestep ........... timestep src/timestep.h:114-167 (no stochastic
+ *                            forcing), AB3 coefficients doc/timestep.org:95-97
+ *   orc_set_forcing4d ...... the records of init_4d_forcing/calc_4d_forcing
+ *                            src/forcing.h:226-289, applied in orc_timestep
  *   orc_dst2d / orc_dst1d .. FFTW RODFT00 definition (src/elliptic.h:63-66)
  *   orc_diagnostics ........ test/test_energy_injection.py:24-25
  *
@@ -32,7 +34,9 @@
  * coefficients against an independent sympy integration (the construction of
  * doc/timestep.org); check_timestep against hand-expanded cases of the macro
  * quirks; adjust_timestep against hand-computed CFL/doubling/snap cases;
- * diagnostics against a sine-mode closed form.  No function is unpinned.
+ * diagnostics against a sine-mode closed form; the 4-D forcing against the
+ * hand-computed triangle wave of a three-record ramp on a sine mode, where
+ * the dynamics vanish exactly.  No function is unpinned.
  */
 #ifndef QGW_ORACLE_H
 #define QGW_ORACLE_H
@@ -79,6 +83,9 @@ void orc_rhs(orc *s, int which_out);
 double orc_adjust_timestep(orc *s);
 void orc_timestep(orc *s);
 void orc_run(orc *s, int nsteps);
+/* 4-D forcing records: n_rec x (nl storage fields), spaced dt_forc (copied);
+ * n_rec = 0 switches it off */
+int orc_set_forcing4d(orc *s, int n_rec, double dt_forc, const double *records);
 void orc_get_time(orc *s, double *t, double *dt, double *dt0, double *dt1, int *it, double *DT_max);
 void orc_set_time(orc *s, double t, double dt, double dt0, double dt1, int it, double t_out);
 void orc_diagnostics(orc *s, double *E_layer, double *Z_layer, double *E_tot, double *Z_tot);

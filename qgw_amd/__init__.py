@@ -27,7 +27,7 @@ NL_MAX = 1000
 
 FP64, FP32, FP32_PLAIN = 0, 1, 2
 FIELD_Q, FIELD_PSI, FIELD_TOPO, FIELD_F0, FIELD_F1, FIELD_F2 = 1, 2, 3, 4, 5, 6
-PASS_NAMES = ["P1", "T1", "P2", "T2", "P3", "P4", "CFL", "DIAG"]
+PASS_NAMES = ["P1", "T1", "P2", "T2", "P3", "P4", "CFL", "DIAG", "FORC"]
 TUNE_P4_TILE = 1
 TUNE_T_KERNEL = 2
 TUNE_P4_WAVES = 3
@@ -157,6 +157,7 @@ def hip_lib():
             ("qgw_set_time", [vp, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                               ctypes.c_int]),
             ("qgw_set_next_output", [vp, ctypes.c_double]),
+            ("qgw_set_forcing_file", [vp, ctypes.c_char_p]),
             ("qgw_synchronize", [vp]), ("qgw_time_run", [vp, ctypes.c_int, _dp]),
             ("qgw_profile_enable", [vp, ctypes.c_int]), ("qgw_profile_read", [vp, _dp, ip, ctypes.c_int]),
             ("qgw_device_bytes", [vp, ctypes.POINTER(ctypes.c_size_t)]),
@@ -483,6 +484,11 @@ class Model:
 
     def set_next_output(self, t_out):
         self._chk(self.lib.qgw_set_next_output(self.ctx, t_out), "set_next_output")
+
+    def set_forcing_file(self, path):
+        """File of the 4-D forcing records (variable q_forcing_3d); default
+        input_vars.nc in the working directory."""
+        self._chk(self.lib.qgw_set_forcing_file(self.ctx, os.fsencode(path)), "set_forcing_file")
 
     def profile(self, on=True):
         self._chk(self.lib.qgw_profile_enable(self.ctx, 1 if on else 0), "profile_enable")

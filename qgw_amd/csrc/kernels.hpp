@@ -1453,6 +1453,42 @@ __global__ void __launch_bounds__(NB) k_diag_final(const double *part, int nblk,
 }
 
 // ===========================================================================
+// Forcing added after the AB3 update (src/timestep.h:143-163): the stochastic
+// field S times s_fac = sqrt(dt) on layer 0, then the 4-D forcing
+// w1 F1 + w2 F2 (records around the time before the step) times dt, on the
+// interior of every layer.  One block row per (layer, grid row); 256 columns
+// per block.  Either term may be absent (null pointer).
+// ===========================================================================
+template <typename T, bool KAHAN>
+__global__ void __launch_bounds__(256)
+k_forcing(Geo g, T *__restrict__ q, T *__restrict__ comp, const T *__restrict__ S, double s_fac,
+          const T *__restrict__ F1, const T *__restrict__ F2, double w1, double w2, double dt) {
+  const int k = blockIdx.x / g.m, j = blockIdx.x % g.m + 1;
+  const int i = blockIdx.y * 256 + threadIdx.x + 1;
+  if (i > g.n) return;
+  const long o = (long)k * g.LS + (long)j * g.P + i;
+  T v = q[o];
+  T c = KAHAN ? comp[o] : T(0);
+  auto add = [&](T inc) {
+    if constexpr (KAHAN) {
+      const T y = inc - c;
+      const T t = v + y;
+      c = (t - v) - y;
+      v = t;
+    } else {
+      v = v + inc;
+    }
+  };
+  if (S && k == 0) add((T)((double)S[o] * s_fac));
+  if (F1) {
+    const double qf = w1 * (double)F1[o] + w2 * (double)F2[o];
+    add((T)(qf * dt));
+  }
+  q[o] = v;
+  if constexpr (KAHAN) comp[o] = c;
+}
+
+// ===========================================================================
 // Conversions between the host FP64 layout and the device arrays.
 // ===========================================================================
 template <typename T>

@@ -293,6 +293,20 @@ int launch_from_f64_t<T>(const LaunchArgs &a, int layers, const double *src, voi
 }
 
 template <>
+int launch_forcing_t<T>(const LaunchArgs &a, int layers, const void *S, double s_fac, const void *F1, const void *F2,
+                        double w1, double w2, double dt) {
+  if (layers < 1 || layers > a.g.nl || a.g.n < 1 || a.g.m < 1) return QGW_ERR_ARG;
+  const dim3 grid((unsigned)(a.g.m * layers), (unsigned)((a.g.n + 255) / 256));
+  if (sizeof(T) == 4 && a.comp)
+    hipLaunchKernelGGL((k_forcing<T, true>), grid, dim3(256), 0, a.stream, a.g, (T *)a.q, (T *)a.comp,
+                       (const T *)S, s_fac, (const T *)F1, (const T *)F2, w1, w2, dt);
+  else
+    hipLaunchKernelGGL((k_forcing<T, false>), grid, dim3(256), 0, a.stream, a.g, (T *)a.q, (T *)nullptr,
+                       (const T *)S, s_fac, (const T *)F1, (const T *)F2, w1, w2, dt);
+  return 0;
+}
+
+template <>
 int launch_to_f64_t<T>(const LaunchArgs &a, int layers, const void *src, const void *resid, double *dst) {
   long tot = (long)(a.g.n + 2) * (a.g.m + 2) * layers;
   long blocks = (tot + 255) / 256;

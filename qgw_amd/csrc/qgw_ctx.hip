@@ -14,9 +14,12 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <string>
+#include <utility>
 #include <vector>
 
 #include "qgw_hip.h"
+#include "qgw_nc.h"
 #include "launch.hpp"
 
 namespace {
@@ -67,6 +70,13 @@ struct qgw_ctx {
   double t, dt, dt0, dt1, DT_max, t_out;
   int it;
   double nu_hyper;
+  // 4-D forcing (init_4d_forcing/calc_4d_forcing, src/forcing.h:226-289):
+  // nrec4 records spaced dt_forc in file forc_path; slots forc4[0..1] hold
+  // records irec4[0..1] (-1: empty)
+  int nrec4;
+  void *forc4[2];
+  int irec4[2];
+  std::string forc_path;
   size_t bytes;
   // profiling
   bool prof;
@@ -384,7 +394,78 @@ int adaptive_dt(qgw_ctx *c, double *dt_new) {
   return QGW_OK;
 }
 
+int ensure_stage(qgw_ctx *c);
+
+// Record rec of the 4-D forcing file into slot `slot` (read_nc of
+// "q_forcing_3d", src/forcing.h:262-263).
+int load_forcing_record(qgw_ctx *c, int slot, int rec) {
+  const size_t cnt = (size_t)(c->g.n + 2) * (c->g.m + 2) * c->p.nl;
+  std::vector<double> h(cnt, 0.0);
+  int found = 0;
+  int st = qgwh_nc_read_var(c->forc_path.c_str(), &c->p, "q_forcing_3d", c->p.nl, rec, h.data(), &found);
+  if (st || !found) {
+    snprintf(c->err, sizeof c->err, "4-D forcing: %s record %d of %s", st ? "cannot read" : "no q_forcing_3d in",
+             rec, c->forc_path.c_str());
+    return st ? st : QGW_ERR_IO;
+  }
+  if ((st = ensure_stage(c))) return st;
+  HIPCHK(c, hipMemcpyAsync(c->stage, h.data(), cnt * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  qgw::LaunchArgs a;
+  a.g = c->g;
+  a.prec = c->prec;
+  a.stream = c->stream;
+  st = qgw::launch_from_f64(a, c->p.nl, c->stage, c->forc4[slot], nullptr);
+  if (st || (st = check_launch(c, "from_f64"))) return st;
+  HIPCHK(c, hipStreamSynchronize(c->stream));  // h is released on return
+  c->irec4[slot] = rec;
+  return QGW_OK;
+}
+
+// The two records around time t (calc_4d_forcing, src/forcing.h:253-270):
+// irec1 = floor(t/dt_forc) mod nrec and its successor.  A slot already
+// holding a wanted record is reused instead of read again.
+int forcing_records_at(qgw_ctx *c, double t) {
+  const long long fl = (long long)floor(t / c->p.dt_forc);
+  const int r1 = (int)(((fl % c->nrec4) + c->nrec4) % c->nrec4), r2 = (r1 + 1) % c->nrec4;
+  if (c->irec4[0] == r1 && c->irec4[1] == r2) return QGW_OK;
+  if (c->irec4[1] == r1) {
+    std::swap(c->forc4[0], c->forc4[1]);
+    std::swap(c->irec4[0], c->irec4[1]);
+  }
+  int st = QGW_OK;
+  if (c->irec4[0] != r1) st = load_forcing_record(c, 0, r1);
+  if (!st && c->irec4[1] != r2) st = load_forcing_record(c, 1, r2);
+  return st;
+}
+
+// q += (w1 F1 + w2 F2) dt on the interior of every layer, with the weights of
+// the time before the step and the new dt (src/forcing.h:273-286,
+// src/timestep.h:152-163).
+int run_forcing(qgw_ctx *c, double t_old) {
+  const double x = t_old / c->p.dt_forc;
+  const double w2 = x - floor(x), w1 = 1 - w2;
+  qgw::LaunchArgs a;
+  a.g = c->g;
+  a.prec = c->prec;
+  a.stream = c->stream;
+  a.q = c->q[c->qcur];
+  a.comp = (c->prec == QGW_FP32) ? c->comp : nullptr;
+  int st;
+  {
+    PassScope ps(c, QGW_PASS_FORC);
+    st = qgw::launch_forcing(a, c->p.nl, nullptr, 0.0, c->forc4[0], c->forc4[1], w1, w2, c->dt);
+  }
+  if (st || (st = check_launch(c, "forcing"))) return st;
+  return QGW_OK;
+}
+
 int one_step(qgw_ctx *c) {
+  // records of the 4-D forcing first, so that a failed read leaves the state as it was
+  const double t_old = c->t;
+  if (c->nrec4 > 0) {
+    int st = forcing_records_at(c, t_old);
+    if (st) return st;
+  }
   // rotate the tendency history (src/timestep.h:117-120)
   void *swap = c->fa[2];
   c->fa[2] = c->fa[1];
@@ -406,6 +487,7 @@ int one_step(qgw_ctx *c) {
   ab3_coeffs(c->dt, c->dt0, c->dt1, cab);
   st = run_p4(c, cab, 1);
   if (st) return st;
+  if (c->nrec4 > 0 && (st = run_forcing(c, t_old))) return st;
   c->t = c->t + c->dt;
   c->it++;
   return QGW_OK;
@@ -420,10 +502,20 @@ bool same_group(const qgw_params &a, const qgw_params &b) {
          a.dh[a.nl - 1] == b.dh[b.nl - 1] && a.bc_fac == b.bc_fac && a.dt_fixed == b.dt_fixed;
 }
 
+// Records of the 4-D forcing: dt_forc_period must be a positive multiple of
+// dt_forc > 0 (init_4d_forcing, src/forcing.h:238-251); 0 when it is not.
+int forcing_records(const qgw_params *p) {
+  if (!(p->dt_forc > 0) || !(p->dt_forc_period > 0)) return 0;
+  const double r = floor(p->dt_forc_period / p->dt_forc);
+  if (r < 1 || r > 1e9 || r * p->dt_forc != p->dt_forc_period) return 0;
+  return (int)r;
+}
+
 int check_params(const qgw_params *p, qgw_grid *grid, int *Lx, int *Ly) {
   if (p->nl < 1 || p->nl > qgw::kMaxLayers) return QGW_ERR_ARG;
   if (p->n_hyper < 0 || (p->n_hyper % 2) != 0 || p->n_hyper > 8) return QGW_ERR_ARG;
-  if (p->k_f != 0 || p->dt_forc_period != 0) return QGW_ERR_ARG;  // stochastic / file forcing not provided
+  if (p->k_f != 0) return QGW_ERR_ARG;  // stochastic forcing not provided
+  if (p->dt_forc_period != 0 && forcing_records(p) < 1) return QGW_ERR_ARG;
   if (qgwh_grid(p, grid, nullptr, nullptr, nullptr, nullptr) != QGW_OK) return QGW_ERR_ARG;
   // transform lengths: the odd extension 2(n+1) (bounded) or the period N (periodic)
   const bool per = (p->bc_fac == -1.0);
@@ -634,6 +726,12 @@ qgw_ctx *create_group(const qgw_params *ps, int nb, int precision, hipStream_t e
   st = st ? st : dalloc(c, (void **)&c->diag_part, sizeof(double) * 2 * c->nblk_diag * NL);
   st = st ? st : dalloc(c, (void **)&c->diag_out, sizeof(double) * 2 * NL);
   st = st ? st : dalloc(c, (void **)&c->cfl_max, sizeof(unsigned long long));
+  c->nrec4 = (p->dt_forc_period != 0) ? forcing_records(p) : 0;
+  c->irec4[0] = c->irec4[1] = -1;
+  c->forc4[0] = c->forc4[1] = nullptr;
+  c->forc_path = "input_vars.nc";  // src/forcing.h:262
+  if (c->nrec4 > 0 && nb != 1) st = st ? st : QGW_ERR_ARG;  // one run per context
+  for (int i = 0; i < 2 && c->nrec4 > 0; i++) st = st ? st : dalloc_field(c, &c->forc4[i], field);
   if (!st) {
     if (precision == QGW_FP64) {
       st = make_twiddles<double>(c, Lx, &c->twx);
@@ -1109,6 +1207,17 @@ int qgw_set_next_output(qgw_ctx *c, double t_out) {
   return QGW_OK;
 }
 
+int qgw_set_forcing_file(qgw_ctx *c, const char *path) {
+  if (!c || !path || !*path) return QGW_ERR_ARG;
+  if (c->nrec4 <= 0) {
+    snprintf(c->err, sizeof c->err, "no 4-D forcing in this run (dt_forc_period = 0)");
+    return QGW_ERR_STATE;
+  }
+  c->forc_path = path;
+  c->irec4[0] = c->irec4[1] = -1;  // read again from the new file
+  return QGW_OK;
+}
+
 int qgw_synchronize(qgw_ctx *c) {
   if (!c) return QGW_ERR_ARG;
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1234,7 +1343,8 @@ qgw_batch *qgw_batch_create(int nmembers, const qgw_params *params, int precisio
   *status = QGW_ERR_ARG;
   if (nmembers < 1 || !params) return nullptr;
   for (int i = 0; i < nmembers; i++)
-    if (!(params[i].dt_fixed > 0)) return nullptr;  // the graph step needs a fixed dt
+    if (!(params[i].dt_fixed > 0) || params[i].dt_forc_period != 0)
+      return nullptr;  // the graph step needs a fixed dt and no file forcing
   qgw_batch *b = new qgw_batch();
   memset(b->err, 0, sizeof b->err);
   b->nmem = nmembers;

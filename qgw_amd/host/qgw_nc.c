@@ -322,156 +322,192 @@ typedef struct {
   int is_rec;
 } ncvar;
 
-int qgwh_nc_read(const char *path, const qgw_params *p, int rec, double *q, double *topo, int *found) {
-  if (found) *found = 0;
-  if (!path || !p || rec < 0) return QGW_ERR_ARG;
-  geom g;
-  if (geom_of(p, &g)) return QGW_ERR_ARG;
-  FILE *f = fopen(path, "rb");
-  if (!f) return QGW_ERR_IO;
-  /* the header is read whole (it is small); data by seek */
-  size_t hcap = 1 << 16;
+/* A file opened for reading: its header parsed. */
+typedef struct {
+  FILE *f;
+  uint32_t numrecs;
+  uint32_t dimlen[64];
+  ncvar *vars;
+  uint32_t nvar;
+  uint64_t recsize;
+} ncfile;
+
+static void nc_release(ncfile *F) {
+  if (F->f) fclose(F->f);
+  free(F->vars);
+  memset(F, 0, sizeof *F);
+}
+
+/* Parses the header (read whole; up to 1 MiB) of a CDF-1 or CDF-2 file. */
+static int nc_open_read(const char *path, ncfile *F) {
+  memset(F, 0, sizeof *F);
+  F->f = fopen(path, "rb");
+  if (!F->f) return QGW_ERR_IO;
+  const size_t hcap = 1 << 20;
   unsigned char *hb = (unsigned char *)malloc(hcap);
-  size_t hn = hb ? fread(hb, 1, hcap, f) : 0;
+  if (!hb) {
+    nc_release(F);
+    return QGW_ERR_NOMEM;
+  }
+  const size_t hn = fread(hb, 1, hcap, F->f);
   int st = QGW_OK;
-  ncvar *vars = NULL;
-  double *fd = NULL;
-  if (!hb || hn < 8 || memcmp(hb, "CDF", 3) || (hb[3] != 1 && hb[3] != 2)) {
-    st = hb ? QGW_ERR_IO : QGW_ERR_NOMEM;
-    goto done;
+  if (hn < 8 || memcmp(hb, "CDF", 3) || (hb[3] != 1 && hb[3] != 2)) {
+    st = QGW_ERR_IO;
+    goto out;
   }
   {
     const int v64 = (hb[3] == 2);
     rbuf b = {hb, hn, 4, 0};
-    const uint32_t numrecs = rb_u32(&b);
-    uint32_t dimlen[64] = {0};
+    F->numrecs = rb_u32(&b);
     int recdim = -1;
     uint32_t tag = rb_u32(&b), ndim = rb_u32(&b);
     if (tag != 0 && (tag != NC_DIMENSION || ndim > 64)) b.err = 1;
     for (uint32_t d = 0; d < ndim && !b.err && tag; d++) {
       uint32_t len;
       rb_name(&b, &len);
-      dimlen[d] = rb_u32(&b);
-      if (dimlen[d] == 0) recdim = (int)d;
+      F->dimlen[d] = rb_u32(&b);
+      if (F->dimlen[d] == 0) recdim = (int)d;
     }
     rb_skip_atts(&b);
     tag = rb_u32(&b);
-    uint32_t nvar = rb_u32(&b);
-    if (tag != 0 && tag != NC_VARIABLE) b.err = 1;
-    if (nvar > 4096) b.err = 1;
-    vars = (ncvar *)calloc(nvar ? nvar : 1, sizeof(ncvar));
-    if (!vars) {
-      st = QGW_ERR_NOMEM;
-      goto done;
+    const uint32_t nvar = rb_u32(&b);
+    if ((tag != 0 && tag != NC_VARIABLE) || nvar > 4096) b.err = 1;
+    if (b.err) {
+      st = QGW_ERR_IO;
+      goto out;
     }
-    uint64_t recsize = 0;
+    F->vars = (ncvar *)calloc(nvar ? nvar : 1, sizeof(ncvar));
+    if (!F->vars) {
+      st = QGW_ERR_NOMEM;
+      goto out;
+    }
+    F->nvar = nvar;
     int nrecvars = 0;
     for (uint32_t v = 0; v < nvar && !b.err; v++) {
+      ncvar *V = &F->vars[v];
       uint32_t len;
       const char *nm = rb_name(&b, &len);
       if (b.err) break;
-      size_t cl = len < 63 ? len : 63;
-      memcpy(vars[v].name, nm, cl);
-      vars[v].name[cl] = 0;
-      uint32_t nd = rb_u32(&b);
+      const size_t cl = len < 63 ? len : 63;
+      memcpy(V->name, nm, cl);
+      V->name[cl] = 0;
+      const uint32_t nd = rb_u32(&b);
       if (nd > 8) {
         b.err = 1;
         break;
       }
-      vars[v].nd = (int)nd;
-      for (uint32_t d = 0; d < nd; d++) vars[v].dims[d] = rb_u32(&b);
+      V->nd = (int)nd;
+      for (uint32_t d = 0; d < nd; d++) {
+        V->dims[d] = rb_u32(&b);
+        if (V->dims[d] >= ndim) b.err = 1;
+      }
       rb_skip_atts(&b);
-      vars[v].type = (int)rb_u32(&b);
-      vars[v].vsize = rb_u32(&b);
-      vars[v].begin = rb_off(&b, v64);
-      vars[v].is_rec = (nd > 0 && (int)vars[v].dims[0] == recdim);
-      if (vars[v].is_rec) {
-        recsize += vars[v].vsize;
+      V->type = (int)rb_u32(&b);
+      V->vsize = rb_u32(&b);
+      V->begin = rb_off(&b, v64);
+      if (V->type < 1 || V->type > 6) b.err = 1;
+      V->is_rec = (nd > 0 && (int)V->dims[0] == recdim);
+      if (V->is_rec) {
+        F->recsize += V->vsize;
         nrecvars++;
       }
     }
     if (b.err) {
       st = QGW_ERR_IO;
-      goto done;
+      goto out;
     }
     /* one record variable: its slab is not padded */
     if (nrecvars == 1)
       for (uint32_t v = 0; v < nvar; v++)
-        if (vars[v].is_rec) {
+        if (F->vars[v].is_rec) {
           uint64_t e = 1;
-          for (int d = 1; d < vars[v].nd; d++) e *= dimlen[vars[v].dims[d]];
-          recsize = e * (uint64_t)type_size[vars[v].type >= 1 && vars[v].type <= 6 ? vars[v].type : 0];
+          for (int d = 1; d < F->vars[v].nd; d++) e *= F->dimlen[F->vars[v].dims[d]];
+          F->recsize = e * (uint64_t)type_size[F->vars[v].type];
         }
-    const struct {
-      const char *name;
-      double *dst;
-      int layers;
-      int bit;
-    } want[2] = {{"q", q, g.nl, 1}, {"topo", topo, 1, 2}};
-    const size_t per = (size_t)g.nxo * g.nyo;
-    fd = (double *)malloc(sizeof(double) * per * g.nl);
-    if (!fd) {
-      st = QGW_ERR_NOMEM;
-      goto done;
-    }
-    for (int w = 0; w < 2 && !st; w++) {
-      if (!want[w].dst) continue;
-      for (uint32_t v = 0; v < nvar; v++) {
-        if (strcmp(vars[v].name, want[w].name)) continue;
-        const ncvar *V = &vars[v];
-        if (V->type != NC_FLOAT && V->type != NC_DOUBLE) {
-          st = QGW_ERR_ARG;
-          break;
-        }
-        uint64_t cnt = 1;
-        for (int d = V->is_rec ? 1 : 0; d < V->nd; d++) cnt *= dimlen[V->dims[d]];
-        /* a slab of the layers x storage extents (src/netcdf_io.h:330-357) */
-        if (cnt != (uint64_t)want[w].layers * per || (V->is_rec && (uint32_t)rec >= numrecs)) {
-          st = QGW_ERR_ARG;
-          break;
-        }
-        const uint64_t off = V->begin + (V->is_rec ? (uint64_t)rec * recsize : 0);
-        const int ts = type_size[V->type];
-        unsigned char *raw = (unsigned char *)malloc((size_t)cnt * ts);
-        if (!raw) {
-          st = QGW_ERR_NOMEM;
-          break;
-        }
-        if (fseek(f, (long)off, SEEK_SET) || fread(raw, (size_t)ts, (size_t)cnt, f) != cnt) {
-          free(raw);
-          st = QGW_ERR_IO;
-          break;
-        }
-        for (uint64_t e = 0; e < cnt; e++) {
-          const unsigned char *c = raw + e * ts;
-          if (ts == 4) {
-            uint32_t u = get_be32(c);
-            float x;
-            memcpy(&x, &u, 4);
-            fd[e] = x;
-          } else {
-            uint64_t u = ((uint64_t)get_be32(c) << 32) | get_be32(c + 4);
-            double x;
-            memcpy(&x, &u, 8);
-            fd[e] = (float)x; /* nc_get_vara_float converts to float */
-          }
-        }
-        free(raw);
-        double *dst = want[w].dst;
-        for (int k = 0; k < want[w].layers; k++)
-          for (int j = g.ibc; j < g.nyp2 - g.ibc; j++)
-            for (int i = g.ibc; i < g.nxp2 - g.ibc; i++)
-              dst[(size_t)k * g.nxp2 * g.nyp2 + (size_t)j * g.nxp2 + i] =
-                  fd[(size_t)k * per + (size_t)(j - g.ibc) * g.nxo + (i - g.ibc)];
-        if (found) *found |= want[w].bit;
-        break;
-      }
-    }
   }
-done:
+out:
   free(hb);
-  free(vars);
-  free(fd);
-  fclose(f);
+  if (st) nc_release(F);
+  return st;
+}
+
+/* Variable `name` (layers levels of the output grid) at record rec, or its
+ * single slab, into storage arrays (src/netcdf_io.h:330-366).  Returns
+ * QGW_OK with *got = 0 when the file has no such variable. */
+static int nc_get(ncfile *F, const geom *g, const char *name, int layers, int rec, double *dst, int *got) {
+  *got = 0;
+  for (uint32_t v = 0; v < F->nvar; v++) {
+    const ncvar *V = &F->vars[v];
+    if (strcmp(V->name, name)) continue;
+    if (V->type != NC_FLOAT && V->type != NC_DOUBLE) return QGW_ERR_ARG;
+    const size_t per = (size_t)g->nxo * g->nyo;
+    uint64_t cnt = 1;
+    for (int d = V->is_rec ? 1 : 0; d < V->nd; d++) cnt *= F->dimlen[V->dims[d]];
+    /* the count the reference asks for: layers x the output extents */
+    if (cnt != (uint64_t)layers * per || (V->is_rec && (uint32_t)rec >= F->numrecs)) return QGW_ERR_ARG;
+    const uint64_t off = V->begin + (V->is_rec ? (uint64_t)rec * F->recsize : 0);
+    const int ts = type_size[V->type];
+    unsigned char *raw = (unsigned char *)malloc((size_t)cnt * ts);
+    if (!raw) return QGW_ERR_NOMEM;
+    if (fseek(F->f, (long)off, SEEK_SET) || fread(raw, (size_t)ts, (size_t)cnt, F->f) != cnt) {
+      free(raw);
+      return QGW_ERR_IO;
+    }
+    for (int k = 0; k < layers; k++)
+      for (int j = g->ibc; j < g->nyp2 - g->ibc; j++)
+        for (int i = g->ibc; i < g->nxp2 - g->ibc; i++) {
+          const size_t e = (size_t)k * per + (size_t)(j - g->ibc) * g->nxo + (i - g->ibc);
+          const unsigned char *c = raw + e * ts;
+          double x;
+          if (ts == 4) {
+            const uint32_t u = get_be32(c);
+            float xf;
+            memcpy(&xf, &u, 4);
+            x = xf;
+          } else {
+            const uint64_t u = ((uint64_t)get_be32(c) << 32) | get_be32(c + 4);
+            double xd;
+            memcpy(&xd, &u, 8);
+            x = (float)xd; /* nc_get_vara_float converts to float */
+          }
+          dst[(size_t)k * g->nxp2 * g->nyp2 + (size_t)j * g->nxp2 + i] = x;
+        }
+    free(raw);
+    *got = 1;
+    return QGW_OK;
+  }
+  return QGW_OK;
+}
+
+int qgwh_nc_read(const char *path, const qgw_params *p, int rec, double *q, double *topo, int *found) {
+  if (found) *found = 0;
+  if (!path || !p || rec < 0) return QGW_ERR_ARG;
+  geom g;
+  if (geom_of(p, &g)) return QGW_ERR_ARG;
+  ncfile F;
+  int st = nc_open_read(path, &F);
+  if (st) return st;
+  int got = 0, bits = 0;
+  /* read_nc's list of src/qg.c:161-163: q (3d_field), topo (2d_field) */
+  if (q && !(st = nc_get(&F, &g, "q", g.nl, rec, q, &got)) && got) bits |= 1;
+  if (!st && topo && !(st = nc_get(&F, &g, "topo", 1, rec, topo, &got)) && got) bits |= 2;
+  nc_release(&F);
+  if (found) *found = bits;
+  return st;
+}
+
+int qgwh_nc_read_var(const char *path, const qgw_params *p, const char *name, int layers, int rec, double *out,
+                     int *found) {
+  if (found) *found = 0;
+  if (!path || !p || !name || !out || layers < 1 || rec < 0) return QGW_ERR_ARG;
+  geom g;
+  if (geom_of(p, &g)) return QGW_ERR_ARG;
+  ncfile F;
+  int st = nc_open_read(path, &F);
+  if (st) return st;
+  int got = 0;
+  st = nc_get(&F, &g, name, layers, rec, out, &got);
+  nc_release(&F);
+  if (found) *found = got;
   return st;
 }

@@ -17,10 +17,11 @@ def _declared(header):
 
 def test_host_library_exports_header():
     lib = Q.host_lib()
-    names = _declared("qgw_host.h")
-    assert names
-    for n in names:
-        assert hasattr(lib, n), n
+    for h in ("qgw_host.h", "qgw_nc.h"):
+        names = _declared(h)
+        assert names
+        for n in names:
+            assert hasattr(lib, n), (h, n)
 
 
 def test_hip_library_exports_header():
@@ -53,4 +54,20 @@ def test_batch_rejects_adaptive_step():
     st = ctypes.c_int(0)
     b = Q.hip_lib().qgw_batch_create(1, ctypes.byref(p), Q.FP32, ctypes.byref(st))
     assert not b
+    assert st.value == -1
+
+
+def test_forcing_parameters_checked_before_device():
+    """File forcing needs dt_forc_period a positive multiple of dt_forc > 0
+    (src/forcing.h:238-251); stochastic forcing is not provided; a batch takes
+    no file forcing.  All rejected before any device call."""
+    lib = Q.hip_lib()
+    st = ctypes.c_int(0)
+    for kw in (dict(dt_forc=0.0, dt_forc_period=100.0), dict(dt_forc=30.0, dt_forc_period=100.0),
+               dict(dt_forc=-10.0, dt_forc_period=-100.0), dict(k_f=10.0, sigma_f=1.0)):
+        p = Q.default_params(NX=64, NY=64, nl=2, dt_fixed=600.0, **kw)
+        assert not lib.qgw_create(ctypes.byref(p), Q.FP64, ctypes.byref(st)), kw
+        assert st.value == -1, kw
+    p = Q.default_params(NX=64, NY=64, nl=2, dt_fixed=600.0, dt_forc=50.0, dt_forc_period=100.0)
+    assert not lib.qgw_batch_create(1, ctypes.byref(p), Q.FP64, ctypes.byref(st))
     assert st.value == -1
