@@ -58,9 +58,9 @@ typedef struct qgw_ctx qgw_ctx;
  * qgw_set_modes before the first inversion.  Returns NULL and sets *status
  * on failure (1 <= nl <= QGW_DEV_NL_MAX; bounded (bc_fac >= 0): 2(n+1) and
  * 2(m+1) in {2^a: 8..8192} or {3*2^a: 24..6144}; periodic (bc_fac = -1):
- * NX = NY even, in the same length sets; n_hyper even and <= 8; no
- * stochastic (k_f) forcing; with file forcing, dt_forc_period a positive
- * multiple of dt_forc > 0). */
+ * NX = NY even, in the same length sets; n_hyper even and <= 8; with
+ * stochastic forcing (k_f != 0), NX = NY; with file forcing, dt_forc_period a
+ * positive multiple of dt_forc > 0). */
 qgw_ctx *qgw_create(const qgw_params *p, int precision, int *status);
 
 /* Free the run (clean_fft, clean_timestep, clean_eigmode: src/elliptic.h:157,
@@ -134,6 +134,17 @@ int qgw_set_next_output(qgw_ctx *ctx, double t_out);
  * error and carries on with the previous records). */
 int qgw_set_forcing_file(qgw_ctx *ctx, const char *path);
 
+/* Stochastic ring forcing (init_stoch_forc/calc_forc, src/forcing.h:67-208),
+ * on when k_f != 0: each step draws sigma_f N(0,1) e^{i theta} times the
+ * envelope for every forced point of qgwh_stoch_ring (qgwh_stoch_coeffs, on
+ * the host), the device forms the real field of that half spectrum (the
+ * unnormalised c2r of the reference's N x N plan, N = NX) and adds it times
+ * sqrt(dt) to the interior of layer 0 after the AB3 update, before the 4-D
+ * forcing (src/timestep.h:143-150).  The stream starts as the C library's
+ * rand() stream does (seed 1), so a run draws the reference's sequence; it
+ * is private to the context.  This call restarts it from another seed. */
+int qgw_set_stoch_seed(qgw_ctx *ctx, unsigned seed);
+
 /* Wait for all work queued on the context's stream. */
 int qgw_synchronize(qgw_ctx *ctx);
 
@@ -153,7 +164,7 @@ int qgw_time_run(qgw_ctx *ctx, int nsteps, double *ms);
 #define QGW_PASS_P4 5      /* tendency + AB3 update */
 #define QGW_PASS_CFL 6     /* CFL reduction (adaptive dt only) */
 #define QGW_PASS_DIAG 7    /* diagnostics reduction */
-#define QGW_PASS_FORC 8    /* forcing added after the AB3 update (4-D file forcing) */
+#define QGW_PASS_FORC 8    /* forcing after the AB3 update (stochastic synthesis, 4-D records) */
 #define QGW_NPASS 9
 int qgw_profile_enable(qgw_ctx *ctx, int on);
 int qgw_profile_read(qgw_ctx *ctx, double *ms, int *count, int npass);

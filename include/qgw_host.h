@@ -118,6 +118,39 @@ void qgwh_check_timestep(const qgw_params *p, double nu_hyper_signed, double *dt
 /* nu_hyper with the sign rule of src/domain.h:210. */
 double qgwh_nu_hyper_signed(const qgw_params *p);
 
+/* ---- stochastic ring forcing (src/forcing.h:67-208) ---------------------- */
+
+/* Forced wavenumbers of init_stoch_forc (src/forcing.h:131-149): with
+ * N = NX (the reference's N_F: NX periodic, NXp1 = NX bounded), k = k_f/(2 pi)
+ * and dk = 1/Lx, the half-spectrum points (i, j), i in [0, N/2], j in [0, N),
+ * whose l = ((j + N/2) mod N - N/2) dk and k_i = i dk lie within dk/2 of the
+ * circle |(k_i, l)| = k along x or along y, in the reference's order (j outer,
+ * i inner).  *n_forced gets their number N_P; up to cap of them go to ii, jj
+ * (may be NULL).  *envelope (may be NULL) gets sqrt(8/(2(N_P-1))) pi k, the
+ * amplitude of calc_forc (src/forcing.h:180), or 0 when N_P <= 1 (then the
+ * reference adds nothing). */
+int qgwh_stoch_ring(const qgw_params *p, int cap, int *ii, int *jj, int *n_forced, double *envelope);
+
+/* Random stream of the forcing.  Seed 1 is the stream of the C library's
+ * rand() without srand, which the reference draws from (src/forcing.h:62-63);
+ * each stream is private to its handle. */
+typedef struct qgwh_rng qgwh_rng;
+qgwh_rng *qgwh_rng_create(unsigned seed);
+void qgwh_rng_destroy(qgwh_rng *rng);
+/* The next value, as rand() would return it. */
+int qgwh_rng_next(qgwh_rng *rng);
+/* A copy of a stream's state (the copy continues where the source is). */
+qgwh_rng *qgwh_rng_clone(const qgwh_rng *rng);
+int qgwh_rng_assign(qgwh_rng *dst, const qgwh_rng *src);
+
+/* One step of calc_forc's coefficients (src/forcing.h:174-190) for count forced
+ * points: per point, in order, magnitude = sigma_f * normal_noise() and
+ * phase = noise() * 2 pi, coef[2p] = envelope * magnitude * cos(phase) and
+ * coef[2p+1] = envelope * magnitude * sin(phase).  normal_noise() draws two
+ * values (the log argument first: the order GCC evaluates the reference's
+ * macro in), noise() one. */
+int qgwh_stoch_coeffs(qgwh_rng *rng, double sigma_f, double envelope, int count, double *coef);
+
 /* Human-readable text for a status code. */
 const char *qgwh_strerror(int status);
 

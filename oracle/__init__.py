@@ -12,7 +12,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 NL_MAX = 1000
-Q, PSI, TOPO, F0, F1, F2, LAPN = 1, 2, 3, 4, 5, 6, 7
+Q, PSI, TOPO, F0, F1, F2, LAPN, FORC = 1, 2, 3, 4, 5, 6, 7, 8
 
 _dp = ctypes.POINTER(ctypes.c_double)
 
@@ -63,6 +63,10 @@ def lib():
         L.orc_adjust_timestep.restype = ctypes.c_double
         L.orc_run.argtypes = [vp, ctypes.c_int]
         L.orc_set_forcing4d.argtypes = [vp, ctypes.c_int, ctypes.c_double, _dp]
+        L.orc_set_stochastic.argtypes = [vp, ctypes.c_double, ctypes.c_double]
+        L.orc_stoch_points.argtypes = [vp, ip, ip]
+        L.orc_stoch_draw.argtypes = [vp, _dp]
+        L.orc_c2r2d.argtypes = [_dp, ctypes.c_int, _dp]
         L.orc_get_time.argtypes = [vp, _dp, _dp, _dp, _dp, ip, _dp]
         L.orc_set_time.argtypes = [vp, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                    ctypes.c_int, ctypes.c_double]
@@ -128,9 +132,10 @@ class Oracle:
     def field(self, which):
         """A numpy view of an oracle field (shares memory; copy to keep)."""
         ptr = self.L.orc_field(self.s, which)
-        size = self.shape[1] * self.shape[2] * (1 if which == TOPO else self.nl)
+        one = which in (TOPO, FORC)
+        size = self.shape[1] * self.shape[2] * (1 if one else self.nl)
         a = np.ctypeslib.as_array(ptr, shape=(size,))
-        return a.reshape((self.shape[1], self.shape[2]) if which == TOPO else self.shape)
+        return a.reshape((self.shape[1], self.shape[2]) if one else self.shape)
 
     def modes(self):
         nl = self.nl
@@ -147,6 +152,26 @@ class Oracle:
 
     def invert_pv(self):
         self.L.orc_invert_pv(self.s)
+
+    def set_stochastic(self, k_f, sigma_f):
+        """Stochastic ring forcing; resets the C library's rand() stream.
+        Returns the number of forced points."""
+        n = self.L.orc_set_stochastic(self.s, float(k_f), float(sigma_f))
+        if n < 0:
+            raise ValueError("stochastic forcing needs an NX x NX field inside the grid")
+        return n
+
+    def stoch_points(self):
+        n = self.L.orc_stoch_points(self.s, None, None)
+        ii, jj = (ctypes.c_int * max(n, 1))(), (ctypes.c_int * max(n, 1))()
+        self.L.orc_stoch_points(self.s, ii, jj)
+        return np.array(ii[:n]), np.array(jj[:n])
+
+    def stoch_draw(self):
+        n = self.L.orc_stoch_points(self.s, None, None)
+        c = np.zeros(2 * max(n, 1))
+        self.L.orc_stoch_draw(self.s, _p(c))
+        return c[:2 * n]
 
     def set_forcing4d(self, records, dt_forc):
         """4-D forcing records, shape (n_rec, nl, m+2, n+2); None switches it off."""

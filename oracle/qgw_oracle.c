@@ -242,6 +242,13 @@ struct orc {
   int n_forc;
   double dt_forc;
   double *forc;
+  /* stochastic ring forcing: n_P forced half-spectrum points (st_i, st_j) of
+   * an NF x NF grid, amplitude st_env, the spectrum and the field forc_s */
+  int stoch, NF, n_P;
+  int *st_i, *st_j;
+  double st_kf, st_sigma;
+  cplx *st_spec;
+  double *forc_p, *forc_s;
 };
 
 #define IDX(s, i, j, k) ((long)(k) * (s)->Nxp2 * (s)->Nyp2 + (long)(j) * (s)->Nxp2 + (i))
@@ -437,6 +444,7 @@ void orc_destroy(orc *s) {
   free(s->f0); free(s->f1); free(s->f2); free(s->wrk1); free(s->cw);
   free(s->cl2m); free(s->cm2l); free(s->iRd2);
   free(s->px.W); free(s->py.W); free(s->forc);
+  free(s->st_i); free(s->st_j); free(s->st_spec); free(s->forc_p); free(s->forc_s);
   free(s);
 }
 
@@ -461,6 +469,7 @@ double *orc_field(orc *s, int which) {
     case ORC_Q: return s->q;
     case ORC_PSI: return s->psi;
     case ORC_TOPO: return s->topo;
+    case ORC_FORC: return s->forc_s;
     case ORC_F0: return s->f0;
     case ORC_F1: return s->f1;
     case ORC_F2: return s->f2;
@@ -740,6 +749,136 @@ void orc_ab3_coeffs(double dt, double dt0, double dt1, double *c) {
   c[2] = (dt * dt) * (dt / 3 + dt0 / 2) / (dt1 * (dt0 + dt1));
 }
 
+/* c2r of the reference's FFTW plan (unnormalised backward transform of an
+ * N x N real field from its N x (N/2+1) half spectrum, src/forcing.h:119,196)
+ * restated for a half spectrum that need not be Hermitian: inverse complex
+ * DFTs along j for every column i, then along i the real inverse of each
+ * half-spectrum row, which uses only the real parts of its i = 0 and (N even)
+ * i = N/2 entries. */
+int orc_c2r2d(const double *spec, int N, double *out) {
+  if (N < 1) return -1;
+  const int NK = N / 2 + 1;
+  fftplan pl;
+  if (plan_init(&pl, N)) return -1;
+  cplx *G = (cplx *)malloc(sizeof(cplx) * (size_t)N * NK);
+  cplx *b1 = (cplx *)malloc(sizeof(cplx) * N), *b2 = (cplx *)malloc(sizeof(cplx) * N);
+  for (int i = 0; i < NK; i++) {
+    for (int j = 0; j < N; j++) b1[j] = conj(spec[2 * ((long)j * NK + i)] + I * spec[2 * ((long)j * NK + i) + 1]);
+    fft_rec(&pl, b2, b1, N, 1, 1);
+    for (int j = 0; j < N; j++) G[(long)j * NK + i] = conj(b2[j]);
+  }
+  for (int j = 0; j < N; j++) {
+    const cplx *g = G + (long)j * NK;
+    b1[0] = creal(g[0]);
+    for (int i = 1; i < NK; i++) {
+      if (2 * i == N) {
+        b1[i] = creal(g[i]);
+      } else {
+        b1[i] = g[i];
+        b1[N - i] = conj(g[i]);
+      }
+    }
+    for (int i = 0; i < N; i++) b1[i] = conj(b1[i]);
+    fft_rec(&pl, b2, b1, N, 1, 1);
+    for (int i = 0; i < N; i++) out[(long)j * N + i] = creal(conj(b2[i]));
+  }
+  free(G); free(b1); free(b2); free(pl.W);
+  return 0;
+}
+
+/* init_stoch_forc (src/forcing.h:67-166): the forced points of the half
+ * spectrum, j outer and i inner.  The C library's rand() stream is reset
+ * (srand(1) is the stream a program starts with), so one stochastic oracle
+ * at a time draws the reference's sequence. */
+int orc_set_stochastic(orc *s, double k_f, double sigma_f) {
+  free(s->st_i); free(s->st_j); free(s->st_spec); free(s->forc_p); free(s->forc_s);
+  s->st_i = s->st_j = NULL;
+  s->st_spec = NULL;
+  s->forc_p = s->forc_s = NULL;
+  s->stoch = 0;
+  if (k_f == 0) return 0;
+  const int N = s->p.NX; /* N_F: NX periodic, NXp1 = NX bounded */
+  if (N < 2 || s->Nx + 1 < N || s->Ny + 1 < N) return -1; /* the N x N field must fit */
+  const int NK = N / 2 + 1;
+  s->NF = N;
+  s->st_kf = k_f / (2 * PI_REF);
+  s->st_sigma = sigma_f;
+  const double kf = s->st_kf, dk = 1. / s->p.Lx;
+  s->st_i = (int *)malloc(sizeof(int) * (size_t)N * NK);
+  s->st_j = (int *)malloc(sizeof(int) * (size_t)N * NK);
+  int np = 0;
+  for (int j = 0; j < N; j++)
+    for (int i = 0; i < NK; i++) {
+      double l = (((j + N / 2) % N) - N / 2) * dk;
+      double k = i * dk;
+      double dk_ = fabs(k - sqrt(kf * kf - l * l));
+      double m1 = fabs(l - sqrt(kf * kf - k * k)), m2 = fabs(l + sqrt(kf * kf - k * k));
+      double dl_ = m1 > m2 ? m2 : m1;
+      if (dk_ < dk / 2 || dl_ < dk / 2) {
+        s->st_i[np] = i;
+        s->st_j[np] = j;
+        np++;
+      }
+    }
+  s->n_P = np;
+  s->st_spec = (cplx *)calloc((size_t)N * NK, sizeof(cplx));
+  s->forc_p = (double *)calloc((size_t)N * N, sizeof(double));
+  s->forc_s = (double *)calloc((size_t)s->Nxp2 * s->Nyp2, sizeof(double));
+  s->stoch = 1;
+  srand(1);
+  return np;
+}
+
+int orc_stoch_points(orc *s, int *ii, int *jj) {
+  for (int p = 0; s->stoch && p < s->n_P; p++) {
+    if (ii) ii[p] = s->st_i[p];
+    if (jj) jj[p] = s->st_j[p];
+  }
+  return s->stoch ? s->n_P : 0;
+}
+
+/* One step of calc_forc's random coefficients (src/forcing.h:178-190), from
+ * the C library's rand(); the Box-Muller normal is written as one expression
+ * of two rand() calls, as the reference's macro is, so the compiler orders
+ * the two draws as it does there. */
+static void stoch_draw(orc *s, double *coef) {
+  const double pi = PI_REF, k_f = s->st_kf;
+  for (int p = 0; p < s->n_P; p++) {
+    double envelope = sqrt((double)8 / ((s->n_P - 1) * 2)) * pi * k_f;
+    double magnitude = s->st_sigma * (sqrt(-2. * log(((double)(rand()) + 1.) / ((double)(RAND_MAX) + 2.))) *
+                                      cos(2 * pi * rand() / (double)RAND_MAX));
+    double phase = (((double)rand() / RAND_MAX)) * 2 * pi;
+    coef[2 * p] = envelope * magnitude * cos(phase);
+    coef[2 * p + 1] = envelope * magnitude * sin(phase);
+  }
+}
+
+int orc_stoch_draw(orc *s, double *coef) {
+  if (!s->stoch) return -1;
+  stoch_draw(s, coef);
+  return s->n_P;
+}
+
+/* calc_forc (src/forcing.h:168-208): the spectrum of this step's draws, its
+ * c2r, copied to forc_s[i+1][j+1] of layer 0. */
+static void calc_forc(orc *s) {
+  if (s->n_P <= 1) return;
+  const int N = s->NF, NK = N / 2 + 1;
+  double *coef = (double *)malloc(sizeof(double) * 2 * s->n_P);
+  double *spec = (double *)calloc((size_t)2 * N * NK, sizeof(double));
+  stoch_draw(s, coef);
+  for (int p = 0; p < s->n_P; p++) {
+    long o = (long)s->st_j[p] * NK + s->st_i[p];
+    spec[2 * o] = coef[2 * p];
+    spec[2 * o + 1] = coef[2 * p + 1];
+  }
+  orc_c2r2d(spec, N, s->forc_p);
+  for (int j = 0; j < N; j++)
+    for (int i = 0; i < N; i++) s->forc_s[IDX(s, i + 1, j + 1, 0)] = s->forc_p[(long)j * N + i];
+  free(coef);
+  free(spec);
+}
+
 void orc_timestep(orc *s) {
   double *swap = s->f2;
   s->f2 = s->f1;
@@ -762,6 +901,12 @@ void orc_timestep(orc *s) {
         long o = IDX(s, i, j, k);
         q[o] = q[o] + c[0] * f0[o] + c[1] * f1[o] + c[2] * f2[o];
       }
+  }
+  if (s->stoch) {
+    /* stochastic forcing on layer 0, times sqrt(dt) (src/timestep.h:143-150) */
+    calc_forc(s);
+    for (int j = 1; j < s->Nyp1; j++)
+      for (int i = 1; i < s->Nxp1; i++) q[IDX(s, i, j, 0)] += s->forc_s[IDX(s, i, j, 0)] * sqrt(s->dt);
   }
   if (s->n_forc > 0) {
     /* calc_4d_forcing (src/forcing.h:246-289) at the time before the step,

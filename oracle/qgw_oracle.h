@@ -23,6 +23,9 @@
  *                            forcing), AB3 coefficients doc/timestep.org:95-97
  *   orc_set_forcing4d ...... the records of init_4d_forcing/calc_4d_forcing
  *                            src/forcing.h:226-289, applied in orc_timestep
+ *   orc_set_stochastic ..... init_stoch_forc src/forcing.h:67-166; calc_forc
+ *                            src/forcing.h:168-208 in orc_timestep
+ *   orc_c2r2d .............. the c2r plan of src/forcing.h:119,196
  *   orc_dst2d / orc_dst1d .. FFTW RODFT00 definition (src/elliptic.h:63-66)
  *   orc_diagnostics ........ test/test_energy_injection.py:24-25
  *
@@ -36,7 +39,10 @@
  * quirks; adjust_timestep against hand-computed CFL/doubling/snap cases;
  * diagnostics against a sine-mode closed form; the 4-D forcing against the
  * hand-computed triangle wave of a three-record ramp on a sine mode, where
- * the dynamics vanish exactly.  No function is unpinned.
+ * the dynamics vanish exactly; the stochastic ring against a hand-counted
+ * ring and the integer geometry of the selection rule, its c2r against
+ * numpy.fft.irfft2, its energy injection against the sigma^2 rate of
+ * doc/forcing.org.  No function is unpinned.
  */
 #ifndef QGW_ORACLE_H
 #define QGW_ORACLE_H
@@ -69,6 +75,7 @@ typedef struct orc orc;
 #define ORC_F1 5
 #define ORC_F2 6
 #define ORC_LAPN 7 /* lap_n_diff */
+#define ORC_FORC 8 /* stochastic forcing field of the last step (layer 0) */
 
 orc *orc_create(const orc_params *p, int *status);
 void orc_destroy(orc *s);
@@ -86,6 +93,12 @@ void orc_run(orc *s, int nsteps);
 /* 4-D forcing records: n_rec x (nl storage fields), spaced dt_forc (copied);
  * n_rec = 0 switches it off */
 int orc_set_forcing4d(orc *s, int n_rec, double dt_forc, const double *records);
+/* stochastic ring forcing (k_f = 0 switches it off); returns the number of
+ * forced points N_P.  Resets the C library's rand() stream (srand(1)). */
+int orc_set_stochastic(orc *s, double k_f, double sigma_f);
+int orc_stoch_points(orc *s, int *ii, int *jj);
+int orc_stoch_draw(orc *s, double *coef);   /* one step's coefficients, 2 per point */
+int orc_c2r2d(const double *spec, int N, double *out); /* N x (N/2+1) complex -> N x N */
 void orc_get_time(orc *s, double *t, double *dt, double *dt0, double *dt1, int *it, double *DT_max);
 void orc_set_time(orc *s, double t, double dt, double dt0, double dt1, int it, double t_out);
 void orc_diagnostics(orc *s, double *E_layer, double *Z_layer, double *E_tot, double *Z_tot);

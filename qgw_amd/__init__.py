@@ -127,6 +127,15 @@ def host_lib():
         lib.qgwh_nc_records.argtypes = [vp]
         lib.qgwh_nc_close.argtypes = [vp]
         lib.qgwh_nc_read.argtypes = [ctypes.c_char_p, P, ctypes.c_int, _dp, _dp, ctypes.POINTER(ctypes.c_int)]
+        lib.qgwh_nc_read_var.argtypes = [ctypes.c_char_p, P, ctypes.c_char_p, ctypes.c_int, ctypes.c_int, _dp,
+                                         ctypes.POINTER(ctypes.c_int)]
+        ip_ = ctypes.POINTER(ctypes.c_int)
+        lib.qgwh_stoch_ring.argtypes = [P, ctypes.c_int, ip_, ip_, ip_, _dp]
+        lib.qgwh_rng_create.argtypes = [ctypes.c_uint]
+        lib.qgwh_rng_create.restype = vp
+        lib.qgwh_rng_destroy.argtypes = [vp]
+        lib.qgwh_rng_next.argtypes = [vp]
+        lib.qgwh_stoch_coeffs.argtypes = [vp, ctypes.c_double, ctypes.c_double, ctypes.c_int, _dp]
         if lib.qgwh_params_size() != ctypes.sizeof(Params):
             raise QGWError("qgw_params layout mismatch between C and Python")
         _host = lib
@@ -157,7 +166,7 @@ def hip_lib():
             ("qgw_set_time", [vp, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                               ctypes.c_int]),
             ("qgw_set_next_output", [vp, ctypes.c_double]),
-            ("qgw_set_forcing_file", [vp, ctypes.c_char_p]),
+            ("qgw_set_forcing_file", [vp, ctypes.c_char_p]), ("qgw_set_stoch_seed", [vp, ctypes.c_uint]),
             ("qgw_synchronize", [vp]), ("qgw_time_run", [vp, ctypes.c_int, _dp]),
             ("qgw_profile_enable", [vp, ctypes.c_int]), ("qgw_profile_read", [vp, _dp, ip, ctypes.c_int]),
             ("qgw_device_bytes", [vp, ctypes.POINTER(ctypes.c_size_t)]),
@@ -299,6 +308,43 @@ def check_timestep(p):
     h.qgwh_check_timestep(ctypes.byref(p), h.qgwh_nu_hyper_signed(ctypes.byref(p)), ctypes.byref(dt),
                           ctypes.byref(DT))
     return dt.value, DT.value
+
+
+def stoch_ring(p):
+    """Forced half-spectrum points (i, j) of the stochastic ring forcing in the
+    reference's order, and the coefficient amplitude (init_stoch_forc,
+    src/forcing.h:131-149,180)."""
+    h = host_lib()
+    n, env = ctypes.c_int(0), ctypes.c_double(0.0)
+    st = h.qgwh_stoch_ring(ctypes.byref(p), 0, None, None, ctypes.byref(n), ctypes.byref(env))
+    if st != QGW_OK:
+        raise QGWError("stoch_ring: %s" % h.qgwh_strerror(st).decode())
+    ii, jj = (ctypes.c_int * max(n.value, 1))(), (ctypes.c_int * max(n.value, 1))()
+    h.qgwh_stoch_ring(ctypes.byref(p), n.value, ii, jj, ctypes.byref(n), ctypes.byref(env))
+    return np.array(ii[:n.value]), np.array(jj[:n.value]), env.value
+
+
+class Rng:
+    """The forcing's random stream (seed 1: the C library's rand() stream)."""
+
+    def __init__(self, seed=1):
+        self._h = host_lib().qgwh_rng_create(seed)
+        if not self._h:
+            raise QGWError("rng_create failed")
+
+    def next(self, n=1):
+        return np.array([host_lib().qgwh_rng_next(self._h) for _ in range(n)])
+
+    def coeffs(self, sigma_f, envelope, count):
+        """One step of calc_forc's coefficients: (count, 2) re/im."""
+        c = np.zeros(2 * max(count, 1))
+        host_lib().qgwh_stoch_coeffs(self._h, sigma_f, envelope, count, _ptr(c))
+        return c[:2 * count].reshape(count, 2)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            host_lib().qgwh_rng_destroy(self._h)
+            self._h = None
 
 
 class NcWriter:
@@ -484,6 +530,11 @@ class Model:
 
     def set_next_output(self, t_out):
         self._chk(self.lib.qgw_set_next_output(self.ctx, t_out), "set_next_output")
+
+    def set_stoch_seed(self, seed):
+        """Restart the stochastic forcing's random stream (seed 1: the C
+        library's rand() stream, the reference's)."""
+        self._chk(self.lib.qgw_set_stoch_seed(self.ctx, seed), "set_stoch_seed")
 
     def set_forcing_file(self, path):
         """File of the 4-D forcing records (variable q_forcing_3d); default

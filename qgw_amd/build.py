@@ -45,7 +45,8 @@ def build(verbose=False):
     os.makedirs(BUILD, exist_ok=True)
     host_so = os.path.join(HERE, "libqgw_host.so")
     _run(["gcc"] + CC_FLAGS + ["-shared", "-o", host_so,
-                               os.path.join(HOST, "qgw_host.c"), os.path.join(HOST, "qgw_nc.c"), "-lm"])
+                               os.path.join(HOST, "qgw_host.c"), os.path.join(HOST, "qgw_nc.c"),
+                               os.path.join(HOST, "qgw_stoch.c"), "-lm"])
 
     def comp(src):
         obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")

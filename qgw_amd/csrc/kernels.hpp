@@ -1453,6 +1453,63 @@ __global__ void __launch_bounds__(NB) k_diag_final(const double *part, int nblk,
 }
 
 // ===========================================================================
+// Stochastic ring forcing field (calc_forc, src/forcing.h:168-208): the
+// unnormalised c2r of a sparse N x (N/2+1) half spectrum, synthesised
+// directly.  The nc forced columns are col_i[c] with their points
+// mode_j/mode_k[col_start[c] .. col_start[c+1]) (row j, index into coef).
+//   G[j'][c] = sum_p coef[k_p] e^{+2 pi i j_p j'/N}            (k_stoch_cols)
+//   S[j'][i'] = sum_c w_c Re(G[j'][c] e^{+2 pi i col_i[c] i'/N})   (k_stoch_rows)
+// with w = 1 for the zero and Nyquist columns (only their real parts enter a
+// real inverse), 2 otherwise.  tw[t] = e^{+2 pi i t/N}.  S point (i', j') is
+// storage point (i'+1, j'+1) of layer 0; rows j' < m and columns i' < n.
+// ===========================================================================
+template <typename T>  // the field type of the launch unit (the sums are FP64 either way)
+__global__ void __launch_bounds__(256)
+k_stoch_cols(int N, int nc, const int *__restrict__ col_start, const int *__restrict__ mode_j,
+             const int *__restrict__ mode_k, const double2 *__restrict__ coef, const double2 *__restrict__ tw,
+             double2 *__restrict__ G) {
+  const long e = blockIdx.x * 256L + threadIdx.x;
+  if (e >= (long)N * nc) return;
+  const int jp = (int)(e / nc), c = (int)(e % nc);
+  double re = 0, im = 0;
+  for (int p = col_start[c]; p < col_start[c + 1]; p++) {
+    const double2 a = coef[mode_k[p]];
+    const double2 w = tw[(int)(((long)mode_j[p] * jp) % N)];
+    re += a.x * w.x - a.y * w.y;
+    im += a.x * w.y + a.y * w.x;
+  }
+  G[e] = make_double2(re, im);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_stoch_rows(Geo g, int N, int nc, const int *__restrict__ col_i, const double2 *__restrict__ G,
+             const double2 *__restrict__ tw, T *__restrict__ S) {
+  __shared__ double2 sg[256];
+  __shared__ int si[256];
+  const int jp = blockIdx.x;                     // output row j' < m
+  const int ip = blockIdx.y * 256 + threadIdx.x;  // output column i' < n
+  double acc = 0;
+  for (int c0 = 0; c0 < nc; c0 += 256) {
+    const int cn = min(256, nc - c0);
+    __syncthreads();
+    if ((int)threadIdx.x < cn) {
+      sg[threadIdx.x] = G[(long)jp * nc + c0 + threadIdx.x];
+      si[threadIdx.x] = col_i[c0 + threadIdx.x];
+    }
+    __syncthreads();
+    if (ip < g.n)
+      for (int c = 0; c < cn; c++) {
+        const int ci = si[c];
+        const double2 w = tw[(int)(((long)ci * ip) % N)];
+        const double v = sg[c].x * w.x - sg[c].y * w.y;
+        acc += (ci == 0 || 2 * ci == N) ? v : 2 * v;
+      }
+  }
+  if (ip < g.n) S[(long)(jp + 1) * g.P + ip + 1] = (T)acc;
+}
+
+// ===========================================================================
 // Forcing added after the AB3 update (src/timestep.h:143-163): the stochastic
 // field S times s_fac = sqrt(dt) on layer 0, then the 4-D forcing
 // w1 F1 + w2 F2 (records around the time before the step) times dt, on the

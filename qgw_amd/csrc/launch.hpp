@@ -52,6 +52,15 @@ template <typename T> int launch_cfl_t(const LaunchArgs &a);
 template <typename T> int launch_diag_t(const LaunchArgs &a, double *part, int nblk, double *out);
 template <typename T> int launch_from_f64_t(const LaunchArgs &a, int layers, const double *src, void *dst, void *resid);
 template <typename T> int launch_to_f64_t(const LaunchArgs &a, int layers, const void *src, const void *resid, double *dst);
+// Stochastic forcing field into S (layer 0): k_stoch_cols then k_stoch_rows.
+struct StochArgs {
+  int N = 0, nc = 0;
+  const int *col_i = nullptr, *col_start = nullptr, *mode_j = nullptr, *mode_k = nullptr;
+  const double *coef = nullptr, *tw = nullptr;
+  double *G = nullptr;
+  void *S = nullptr;
+};
+template <typename T> int launch_stoch_t(const LaunchArgs &a, const StochArgs &s);
 // Post-AB3 forcing on a.q (k_forcing): layers = 1 when only S is given.
 template <typename T>
 int launch_forcing_t(const LaunchArgs &a, int layers, const void *S, double s_fac, const void *F1, const void *F2,
@@ -74,7 +83,8 @@ bool launch_has_half_length(int M);  // lengths M = L/2 of the half-length trans
   template <> int launch_to_f64_t<T>(const LaunchArgs &a, int layers, const void *src, const void *resid, \
                                      double *dst);                                                        \
   template <> int launch_forcing_t<T>(const LaunchArgs &a, int layers, const void *S, double s_fac,       \
-                                      const void *F1, const void *F2, double w1, double w2, double dt);
+                                      const void *F1, const void *F2, double w1, double w2, double dt);  \
+  template <> int launch_stoch_t<T>(const LaunchArgs &a, const StochArgs &s);
 QGW_DECLARE_SPECS(float)
 QGW_DECLARE_SPECS(double)
 #undef QGW_DECLARE_SPECS
@@ -98,6 +108,7 @@ inline int launch_from_f64(const LaunchArgs &a, int layers, const double *src, v
 inline int launch_to_f64(const LaunchArgs &a, int layers, const void *src, const void *resid, double *dst) {
   return QGW_DISPATCH(launch_to_f64, a, layers, src, resid, dst);
 }
+inline int launch_stoch(const LaunchArgs &a, const StochArgs &s) { return QGW_DISPATCH(launch_stoch, a, s); }
 inline int launch_forcing(const LaunchArgs &a, int layers, const void *S, double s_fac, const void *F1,
                           const void *F2, double w1, double w2, double dt) {
   return QGW_DISPATCH(launch_forcing, a, layers, S, s_fac, F1, F2, w1, w2, dt);

@@ -292,6 +292,18 @@ int launch_from_f64_t<T>(const LaunchArgs &a, int layers, const double *src, voi
   return 0;
 }
 
+template <> int launch_stoch_t<T>(const LaunchArgs &a, const StochArgs &s) {
+  // the N x N field covers the m x n interior (N = n+1 bounded, n periodic)
+  if (s.N < 2 || s.nc < 1 || a.g.n > s.N || a.g.m > s.N) return QGW_ERR_ARG;
+  const long cells = (long)s.N * s.nc;
+  hipLaunchKernelGGL((k_stoch_cols<T>), dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, a.stream, s.N, s.nc,
+                     s.col_start, s.mode_j, s.mode_k, (const double2 *)s.coef, (const double2 *)s.tw,
+                     (double2 *)s.G);
+  hipLaunchKernelGGL((k_stoch_rows<T>), dim3((unsigned)a.g.m, (unsigned)((a.g.n + 255) / 256)), dim3(256), 0,
+                     a.stream, a.g, s.N, s.nc, s.col_i, (const double2 *)s.G, (const double2 *)s.tw, (T *)s.S);
+  return 0;
+}
+
 template <>
 int launch_forcing_t<T>(const LaunchArgs &a, int layers, const void *S, double s_fac, const void *F1, const void *F2,
                         double w1, double w2, double dt) {

@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "qgw_hip.h"
+#include "qgw_host.h"
 #include "qgw_nc.h"
 #include "launch.hpp"
 
@@ -77,6 +78,17 @@ struct qgw_ctx {
   void *forc4[2];
   int irec4[2];
   std::string forc_path;
+  // stochastic ring forcing (init_stoch_forc/calc_forc, src/forcing.h:67-208):
+  // nst forced points (coefficients drawn on the host from rng each step,
+  // synthesised into st_S on the device); nothing is added when nst <= 1
+  int nst;
+  double st_env;
+  qgwh_rng *rng;
+  qgw::StochArgs sa;
+  void *st_S;
+  double *st_host[2];  // pinned staging of the coefficients, alternating
+  hipEvent_t st_ev[2];
+  int st_slot;
   size_t bytes;
   // profiling
   bool prof;
@@ -219,6 +231,11 @@ int pitch_of(int w) { return (w + 63) / 64 * 64; }
 
 void release(qgw_ctx *c) {
   if (!c) return;
+  if (c->rng) qgwh_rng_destroy(c->rng);
+  for (int i = 0; i < 2; i++) {
+    if (c->st_host[i]) hipHostFree(c->st_host[i]);
+    if (c->st_ev[i]) hipEventDestroy(c->st_ev[i]);
+  }
   for (void *p : c->allocs)
     if (p) hipFree(p);
   for (auto &r : c->recs) {
@@ -438,22 +455,44 @@ int forcing_records_at(qgw_ctx *c, double t) {
   return st;
 }
 
-// q += (w1 F1 + w2 F2) dt on the interior of every layer, with the weights of
-// the time before the step and the new dt (src/forcing.h:273-286,
-// src/timestep.h:152-163).
+// The forcing after the AB3 update (src/timestep.h:143-163), in the
+// reference's order: the stochastic field of this step's draws times sqrt(dt)
+// on layer 0, then (w1 F1 + w2 F2) dt on every layer with the weights of the
+// time before the step (src/forcing.h:273-286); dt is the new step.
 int run_forcing(qgw_ctx *c, double t_old) {
-  const double x = t_old / c->p.dt_forc;
-  const double w2 = x - floor(x), w1 = 1 - w2;
+  const bool stoch = c->nst > 1, f4 = c->nrec4 > 0;
+  if (!stoch && !f4) return QGW_OK;
   qgw::LaunchArgs a;
   a.g = c->g;
   a.prec = c->prec;
   a.stream = c->stream;
   a.q = c->q[c->qcur];
   a.comp = (c->prec == QGW_FP32) ? c->comp : nullptr;
-  int st;
+  double w1 = 0, w2 = 0;
+  if (f4) {
+    const double x = t_old / c->p.dt_forc;
+    w2 = x - floor(x);
+    w1 = 1 - w2;
+  }
+  int st = QGW_OK;
   {
     PassScope ps(c, QGW_PASS_FORC);
-    st = qgw::launch_forcing(a, c->p.nl, nullptr, 0.0, c->forc4[0], c->forc4[1], w1, w2, c->dt);
+    if (stoch) {
+      // this step's coefficients (calc_forc's draws) through an alternating
+      // pinned buffer: the copy that last read it must be done
+      const int slot = c->st_slot;
+      c->st_slot ^= 1;
+      HIPCHK(c, hipEventSynchronize(c->st_ev[slot]));
+      st = qgwh_stoch_coeffs(c->rng, c->p.sigma_f, c->st_env, c->nst, c->st_host[slot]);
+      if (st) return st;
+      HIPCHK(c, hipMemcpyAsync((void *)c->sa.coef, c->st_host[slot], sizeof(double) * 2 * c->nst,
+                               hipMemcpyHostToDevice, c->stream));
+      HIPCHK(c, hipEventRecord(c->st_ev[slot], c->stream));
+      st = qgw::launch_stoch(a, c->sa);
+    }
+    if (!st)
+      st = qgw::launch_forcing(a, f4 ? c->p.nl : 1, stoch ? c->st_S : nullptr, sqrt(c->dt),
+                               f4 ? c->forc4[0] : nullptr, f4 ? c->forc4[1] : nullptr, w1, w2, c->dt);
   }
   if (st || (st = check_launch(c, "forcing"))) return st;
   return QGW_OK;
@@ -487,7 +526,7 @@ int one_step(qgw_ctx *c) {
   ab3_coeffs(c->dt, c->dt0, c->dt1, cab);
   st = run_p4(c, cab, 1);
   if (st) return st;
-  if (c->nrec4 > 0 && (st = run_forcing(c, t_old))) return st;
+  if ((st = run_forcing(c, t_old))) return st;
   c->t = c->t + c->dt;
   c->it++;
   return QGW_OK;
@@ -514,7 +553,8 @@ int forcing_records(const qgw_params *p) {
 int check_params(const qgw_params *p, qgw_grid *grid, int *Lx, int *Ly) {
   if (p->nl < 1 || p->nl > qgw::kMaxLayers) return QGW_ERR_ARG;
   if (p->n_hyper < 0 || (p->n_hyper % 2) != 0 || p->n_hyper > 8) return QGW_ERR_ARG;
-  if (p->k_f != 0) return QGW_ERR_ARG;  // stochastic forcing not provided
+  // stochastic forcing: the reference's N x N field (N = NX) must be the grid's
+  if (p->k_f != 0 && (p->NX != p->NY || !(p->Lx > 0))) return QGW_ERR_ARG;
   if (p->dt_forc_period != 0 && forcing_records(p) < 1) return QGW_ERR_ARG;
   if (qgwh_grid(p, grid, nullptr, nullptr, nullptr, nullptr) != QGW_OK) return QGW_ERR_ARG;
   // transform lengths: the odd extension 2(n+1) (bounded) or the period N (periodic)
@@ -653,6 +693,78 @@ void member_energy(const qgw_ctx *c, int member, const double *h, double *E_laye
   if (Z_tot) *Z_tot = Zt / H;
 }
 
+// Stochastic ring forcing: the forced points grouped by column for the
+// synthesis kernels, the twiddles e^{+2 pi i t/N}, the random stream (seed 1:
+// the C library's rand() stream the reference draws from) and the buffers.
+int setup_stochastic(qgw_ctx *c, const qgw_params *p, size_t field) {
+  const int N = p->NX;
+  int np = 0;
+  double env = 0;
+  if (qgwh_stoch_ring(p, 0, nullptr, nullptr, &np, &env) != QGW_OK) return QGW_ERR_ARG;
+  c->nst = np;
+  c->st_env = env;
+  if (np <= 1) return QGW_OK;  // nothing is forced (src/forcing.h:170)
+  std::vector<int> ii(np), jj(np);
+  qgwh_stoch_ring(p, np, ii.data(), jj.data(), &np, &env);
+  // columns in increasing i, their points in draw order
+  std::vector<int> cols;
+  for (int i = 0; i <= N / 2; i++)
+    for (int k = 0; k < np; k++)
+      if (ii[k] == i) {
+        cols.push_back(i);
+        break;
+      }
+  const int nc = (int)cols.size();
+  std::vector<int> idx;  // col_i[nc], col_start[nc+1], mode_j[np], mode_k[np]
+  idx.insert(idx.end(), cols.begin(), cols.end());
+  std::vector<int> mj, mk;
+  idx.push_back(0);
+  for (int ci = 0; ci < nc; ci++) {
+    for (int k = 0; k < np; k++)
+      if (ii[k] == cols[ci]) {
+        mj.push_back(jj[k]);
+        mk.push_back(k);
+      }
+    idx.push_back((int)mj.size());
+  }
+  idx.insert(idx.end(), mj.begin(), mj.end());
+  idx.insert(idx.end(), mk.begin(), mk.end());
+  std::vector<double> tw(2 * (size_t)N);
+  for (int t = 0; t < N; t++) {
+    const double a = 2.0 * M_PI * (double)t / (double)N;
+    tw[2 * t] = cos(a);
+    tw[2 * t + 1] = sin(a);
+  }
+  void *d_idx = nullptr, *d_tw = nullptr, *d_coef = nullptr, *d_G = nullptr;
+  int st = dalloc(c, &d_idx, sizeof(int) * idx.size());
+  st = st ? st : dalloc(c, &d_tw, sizeof(double) * tw.size());
+  st = st ? st : dalloc(c, &d_coef, sizeof(double) * 2 * np);
+  st = st ? st : dalloc(c, &d_G, sizeof(double) * 2 * (size_t)N * nc);
+  st = st ? st : dalloc_field(c, &c->st_S, c->esz * (size_t)c->g.LS);
+  if (st) return st;
+  if (hipMemcpy(d_idx, idx.data(), sizeof(int) * idx.size(), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(d_tw, tw.data(), sizeof(double) * tw.size(), hipMemcpyHostToDevice) != hipSuccess)
+    return QGW_ERR_HIP;
+  const int *di = (const int *)d_idx;
+  c->sa.N = N;
+  c->sa.nc = nc;
+  c->sa.col_i = di;
+  c->sa.col_start = di + nc;
+  c->sa.mode_j = di + 2 * nc + 1;
+  c->sa.mode_k = di + 2 * nc + 1 + np;
+  c->sa.coef = (const double *)d_coef;
+  c->sa.tw = (const double *)d_tw;
+  c->sa.G = (double *)d_G;
+  c->sa.S = c->st_S;
+  for (int i = 0; i < 2; i++) {
+    if (hipHostMalloc((void **)&c->st_host[i], sizeof(double) * 2 * np, hipHostMallocDefault) != hipSuccess)
+      return QGW_ERR_NOMEM;
+    if (hipEventCreateWithFlags(&c->st_ev[i], hipEventDisableTiming) != hipSuccess) return QGW_ERR_HIP;
+  }
+  c->rng = qgwh_rng_create(1);
+  return c->rng ? QGW_OK : QGW_ERR_NOMEM;
+}
+
 // A group of nb members (ps[0..nb-1], all same_group, or nb = 1) on stream
 // `ext` (owned by the caller) or on a stream of its own (ext = null).
 qgw_ctx *create_group(const qgw_params *ps, int nb, int precision, hipStream_t ext, int *status) {
@@ -732,6 +844,10 @@ qgw_ctx *create_group(const qgw_params *ps, int nb, int precision, hipStream_t e
   c->forc_path = "input_vars.nc";  // src/forcing.h:262
   if (c->nrec4 > 0 && nb != 1) st = st ? st : QGW_ERR_ARG;  // one run per context
   for (int i = 0; i < 2 && c->nrec4 > 0; i++) st = st ? st : dalloc_field(c, &c->forc4[i], field);
+  if (!st && p->k_f != 0) {
+    if (nb != 1) st = QGW_ERR_ARG;
+    if (!st) st = setup_stochastic(c, p, field);
+  }
   if (!st) {
     if (precision == QGW_FP64) {
       st = make_twiddles<double>(c, Lx, &c->twx);
@@ -815,6 +931,7 @@ qgw_ctx *create_group(const qgw_params *ps, int nb, int precision, hipStream_t e
 // runs can be undone.
 struct Snapshot {
   void *buf = nullptr;
+  qgwh_rng *rng = nullptr;  // the forcing's random stream
   int qcur = 0;
   void *fa[3] = {nullptr, nullptr, nullptr};
   double t = 0, dt = 0, dt0 = 0, dt1 = 0;
@@ -833,6 +950,7 @@ int snapshot_take(qgw_ctx *c, Snapshot &s) {
   HIPCHK(c, hipMalloc(&s.buf, field * f.size()));
   for (size_t i = 0; i < f.size(); i++)
     HIPCHK(c, hipMemcpyAsync((char *)s.buf + i * field, f[i], field, hipMemcpyDeviceToDevice, c->stream));
+  if (c->rng && !(s.rng = qgwh_rng_clone(c->rng))) return QGW_ERR_NOMEM;
   s.qcur = c->qcur;
   for (int i = 0; i < 3; i++) s.fa[i] = c->fa[i];
   s.t = c->t;
@@ -852,6 +970,11 @@ int snapshot_restore(qgw_ctx *c, Snapshot &s) {
   c->dt0 = s.dt0;
   c->dt1 = s.dt1;
   c->it = s.it;
+  if (s.rng) {
+    qgwh_rng_assign(c->rng, s.rng);
+    qgwh_rng_destroy(s.rng);
+    s.rng = nullptr;
+  }
   auto f = state_fields(c);
   for (size_t i = 0; i < f.size(); i++)
     HIPCHK(c, hipMemcpyAsync(f[i], (char *)s.buf + i * field, field, hipMemcpyDeviceToDevice, c->stream));
@@ -1207,6 +1330,19 @@ int qgw_set_next_output(qgw_ctx *c, double t_out) {
   return QGW_OK;
 }
 
+int qgw_set_stoch_seed(qgw_ctx *c, unsigned seed) {
+  if (!c) return QGW_ERR_ARG;
+  if (!c->rng) {
+    snprintf(c->err, sizeof c->err, "no stochastic forcing in this run (k_f = 0 or at most one forced point)");
+    return QGW_ERR_STATE;
+  }
+  qgwh_rng *r = qgwh_rng_create(seed);
+  if (!r) return QGW_ERR_NOMEM;
+  qgwh_rng_destroy(c->rng);
+  c->rng = r;
+  return QGW_OK;
+}
+
 int qgw_set_forcing_file(qgw_ctx *c, const char *path) {
   if (!c || !path || !*path) return QGW_ERR_ARG;
   if (c->nrec4 <= 0) {
@@ -1343,8 +1479,8 @@ qgw_batch *qgw_batch_create(int nmembers, const qgw_params *params, int precisio
   *status = QGW_ERR_ARG;
   if (nmembers < 1 || !params) return nullptr;
   for (int i = 0; i < nmembers; i++)
-    if (!(params[i].dt_fixed > 0) || params[i].dt_forc_period != 0)
-      return nullptr;  // the graph step needs a fixed dt and no file forcing
+    if (!(params[i].dt_fixed > 0) || params[i].dt_forc_period != 0 || params[i].k_f != 0)
+      return nullptr;  // the graph step needs a fixed dt and no host-fed forcing
   qgw_batch *b = new qgw_batch();
   memset(b->err, 0, sizeof b->err);
   b->nmem = nmembers;
