@@ -21,7 +21,8 @@ kernels run on, over exactly --steps steps after --warmup untimed steps.  The
 line also carries the FP64 figure and the ratio, the energy/enstrophy error of
 the FP32 form against the FP64 form after --accuracy-steps steps, a roofline
 entry for the dominant kernel, the 96-member ensemble (uneven sizes, FP32,
-dt 1200 s) as a second throughput figure, and a CPU baseline (the FP64 oracle,
+dt 1200 s) as a second throughput figure, the forcing passes and the netCDF
+record write at the same workload ("rows"), and a CPU baseline (the FP64 oracle,
 a C restatement of the reference step, on the host cores, over a bounded
 sample).
 """
@@ -154,6 +155,80 @@ def ensemble(Q, configs, steps):
     return out
 
 
+def forcing_rows(Q, p, modes, q0, steps):
+    """The forcing rows at the same workload, one run per form and forcing:
+    the pass after the AB3 update (QGW_PASS_FORC) timed with per-pass events.
+    4-D file forcing: two records in a netCDF file, no record change inside the
+    timed steps (k_forcing only; algorithmic bytes per interior point.layer:
+    q read+write and two records, plus the compensation read+write in the
+    FP32 form).  Stochastic: a ring of radius 32 grid wavenumbers; the pass
+    holds the coefficient upload, the two synthesis kernels and k_forcing on
+    layer 0.  Also the netCDF write of one vars.nc record (psi and q, float)."""
+    import tempfile
+    from scipy.io import netcdf_file
+    g = Q.grid(p)[0]
+    interior = g.n * g.m * p.nl
+    out = {}
+    with tempfile.TemporaryDirectory() as td:
+        fpath = os.path.join(td, "input_vars.nc")
+        f = netcdf_file(fpath, "w")
+        f.createDimension("t", None)
+        f.createDimension("level", p.nl)
+        f.createDimension("y", g.nyp2)
+        f.createDimension("x", g.nxp2)
+        v = f.createVariable("q_forcing_3d", "f", ("t", "level", "y", "x"))
+        rng = np.random.default_rng(1)
+        for r in range(2):
+            v[r] = (1e-14 * rng.standard_normal((p.nl, g.nyp2, g.nxp2))).astype(np.float32)
+        f.close()
+        legs = {"forcing4d": dict(dt_forc=1e9, dt_forc_period=2e9),
+                "stochastic": dict(k_f=2 * np.pi * 32 / p.Lx, sigma_f=1e-12)}
+        for name, kw in legs.items():
+            pf = Q.default_params(**dict(p.as_dict(), **kw))
+            res = {}
+            for form, prec, words in (("fp64", Q.FP64, 4), ("fp32", Q.FP32, 6)):
+                m = Q.Model(pf, prec, modes)
+                m.set_q(q0)
+                if name == "forcing4d":
+                    m.set_forcing_file(fpath)
+                m.run(2)
+                m.synchronize()
+                t_step = m.time_run(steps) / steps
+                m.profile(True)
+                m.run(steps)
+                m.profile(False)
+                ms, cnt = m.profile_read()["FORC"]
+                m.close()
+                per = ms / max(cnt, 1)
+                r = {"ms_per_step": t_step, "forcing_ms_per_step": per}
+                if name == "forcing4d":
+                    alg = words * (8 if form == "fp64" else 4) * interior
+                    r.update({"kernel": "k_forcing", "algorithmic_bytes": alg,
+                              "GBs_algorithmic": alg / (per * 1e-3) / 1e9,
+                              "roofline_frac": alg / (per * 1e-3) / 1e9 / HBM_PEAK_GBS})
+                else:
+                    ii = Q.stoch_ring(pf)[0]
+                    r.update({"forced_points": int(len(ii)), "forced_columns": int(len(set(ii.tolist())))})
+                res[form] = r
+            out[name] = res
+        # one vars.nc record of the run (device -> host download, float conversion, write)
+        m = Q.Model(p, Q.FP64, modes)
+        m.set_q(q0)
+        m.invert_pv()
+        w = Q.NcWriter(os.path.join(td, "vars.nc"), p)
+        t0 = time.perf_counter()
+        psi, q = m.psi, m.q
+        t1 = time.perf_counter()
+        w.write(0.0, psi, q)
+        t2 = time.perf_counter()
+        w.close()
+        m.close()
+        nbytes = 2 * 4 * p.nl * g.nyp2 * g.nxp2
+        out["netcdf_record"] = {"bytes": nbytes, "download_s": t1 - t0, "write_s": t2 - t1,
+                                "write_MBs": nbytes / (t2 - t1) / 1e6, "file": "local temporary directory"}
+    return out
+
+
 def cpu_baseline(p, modes, q0, seconds):
     """The FP64 oracle (C restatement of the reference step) on all host cores."""
     import oracle as O
@@ -201,6 +276,8 @@ def main():
     ap.add_argument("--ensemble-steps", type=int, default=24)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--no-rows", action="store_true", help="skip the forcing and netCDF measurements")
+    ap.add_argument("--rows-steps", type=int, default=20)
     ap.add_argument("--forms", default="fp64,fp32")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (default tuning)")
     args = ap.parse_args()
@@ -295,6 +372,9 @@ def main():
     if not args.no_ensemble:
         log("ensemble leg")
         out["ensemble"] = ensemble(Q, configs, args.ensemble_steps)
+    if not args.no_rows:
+        log("forcing and netCDF rows")
+        out["rows"] = forcing_rows(Q, p, modes, q0, args.rows_steps)
     if not args.no_cpu:
         log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(p, modes, q0, args.cpu_seconds)

@@ -1455,13 +1455,16 @@ __global__ void __launch_bounds__(NB) k_diag_final(const double *part, int nblk,
 // ===========================================================================
 // Stochastic ring forcing field (calc_forc, src/forcing.h:168-208): the
 // unnormalised c2r of a sparse N x (N/2+1) half spectrum, synthesised
-// directly.  The nc forced columns are col_i[c] with their points
+// directly.  The nc forced columns have their points
 // mode_j/mode_k[col_start[c] .. col_start[c+1]) (row j, index into coef).
-//   G[j'][c] = sum_p coef[k_p] e^{+2 pi i j_p j'/N}            (k_stoch_cols)
-//   S[j'][i'] = sum_c w_c Re(G[j'][c] e^{+2 pi i col_i[c] i'/N})   (k_stoch_rows)
-// with w = 1 for the zero and Nyquist columns (only their real parts enter a
-// real inverse), 2 otherwise.  tw[t] = e^{+2 pi i t/N}.  S point (i', j') is
-// storage point (i'+1, j'+1) of layer 0; rows j' < m and columns i' < n.
+//   G[j'][c] = sum_p coef[k_p] e^{+2 pi i j_p j'/N}                (k_stoch_cols)
+//   S[j'][i'] = sum_c Re(G[j'][c] E[c][i'])                        (k_stoch_rows)
+// and k_stoch_rows adds S sqrt(dt) to q at once (src/timestep.h:143-150).
+// with E[c][i'] = w_c e^{+2 pi i col_i[c] i'/N}, w = 1 for the zero and
+// Nyquist columns (only their real parts enter a real inverse), 2 otherwise:
+// a real product of [Re G, -Im G] (m x 2nc) and [Re E; Im E] (2nc x n).
+// tw[t] = e^{+2 pi i t/N}.  S point (i', j') is storage point (i'+1, j'+1)
+// of layer 0 of q; rows j' < m and columns i' < n.
 // ===========================================================================
 template <typename T>  // the field type of the launch unit (the sums are FP64 either way)
 __global__ void __launch_bounds__(256)
@@ -1481,45 +1484,82 @@ k_stoch_cols(int N, int nc, const int *__restrict__ col_start, const int *__rest
   G[e] = make_double2(re, im);
 }
 
-template <typename T>
+// One block: 256 columns i' x SR rows j'.  The block's rows of G pass through
+// LDS 64 columns at a time; E streams from L2, one coalesced row per c, with
+// UE rows in flight before their products; q (and the compensation) of the
+// thread's points are fetched before the sums.
+// q += S s_fac on layer 0, Kahan-compensated when KAHAN.
+template <typename T, int SR, bool KAHAN>
 __global__ void __launch_bounds__(256)
-k_stoch_rows(Geo g, int N, int nc, const int *__restrict__ col_i, const double2 *__restrict__ G,
-             const double2 *__restrict__ tw, T *__restrict__ S) {
-  __shared__ double2 sg[256];
-  __shared__ int si[256];
-  const int jp = blockIdx.x;                     // output row j' < m
-  const int ip = blockIdx.y * 256 + threadIdx.x;  // output column i' < n
-  double acc = 0;
-  for (int c0 = 0; c0 < nc; c0 += 256) {
-    const int cn = min(256, nc - c0);
+k_stoch_rows(Geo g, int nc, const double2 *__restrict__ G, const double2 *__restrict__ E, double s_fac,
+             T *__restrict__ q, T *__restrict__ comp) {
+  constexpr int CC = 64, UE = 8;
+  __shared__ double2 sg[SR][CC + UE];  // zero past the chunk: the unrolled steps need no branch
+  const int j0 = blockIdx.x * SR;
+  const int ip = blockIdx.y * 256 + threadIdx.x;
+  const bool act = ip < g.n;
+  const int ipc = act ? ip : g.n - 1;  // clamped column: every lane loads, uniform control flow
+  const int rows = min(SR, g.m - j0);
+  T qv[SR], cv[SR];
+#pragma unroll
+  for (int r = 0; r < SR; r++) {
+    const long o = (long)(j0 + (r < rows ? r : 0) + 1) * g.P + ipc + 1;
+    qv[r] = q[o];
+    cv[r] = KAHAN ? comp[o] : T(0);
+  }
+  double acc[SR];
+#pragma unroll
+  for (int r = 0; r < SR; r++) acc[r] = 0;
+  for (int c0 = 0; c0 < nc; c0 += CC) {
+    const int cn = min(CC, nc - c0);
     __syncthreads();
-    if ((int)threadIdx.x < cn) {
-      sg[threadIdx.x] = G[(long)jp * nc + c0 + threadIdx.x];
-      si[threadIdx.x] = col_i[c0 + threadIdx.x];
+    for (int e = threadIdx.x; e < SR * (CC + UE); e += 256) {
+      const int r = e / (CC + UE), cc = e % (CC + UE);
+      sg[r][cc] = (cc < cn && r < rows) ? G[(long)(j0 + r) * nc + c0 + cc] : make_double2(0.0, 0.0);
     }
     __syncthreads();
-    if (ip < g.n)
-      for (int c = 0; c < cn; c++) {
-        const int ci = si[c];
-        const double2 w = tw[(int)(((long)ci * ip) % N)];
-        const double v = sg[c].x * w.x - sg[c].y * w.y;
-        acc += (ci == 0 || 2 * ci == N) ? v : 2 * v;
+    for (int cc = 0; cc < cn; cc += UE) {
+      double2 w[UE];
+#pragma unroll
+      for (int u = 0; u < UE; u++) w[u] = E[(long)(c0 + min(cc + u, cn - 1)) * g.n + ipc];
+      __builtin_amdgcn_sched_barrier(0);  // the UE loads stay ahead of their products
+#pragma unroll
+      for (int u = 0; u < UE; u++) {
+#pragma unroll
+        for (int r = 0; r < SR; r++) {
+          const double2 gv = sg[r][cc + u];
+          acc[r] = fma(gv.x, w[u].x, acc[r]);
+          acc[r] = fma(-gv.y, w[u].y, acc[r]);
+        }
       }
+    }
   }
-  if (ip < g.n) S[(long)(jp + 1) * g.P + ip + 1] = (T)acc;
+  if (!act) return;
+#pragma unroll
+  for (int r = 0; r < SR; r++) {
+    if (r >= rows) break;
+    const long o = (long)(j0 + r + 1) * g.P + ip + 1;
+    const T inc = (T)(acc[r] * s_fac);
+    if constexpr (KAHAN) {
+      const T y = inc - cv[r], t = qv[r] + y;
+      comp[o] = (t - qv[r]) - y;
+      q[o] = t;
+    } else {
+      q[o] = qv[r] + inc;
+    }
+  }
 }
 
 // ===========================================================================
-// Forcing added after the AB3 update (src/timestep.h:143-163): the stochastic
-// field S times s_fac = sqrt(dt) on layer 0, then the 4-D forcing
-// w1 F1 + w2 F2 (records around the time before the step) times dt, on the
+// 4-D forcing added after the AB3 update (src/timestep.h:152-163):
+// (w1 F1 + w2 F2) dt, records around the time before the step, on the
 // interior of every layer.  One block row per (layer, grid row); 256 columns
-// per block.  Either term may be absent (null pointer).
+// per block.
 // ===========================================================================
 template <typename T, bool KAHAN>
 __global__ void __launch_bounds__(256)
-k_forcing(Geo g, T *__restrict__ q, T *__restrict__ comp, const T *__restrict__ S, double s_fac,
-          const T *__restrict__ F1, const T *__restrict__ F2, double w1, double w2, double dt) {
+k_forcing(Geo g, T *__restrict__ q, T *__restrict__ comp, const T *__restrict__ F1, const T *__restrict__ F2,
+          double w1, double w2, double dt) {
   const int k = blockIdx.x / g.m, j = blockIdx.x % g.m + 1;
   const int i = blockIdx.y * 256 + threadIdx.x + 1;
   if (i > g.n) return;
@@ -1536,11 +1576,8 @@ k_forcing(Geo g, T *__restrict__ q, T *__restrict__ comp, const T *__restrict__ 
       v = v + inc;
     }
   };
-  if (S && k == 0) add((T)((double)S[o] * s_fac));
-  if (F1) {
-    const double qf = w1 * (double)F1[o] + w2 * (double)F2[o];
-    add((T)(qf * dt));
-  }
+  const double qf = w1 * (double)F1[o] + w2 * (double)F2[o];
+  add((T)(qf * dt));
   q[o] = v;
   if constexpr (KAHAN) comp[o] = c;
 }

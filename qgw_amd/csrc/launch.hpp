@@ -52,19 +52,18 @@ template <typename T> int launch_cfl_t(const LaunchArgs &a);
 template <typename T> int launch_diag_t(const LaunchArgs &a, double *part, int nblk, double *out);
 template <typename T> int launch_from_f64_t(const LaunchArgs &a, int layers, const double *src, void *dst, void *resid);
 template <typename T> int launch_to_f64_t(const LaunchArgs &a, int layers, const void *src, const void *resid, double *dst);
-// Stochastic forcing field into S (layer 0): k_stoch_cols then k_stoch_rows.
+// Stochastic forcing, q += S s_fac on layer 0 of a.q: k_stoch_cols then k_stoch_rows.
 struct StochArgs {
   int N = 0, nc = 0;
-  const int *col_i = nullptr, *col_start = nullptr, *mode_j = nullptr, *mode_k = nullptr;
+  const int *col_start = nullptr, *mode_j = nullptr, *mode_k = nullptr;
   const double *coef = nullptr, *tw = nullptr;
+  const double *E = nullptr;  // nc x n weighted twiddles of the forced columns
   double *G = nullptr;
-  void *S = nullptr;
 };
-template <typename T> int launch_stoch_t(const LaunchArgs &a, const StochArgs &s);
-// Post-AB3 forcing on a.q (k_forcing): layers = 1 when only S is given.
+template <typename T> int launch_stoch_t(const LaunchArgs &a, const StochArgs &s, double s_fac);
+// 4-D forcing on a.q (k_forcing), every layer.
 template <typename T>
-int launch_forcing_t(const LaunchArgs &a, int layers, const void *S, double s_fac, const void *F1, const void *F2,
-                     double w1, double w2, double dt);
+int launch_forcing_t(const LaunchArgs &a, const void *F1, const void *F2, double w1, double w2, double dt);
 bool launch_has_length(int L);
 bool launch_has_half_length(int M);  // lengths M = L/2 of the half-length transforms
 
@@ -82,9 +81,9 @@ bool launch_has_half_length(int M);  // lengths M = L/2 of the half-length trans
                                        void *resid);                                                  \
   template <> int launch_to_f64_t<T>(const LaunchArgs &a, int layers, const void *src, const void *resid, \
                                      double *dst);                                                        \
-  template <> int launch_forcing_t<T>(const LaunchArgs &a, int layers, const void *S, double s_fac,       \
-                                      const void *F1, const void *F2, double w1, double w2, double dt);  \
-  template <> int launch_stoch_t<T>(const LaunchArgs &a, const StochArgs &s);
+  template <> int launch_forcing_t<T>(const LaunchArgs &a, const void *F1, const void *F2, double w1,      \
+                                      double w2, double dt);                                              \
+  template <> int launch_stoch_t<T>(const LaunchArgs &a, const StochArgs &s, double s_fac);
 QGW_DECLARE_SPECS(float)
 QGW_DECLARE_SPECS(double)
 #undef QGW_DECLARE_SPECS
@@ -108,10 +107,11 @@ inline int launch_from_f64(const LaunchArgs &a, int layers, const double *src, v
 inline int launch_to_f64(const LaunchArgs &a, int layers, const void *src, const void *resid, double *dst) {
   return QGW_DISPATCH(launch_to_f64, a, layers, src, resid, dst);
 }
-inline int launch_stoch(const LaunchArgs &a, const StochArgs &s) { return QGW_DISPATCH(launch_stoch, a, s); }
-inline int launch_forcing(const LaunchArgs &a, int layers, const void *S, double s_fac, const void *F1,
-                          const void *F2, double w1, double w2, double dt) {
-  return QGW_DISPATCH(launch_forcing, a, layers, S, s_fac, F1, F2, w1, w2, dt);
+inline int launch_stoch(const LaunchArgs &a, const StochArgs &s, double s_fac) {
+  return QGW_DISPATCH(launch_stoch, a, s, s_fac);
+}
+inline int launch_forcing(const LaunchArgs &a, const void *F1, const void *F2, double w1, double w2, double dt) {
+  return QGW_DISPATCH(launch_forcing, a, F1, F2, w1, w2, dt);
 }
 
 }  // namespace qgw

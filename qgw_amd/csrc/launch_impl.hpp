@@ -292,29 +292,34 @@ int launch_from_f64_t<T>(const LaunchArgs &a, int layers, const double *src, voi
   return 0;
 }
 
-template <> int launch_stoch_t<T>(const LaunchArgs &a, const StochArgs &s) {
+template <> int launch_stoch_t<T>(const LaunchArgs &a, const StochArgs &s, double s_fac) {
   // the N x N field covers the m x n interior (N = n+1 bounded, n periodic)
   if (s.N < 2 || s.nc < 1 || a.g.n > s.N || a.g.m > s.N) return QGW_ERR_ARG;
   const long cells = (long)s.N * s.nc;
   hipLaunchKernelGGL((k_stoch_cols<T>), dim3((unsigned)((cells + 255) / 256)), dim3(256), 0, a.stream, s.N, s.nc,
                      s.col_start, s.mode_j, s.mode_k, (const double2 *)s.coef, (const double2 *)s.tw,
                      (double2 *)s.G);
-  hipLaunchKernelGGL((k_stoch_rows<T>), dim3((unsigned)a.g.m, (unsigned)((a.g.n + 255) / 256)), dim3(256), 0,
-                     a.stream, a.g, s.N, s.nc, s.col_i, (const double2 *)s.G, (const double2 *)s.tw, (T *)s.S);
+  constexpr int SR = 8;
+  const dim3 grid((unsigned)((a.g.m + SR - 1) / SR), (unsigned)((a.g.n + 255) / 256));
+  if (sizeof(T) == 4 && a.comp)
+    hipLaunchKernelGGL((k_stoch_rows<T, SR, true>), grid, dim3(256), 0, a.stream, a.g, s.nc, (const double2 *)s.G,
+                       (const double2 *)s.E, s_fac, (T *)a.q, (T *)a.comp);
+  else
+    hipLaunchKernelGGL((k_stoch_rows<T, SR, false>), grid, dim3(256), 0, a.stream, a.g, s.nc, (const double2 *)s.G,
+                       (const double2 *)s.E, s_fac, (T *)a.q, (T *)nullptr);
   return 0;
 }
 
 template <>
-int launch_forcing_t<T>(const LaunchArgs &a, int layers, const void *S, double s_fac, const void *F1, const void *F2,
-                        double w1, double w2, double dt) {
-  if (layers < 1 || layers > a.g.nl || a.g.n < 1 || a.g.m < 1) return QGW_ERR_ARG;
-  const dim3 grid((unsigned)(a.g.m * layers), (unsigned)((a.g.n + 255) / 256));
+int launch_forcing_t<T>(const LaunchArgs &a, const void *F1, const void *F2, double w1, double w2, double dt) {
+  if (!F1 || !F2 || a.g.n < 1 || a.g.m < 1) return QGW_ERR_ARG;
+  const dim3 grid((unsigned)(a.g.m * a.g.nl), (unsigned)((a.g.n + 255) / 256));
   if (sizeof(T) == 4 && a.comp)
     hipLaunchKernelGGL((k_forcing<T, true>), grid, dim3(256), 0, a.stream, a.g, (T *)a.q, (T *)a.comp,
-                       (const T *)S, s_fac, (const T *)F1, (const T *)F2, w1, w2, dt);
+                       (const T *)F1, (const T *)F2, w1, w2, dt);
   else
     hipLaunchKernelGGL((k_forcing<T, false>), grid, dim3(256), 0, a.stream, a.g, (T *)a.q, (T *)nullptr,
-                       (const T *)S, s_fac, (const T *)F1, (const T *)F2, w1, w2, dt);
+                       (const T *)F1, (const T *)F2, w1, w2, dt);
   return 0;
 }
 

@@ -80,12 +80,11 @@ struct qgw_ctx {
   std::string forc_path;
   // stochastic ring forcing (init_stoch_forc/calc_forc, src/forcing.h:67-208):
   // nst forced points (coefficients drawn on the host from rng each step,
-  // synthesised into st_S on the device); nothing is added when nst <= 1
+  // synthesised and added on the device); nothing is added when nst <= 1
   int nst;
   double st_env;
   qgwh_rng *rng;
   qgw::StochArgs sa;
-  void *st_S;
   double *st_host[2];  // pinned staging of the coefficients, alternating
   hipEvent_t st_ev[2];
   int st_slot;
@@ -488,11 +487,9 @@ int run_forcing(qgw_ctx *c, double t_old) {
       HIPCHK(c, hipMemcpyAsync((void *)c->sa.coef, c->st_host[slot], sizeof(double) * 2 * c->nst,
                                hipMemcpyHostToDevice, c->stream));
       HIPCHK(c, hipEventRecord(c->st_ev[slot], c->stream));
-      st = qgw::launch_stoch(a, c->sa);
+      st = qgw::launch_stoch(a, c->sa, sqrt(c->dt));
     }
-    if (!st)
-      st = qgw::launch_forcing(a, f4 ? c->p.nl : 1, stoch ? c->st_S : nullptr, sqrt(c->dt),
-                               f4 ? c->forc4[0] : nullptr, f4 ? c->forc4[1] : nullptr, w1, w2, c->dt);
+    if (!st && f4) st = qgw::launch_forcing(a, c->forc4[0], c->forc4[1], w1, w2, c->dt);
   }
   if (st || (st = check_launch(c, "forcing"))) return st;
   return QGW_OK;
@@ -696,7 +693,7 @@ void member_energy(const qgw_ctx *c, int member, const double *h, double *E_laye
 // Stochastic ring forcing: the forced points grouped by column for the
 // synthesis kernels, the twiddles e^{+2 pi i t/N}, the random stream (seed 1:
 // the C library's rand() stream the reference draws from) and the buffers.
-int setup_stochastic(qgw_ctx *c, const qgw_params *p, size_t field) {
+int setup_stochastic(qgw_ctx *c, const qgw_params *p) {
   const int N = p->NX;
   int np = 0;
   double env = 0;
@@ -735,27 +732,38 @@ int setup_stochastic(qgw_ctx *c, const qgw_params *p, size_t field) {
     tw[2 * t] = cos(a);
     tw[2 * t + 1] = sin(a);
   }
-  void *d_idx = nullptr, *d_tw = nullptr, *d_coef = nullptr, *d_G = nullptr;
+  // E[c][i'] = w_c e^{+2 pi i col_i[c] i'/N} over the n interior columns
+  const int n = c->g.n;
+  std::vector<double> E(2 * (size_t)nc * n);
+  for (int ci = 0; ci < nc; ci++) {
+    const double w = (cols[ci] == 0 || 2 * cols[ci] == N) ? 1.0 : 2.0;
+    for (int i = 0; i < n; i++) {
+      const size_t t = ((size_t)cols[ci] * i) % N, o = 2 * ((size_t)ci * n + i);
+      E[o] = w * tw[2 * t];
+      E[o + 1] = w * tw[2 * t + 1];
+    }
+  }
+  void *d_idx = nullptr, *d_tw = nullptr, *d_coef = nullptr, *d_G = nullptr, *d_E = nullptr;
   int st = dalloc(c, &d_idx, sizeof(int) * idx.size());
   st = st ? st : dalloc(c, &d_tw, sizeof(double) * tw.size());
+  st = st ? st : dalloc(c, &d_E, sizeof(double) * E.size());
   st = st ? st : dalloc(c, &d_coef, sizeof(double) * 2 * np);
   st = st ? st : dalloc(c, &d_G, sizeof(double) * 2 * (size_t)N * nc);
-  st = st ? st : dalloc_field(c, &c->st_S, c->esz * (size_t)c->g.LS);
   if (st) return st;
   if (hipMemcpy(d_idx, idx.data(), sizeof(int) * idx.size(), hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(d_tw, tw.data(), sizeof(double) * tw.size(), hipMemcpyHostToDevice) != hipSuccess)
+      hipMemcpy(d_tw, tw.data(), sizeof(double) * tw.size(), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(d_E, E.data(), sizeof(double) * E.size(), hipMemcpyHostToDevice) != hipSuccess)
     return QGW_ERR_HIP;
   const int *di = (const int *)d_idx;
   c->sa.N = N;
   c->sa.nc = nc;
-  c->sa.col_i = di;
   c->sa.col_start = di + nc;
   c->sa.mode_j = di + 2 * nc + 1;
   c->sa.mode_k = di + 2 * nc + 1 + np;
   c->sa.coef = (const double *)d_coef;
   c->sa.tw = (const double *)d_tw;
+  c->sa.E = (const double *)d_E;
   c->sa.G = (double *)d_G;
-  c->sa.S = c->st_S;
   for (int i = 0; i < 2; i++) {
     if (hipHostMalloc((void **)&c->st_host[i], sizeof(double) * 2 * np, hipHostMallocDefault) != hipSuccess)
       return QGW_ERR_NOMEM;
@@ -846,7 +854,7 @@ qgw_ctx *create_group(const qgw_params *ps, int nb, int precision, hipStream_t e
   for (int i = 0; i < 2 && c->nrec4 > 0; i++) st = st ? st : dalloc_field(c, &c->forc4[i], field);
   if (!st && p->k_f != 0) {
     if (nb != 1) st = QGW_ERR_ARG;
-    if (!st) st = setup_stochastic(c, p, field);
+    if (!st) st = setup_stochastic(c, p);
   }
   if (!st) {
     if (precision == QGW_FP64) {
