@@ -108,13 +108,14 @@ k_p1(Geo g, const T *__restrict__ q, T *__restrict__ what, const cx<T> *__restri
 // cm2l[l,k] w_k, src/elliptic.h:140-147).  Both projections commute with the
 // sine transforms, so they sit where the data is transposed anyway.
 // ===========================================================================
-template <typename T, int TC, int TR, int NLC>
+template <typename T, int TC, int TR, int NLC, int KB>
 __global__ void __launch_bounds__(256)
 k_tproj(const T *__restrict__ src, T *__restrict__ dst, const T *__restrict__ M, int nl, int R, int C, int Ps,
         long Ss, int Pd, long Sd) {
   static_assert(TC * 8 == 256 && TR % 8 == 0, "tile shape");
   constexpr int U = TR / 8;  // source rows per thread
-  __shared__ T tile[TR][TC + 1];
+  // KB output modes per barrier pair: their tiles are formed together, then stored together
+  __shared__ T tile[KB][TR][TC + 1];
   // member blockIdx.z: its nl layers and its own nl x nl matrix
   src += (long)blockIdx.z * nl * Ss;
   dst += (long)blockIdx.z * nl * Sd;
@@ -134,21 +135,31 @@ k_tproj(const T *__restrict__ src, T *__restrict__ dst, const T *__restrict__ M,
   // store mapping: lanes along r (contiguous in dst)
   const int rl = tid % TR, cg = tid / TR;
   constexpr int CS = 256 / TR;  // columns per store pass
-  for (int k = 0; k < nl; k++) {
+  for (int k0 = 0; k0 < nl; k0 += KB) {
 #pragma unroll
-    for (int u = 0; u < U; u++) {
-      T acc = T(0);
+    for (int kb = 0; kb < KB; kb++) {
+      const int k = k0 + kb;
+      if (k >= nl) break;  // uniform
 #pragma unroll
-      for (int l = 0; l < NLC; l++)
-        if (l < nl) acc += M[k * nl + l] * v[l][u];  // uniform address: scalar loads
-      tile[rg + 8 * u][ci] = acc;
+      for (int u = 0; u < U; u++) {
+        T acc = T(0);
+#pragma unroll
+        for (int l = 0; l < NLC; l++)
+          if (l < nl) acc += M[k * nl + l] * v[l][u];  // uniform address: scalar loads
+        tile[kb][rg + 8 * u][ci] = acc;
+      }
     }
     __syncthreads();
     const int r = r0 + rl;
 #pragma unroll
-    for (int cc = cg; cc < TC; cc += CS) {
-      const int c = c0 + cc;
-      if (r < R && c < C) dst[k * Sd + (long)c * Pd + r] = tile[rl][cc];
+    for (int kb = 0; kb < KB; kb++) {
+      const int k = k0 + kb;
+      if (k >= nl) break;
+#pragma unroll
+      for (int cc = cg; cc < TC; cc += CS) {
+        const int c = c0 + cc;
+        if (r < R && c < C) dst[k * Sd + (long)c * Pd + r] = tile[kb][rl][cc];
+      }
     }
     __syncthreads();
   }

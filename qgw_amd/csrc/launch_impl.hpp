@@ -160,12 +160,13 @@ template <> int launch_p2_t<T>(const LaunchArgs &a) {
 // (contiguous in the destination): 128-byte segments on both sides.
 constexpr int kTC = 32;
 constexpr int kTR = (sizeof(T) == 8) ? 16 : 32;
+constexpr int kTKB = 4;  // modes per barrier pair
 
 template <int NLC>
 static void tproj_go(const LaunchArgs &a, const void *src, void *dst, const void *M, int R, int C, int Ps, long Ss,
                      int Pd, long Sd) {
   const dim3 grid((C + kTC - 1) / kTC, (R + kTR - 1) / kTR, a.g.nl / a.g.nlm);
-  hipLaunchKernelGGL((k_tproj<T, kTC, kTR, NLC>), grid, dim3(256), 0, a.stream, (const T *)src, (T *)dst,
+  hipLaunchKernelGGL((k_tproj<T, kTC, kTR, NLC, kTKB>), grid, dim3(256), 0, a.stream, (const T *)src, (T *)dst,
                      (const T *)M, a.g.nlm, R, C, Ps, Ss, Pd, Sd);
 }
 

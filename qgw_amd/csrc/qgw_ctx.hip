@@ -225,8 +225,14 @@ int upload_typed(qgw_ctx *c, void *dst, const double *src, size_t cnt) {
   return c->prec == QGW_FP64 ? upload_vec<double>(c, dst, src, cnt) : upload_vec<float>(c, dst, src, cnt);
 }
 
-// Storage pitch: n+2 rounded up to 64 elements.
-int pitch_of(int w) { return (w + 63) / 64 * 64; }
+// Storage pitch: n+2 rounded up to 64 elements, and 64 more when that is a
+// multiple of 1024 elements, so that the rows a tile touches at once do not
+// all start at the same offset modulo a large power of two (HBM channel
+// interleave).
+int pitch_of(int w) {
+  const int p = (w + 63) / 64 * 64;
+  return (p % 1024 == 0) ? p + 64 : p;
+}
 
 void release(qgw_ctx *c) {
   if (!c) return;
