@@ -378,8 +378,12 @@ k_p1h(Geo g, const T *__restrict__ q, T *__restrict__ what, const cx<T> *__restr
   }
 }
 
+// FP32 with 8 elements per thread: four wavefronts per SIMD (128 registers;
+// the transform pair is issue-bound, and the fourth wavefront hides more of
+// its LDS and load latency than the few spilled registers cost).
 template <typename T, int M, int EP>
 __global__ void __launch_bounds__((DstHalf<M, EP>::NT))
+__attribute__((amdgpu_waves_per_eu((DstHalf<M, EP>::E == 8) ? (sizeof(T) == 4 ? 4 : 3) : 1)))
 k_p2h(Geo g, T *__restrict__ whatT, const double *__restrict__ La2, const double *__restrict__ Kb2,
       const double *__restrict__ iRd2, double norm, const cx<T> *__restrict__ twh, const cx<T> *__restrict__ tw2) {
   constexpr int E = DstHalf<M, EP>::E, NT = DstHalf<M, EP>::NT;
