@@ -160,7 +160,7 @@ template <> int launch_p2_t<T>(const LaunchArgs &a) {
 // (contiguous in the destination): 128-byte segments on both sides.
 constexpr int kTC = 32;
 constexpr int kTR = (sizeof(T) == 8) ? 16 : 32;
-constexpr int kTKB = 4;  // modes per barrier pair
+constexpr int kTKB = 1;  // modes per barrier pair
 
 template <int NLC>
 static void tproj_go(const LaunchArgs &a, const void *src, void *dst, const void *M, int R, int C, int Ps, long Ss,
@@ -188,6 +188,10 @@ static int tproj(const LaunchArgs &a, const void *src, void *dst, const void *M,
                          (float *)dst, (const float *)M, nl, R, C, Ps, Ss, Pd, Sd, ntc, ntr);
   } else if (nl <= 4) {
     tproj_go<4>(a, src, dst, M, R, C, Ps, Ss, Pd, Sd);
+  } else if (nl <= 8) {
+    tproj_go<8>(a, src, dst, M, R, C, Ps, Ss, Pd, Sd);
+  } else if (nl <= 12) {
+    tproj_go<12>(a, src, dst, M, R, C, Ps, Ss, Pd, Sd);
   } else if (nl <= 16) {
     tproj_go<16>(a, src, dst, M, R, C, Ps, Ss, Pd, Sd);
   } else {
