@@ -31,33 +31,45 @@ template <int M, int EP = 0> struct DstHalf {
   static_assert(KPT >= 1 && M % (2 * NT) == 0, "post-pass split");
 };
 
+// DPP permutation of a double (two 32-bit moves); lanes with no source lane,
+// and rows outside ROWS, receive 0.
+template <int CTRL, int ROWS> __device__ __forceinline__ double dpp_d(double v) {
+  const long long b = __builtin_bit_cast(long long, v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffLL), CTRL, ROWS, 0xF, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROWS, 0xF, true);
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
+}
+// Inclusive sum over the 64 lanes of a wavefront: row_shr 1, 2, 4, 8 within
+// rows of 16 lanes, then row_bcast:15 (rows 1, 3) and row_bcast:31 (rows 2, 3).
+__device__ __forceinline__ double wave_incl_scan(double s) {
+  s += dpp_d<0x111, 0xF>(s);
+  s += dpp_d<0x112, 0xF>(s);
+  s += dpp_d<0x114, 0xF>(s);
+  s += dpp_d<0x118, 0xF>(s);
+  s += dpp_d<0x142, 0xA>(s);
+  s += dpp_d<0x143, 0xC>(s);
+  return s;
+}
 // Exclusive prefix sum over the block of one complex value per thread
-// (the real and imaginary parts are independent sums).
-template <typename T, int NT> __device__ __forceinline__ cx<T> block_excl_scan(cx<T> v, int t) {
-  constexpr int WW = (NT < 64) ? NT : 64;  // threads per wavefront in use
+// (the real and imaginary parts are independent sums).  The DPP moves carry
+// values from lower to higher lanes only, so a partial last wavefront (NT not
+// a multiple of 64) sums correctly in its active lanes.
+template <int NT> __device__ __forceinline__ cx<double> block_excl_scan(cx<double> v, int t) {
   constexpr int NW = (NT + 63) / 64;
-  __shared__ cx<T> wtot[NW];
+  __shared__ cx<double> wtot[NW];
   const int lane = t & 63, w = t >> 6;
-  cx<T> s = v;
-#pragma unroll
-  for (int d = 1; d < WW; d <<= 1) {
-    const T x = __shfl_up(s.x, d, 64);
-    const T y = __shfl_up(s.y, d, 64);
-    if (lane >= d) {
-      s.x += x;
-      s.y += y;
-    }
-  }
+  const double sx = wave_incl_scan(v.x), sy = wave_incl_scan(v.y);
+  cx<double> e = mkc<double>(dpp_d<0x138, 0xF>(sx), dpp_d<0x138, 0xF>(sy));  // wave_shr:1: lane i - 1's sum
   if constexpr (NW > 1) {
-    if (lane == 63) wtot[w] = s;
+    if (lane == 63) wtot[w] = mkc<double>(sx, sy);
     __syncthreads();
-    cx<T> off = mkc<T>(T(0), T(0));
+    cx<double> off = mkc<double>(0.0, 0.0);
 #pragma unroll
     for (int i = 0; i < NW - 1; i++)
-      if (i < w) off = cadd<T>(off, wtot[i]);
-    s = cadd<T>(s, off);
+      if (i < w) off = cadd<double>(off, wtot[i]);
+    e = cadd<double>(off, e);
   }
-  return csub<T>(s, v);
+  return e;
 }
 
 // Pre pass: h[m] = y_p at the home positions p = t + m NT.  ld(j) returns
@@ -102,7 +114,7 @@ template <typename T, int M, int EP> __device__ __forceinline__ void dst_post(cx
     run = cadd<double>(run, r);
     cum[i] = run;
   }
-  const cx<double> off = block_excl_scan<double, NT>(run, t);
+  const cx<double> off = block_excl_scan<NT>(run, t);
   __syncthreads();  // every read of Y is done
 #pragma unroll
   for (int i = 0; i < KPT; i++) {

@@ -94,20 +94,29 @@ template <typename T, int R> __device__ __forceinline__ void bfly(cx<T> *v) {
     Bfly<T, R>::run(v);
 }
 
-// Twiddle the R inputs of one butterfly: v[r] *= W^(r k), W = tw[step] (index
-// into the size-L table).  Powers are built from at most log2(R) table loads.
-template <typename T, int R, int L>
-__device__ __forceinline__ void twiddle(cx<T> *v, const cx<T> *__restrict__ tw, int base) {
-  // base = k * L/(Ns*R): the table index of W^1
+// Twiddles of one butterfly: v[r] *= W^(r k), W = tw[base] (index into the
+// size-L table).  The loads fetch W^(2^b), b < log2 R; the other powers are
+// products.  Loads and products are split so that a stage's table loads can
+// be issued a stage early, ahead of the LDS exchange that precedes their use,
+// instead of stalling the stage on an L2 round trip after its barrier.
+template <int R> constexpr int tw_loads() { return R == 2 ? 1 : (R == 4 ? 2 : (R == 8 ? 3 : 4)); }
+
+template <typename T, int R>
+__device__ __forceinline__ void twiddle_load(cx<T> *wl, const cx<T> *__restrict__ tw, int base) {
+#pragma unroll
+  for (int b = 0; b < tw_loads<R>(); b++) wl[b] = tw[base << b];
+}
+
+template <typename T, int R> __device__ __forceinline__ void twiddle_apply(cx<T> *v, const cx<T> *wl) {
   if constexpr (R == 2) {
-    v[1] = cmul<T>(v[1], tw[base]);
+    v[1] = cmul<T>(v[1], wl[0]);
   } else if constexpr (R == 4) {
-    const cx<T> w1 = tw[base], w2 = tw[2 * base];
+    const cx<T> w1 = wl[0], w2 = wl[1];
     v[1] = cmul<T>(v[1], w1);
     v[2] = cmul<T>(v[2], w2);
     v[3] = cmul<T>(v[3], cmul<T>(w1, w2));
   } else if constexpr (R == 8) {
-    const cx<T> w1 = tw[base], w2 = tw[2 * base], w4 = tw[4 * base];
+    const cx<T> w1 = wl[0], w2 = wl[1], w4 = wl[2];
     const cx<T> w3 = cmul<T>(w1, w2);
     v[1] = cmul<T>(v[1], w1);
     v[2] = cmul<T>(v[2], w2);
@@ -117,7 +126,7 @@ __device__ __forceinline__ void twiddle(cx<T> *v, const cx<T> *__restrict__ tw, 
     v[6] = cmul<T>(v[6], cmul<T>(w2, w4));
     v[7] = cmul<T>(v[7], cmul<T>(w3, w4));
   } else if constexpr (R == 16) {
-    const cx<T> w1 = tw[base], w2 = tw[2 * base], w4 = tw[4 * base], w8 = tw[8 * base];
+    const cx<T> w1 = wl[0], w2 = wl[1], w4 = wl[2], w8 = wl[3];
     const cx<T> w3 = cmul<T>(w1, w2), w5 = cmul<T>(w1, w4), w6 = cmul<T>(w2, w4);
     const cx<T> w7 = cmul<T>(w3, w4);
     v[1] = cmul<T>(v[1], w1);
@@ -138,25 +147,43 @@ __device__ __forceinline__ void twiddle(cx<T> *v, const cx<T> *__restrict__ tw, 
   }
 }
 
+// Table loads of every twiddle stage S of the plan needs (none for S = 0):
+// butterfly w of the thread uses wl[w * log2(R) ...], W = tw[k L/(Ns R)] with
+// k = (t + w NT) mod Ns.  At most E/2 entries.
+template <typename T, int L, int S, int EP>
+__device__ __forceinline__ void stage_tw_load(cx<T> *wl, const cx<T> *__restrict__ tw, int t) {
+  using Pl = RegPlan<L, EP>;
+  constexpr int R = Pl::radix(S), Ns = Pl::ns(S), NB = Pl::E / R, LG = tw_loads<R>();
+  if constexpr (Ns > 1) {
+#pragma unroll
+    for (int w = 0; w < NB; w++) twiddle_load<T, R>(wl + w * LG, tw, ((t + w * Pl::NT) % Ns) * (L / (Ns * R)));
+  }
+}
+
 // Stage S of the power-of-two plan.  On entry v[] holds this stage's inputs:
-// v[w*R + r] = X[j_w + r*L/R], j_w = t + w*NT.  On exit (S not last) v[] holds
-// the next stage's inputs; for the last stage v[w*R + r] is the output at
-// position t + (w + r*E/R)*NT.
+// v[w*R + r] = X[j_w + r*L/R], j_w = t + w*NT, and wcur this stage's twiddle
+// loads (stage_tw_load).  On exit (S not last) v[] holds the next stage's
+// inputs; for the last stage v[w*R + r] is the output at position
+// t + (w + r*E/R)*NT.  The next stage's twiddle loads are issued before this
+// stage's butterflies.
 template <typename T, int L, int S, int EP = 0>
-__device__ __forceinline__ void reg_stage(cx<T> *v, cx<T> *buf, const cx<T> *__restrict__ tw, int t) {
+__device__ __forceinline__ void reg_stage(cx<T> *v, cx<T> *buf, const cx<T> *__restrict__ tw, int t,
+                                          const cx<T> *wcur) {
   using Pl = RegPlan<L, EP>;
   constexpr int E = Pl::E, NT = Pl::NT;
   constexpr int R = Pl::radix(S);
   constexpr int Ns = Pl::ns(S);
   constexpr int NB = E / R;  // butterflies per thread
+  constexpr int LG = tw_loads<R>();
+  constexpr bool more = S + 1 < Pl::count();
+  cx<T> wnext[E / 2];
+  if constexpr (more) stage_tw_load<T, L, S + 1, EP>(wnext, tw, t);
 #pragma unroll
   for (int w = 0; w < NB; w++) {
-    const int j = t + w * NT;
-    const int k = j % Ns;
-    if constexpr (Ns > 1) twiddle<T, R, L>(v + w * R, tw, k * (L / (Ns * R)));
+    if constexpr (Ns > 1) twiddle_apply<T, R>(v + w * R, wcur + w * LG);
     bfly<T, R>(v + w * R);
   }
-  if constexpr (S + 1 < Pl::count()) {
+  if constexpr (more) {
     constexpr int R2 = Pl::radix(S + 1);
     constexpr int NB2 = E / R2;
     __syncthreads();  // previous readers of buf are done
@@ -174,7 +201,7 @@ __device__ __forceinline__ void reg_stage(cx<T> *v, cx<T> *buf, const cx<T> *__r
 #pragma unroll
       for (int r = 0; r < R2; r++) v[w * R2 + r] = buf[rslot(j + r * (L / R2))];
     }
-    reg_stage<T, L, S + 1, EP>(v, buf, tw, t);
+    reg_stage<T, L, S + 1, EP>(v, buf, tw, t, wnext);
   }
 }
 
@@ -201,7 +228,7 @@ __device__ __forceinline__ void fft_home(cx<T> *h, cx<T> *buf, const cx<T> *__re
     cx<T> v[E];
 #pragma unroll
     for (int s = 0; s < E; s++) v[s] = h[in_m<L, EP>(s)];
-    reg_stage<T, L, 0, EP>(v, buf, tw, t);
+    reg_stage<T, L, 0, EP>(v, buf, tw, t, v);  // stage 0 has no twiddles
 #pragma unroll
     for (int s = 0; s < E; s++) h[out_m<L, EP>(s)] = v[s];
   } else {

@@ -415,10 +415,11 @@ k_p2h(Geo g, T *__restrict__ whatT, const double *__restrict__ La2, const double
       f0 = (T)(1.0 / (-(d0 + kb + ir0) * norm));
       f1 = (T)(1.0 / (-(d1 + kb + ir1) * norm));
     } else {
-      // FP32 forms: FP64 only for the lowest barotropic wavenumbers (mode 0, a, b <= 16)
+      // FP32 forms: FP64 only for the lowest barotropic wavenumbers (mode 0, a, b <= 16);
+      // elsewhere the hardware reciprocal (1 ulp), well inside the transforms' FP32 rounding
       const T kbf = (T)kb;
-      f0 = T(1) / (-(d0f + kbf) * normf);
-      f1 = T(1) / (-(d1f + kbf) * normf);
+      f0 = -__builtin_amdgcn_rcpf((d0f + kbf) * normf);
+      f1 = -__builtin_amdgcn_rcpf((d1f + kbf) * normf);
       if (b <= 16) {
         if (lo0) f0 = (T)(1.0 / (-(d0 + kb + ir0) * norm));
         if (lo1) f1 = (T)(1.0 / (-(d1 + kb + ir1) * norm));
@@ -1109,10 +1110,10 @@ __device__ __forceinline__ void p4s_walk(const Geo &g, const T *__restrict__ P, 
                                          int y0, int y1, bool ringstrip) {
   constexpr int H = (NLAP > 1) ? NLAP : 1;       // halo, and the delay of the output row
   constexpr int NP = H + 2;                      // psi rows r, r-1, ..., r-H-1
-  constexpr int D = 2;                           // prefetch depth (iterations)
-  constexpr int RING = D + 1;                    // prefetch ring slots; 3 or 6 divide W
-  constexpr int W = (NP <= 6) ? 6 : 12;          // unroll and psi window length (a multiple of 3 and RING)
-  static_assert(W % RING == 0 && W % 3 == 0, "unroll");
+  constexpr int D = 3;                           // prefetch depth (iterations)
+  constexpr int W = (NP <= 6) ? 6 : 12;          // unroll and psi window length (a multiple of 3)
+  constexpr int RING = W;                        // prefetch slots: row r uses slot (r - rbeg) mod W
+  static_assert(D < RING && W % 3 == 0, "unroll");
   constexpr int NL = (NLAP > 1) ? NLAP - 1 : 1;  // stored levels 1..NLAP-1
   static_assert(NP <= W, "psi window");
   const int n = g.n, m = g.m;
@@ -1133,7 +1134,7 @@ __device__ __forceinline__ void p4s_walk(const Geo &g, const T *__restrict__ P, 
   T lv[NL][3];                                        // level a: rows r-a, r-a-1, r-a-2 (circular)
   T jpc[3], jpl[3], jpr[3], jqc[3], jql[3], jqr[3];  // rows y+1, y, y-1 of psi and q (circular)
   T jtc[3], jtl[3], jtr[3];                           // topography rows (bottom layer)
-  T bp[RING], bq[RING], bt[RING], b1[RING], b2[RING], bc[RING], bw[RING];  // loads of r .. r+D (slot r mod RING)
+  T bp[RING], bq[RING], bt[RING], b1[RING], b2[RING], bc[RING], bw[RING];  // loads of r .. r+D (D + 1 slots live)
 #pragma unroll
   for (int i = 0; i < W; i++) pw[i] = nbw[i] = T(0);
 #pragma unroll
