@@ -562,7 +562,7 @@ class Model:
         """The tuning choices, by name."""
         return {"p4": P4_TILES[self.tune_get(TUNE_P4_TILE)], "p4_waves": self.tune_get(TUNE_P4_WAVES),
                 "t_kernel": ["auto", "valu", "mfma32", "mfma16"][self.tune_get(TUNE_T_KERNEL)],
-                "dst": ["odd-extension", "half-length"][self.tune_get(TUNE_DST)],
+                "dst": ["odd-extension", "half-length", "half-length pipelined"][self.tune_get(TUNE_DST)],
                 "fft_e": self.tune_get(TUNE_FFT_E)}
 
     def autotune(self, reps=5):
@@ -718,7 +718,7 @@ class Batch:
         return {"p4": P4_TILES[self.tune_get(group, TUNE_P4_TILE)],
                 "p4_waves": self.tune_get(group, TUNE_P4_WAVES),
                 "t_kernel": ["auto", "valu", "mfma32", "mfma16"][self.tune_get(group, TUNE_T_KERNEL)],
-                "dst": ["odd-extension", "half-length"][self.tune_get(group, TUNE_DST)],
+                "dst": ["odd-extension", "half-length", "half-length pipelined"][self.tune_get(group, TUNE_DST)],
                 "fft_e": self.tune_get(group, TUNE_FFT_E)}
 
     def profile(self, on=True):

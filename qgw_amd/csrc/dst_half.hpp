@@ -75,8 +75,9 @@ template <int NT> __device__ __forceinline__ cx<double> block_excl_scan(cx<doubl
 // Pre pass: h[m] = y_p at the home positions p = t + m NT.  ld(j) returns
 // z_j = (a_j, b_j) for j = 1..N (the caller's row access); z_0 = z_M = 0.
 // tw2: the length-2M table e^{-2 pi i k / 2M}, so sin(pi p / M) = -Im tw2[p].
-template <typename T, int M, int EP, class Ld>
-__device__ __forceinline__ void dst_pre(cx<T> *h, int t, const cx<T> *__restrict__ tw2, Ld ld) {
+// sn(p) returns sin(pi p / M).
+template <typename T, int M, int EP, class Sn, class Ld>
+__device__ __forceinline__ void dst_pre_s(cx<T> *h, int t, const Sn &sn, Ld ld) {
   constexpr int E = DstHalf<M, EP>::E, NT = DstHalf<M, EP>::NT;
 #pragma unroll
   for (int m = 0; m < E; m++) {
@@ -85,12 +86,24 @@ __device__ __forceinline__ void dst_pre(cx<T> *h, int t, const cx<T> *__restrict
       h[m] = mkc<T>(T(0), T(0));
     } else {
       const cx<T> z = ld(p), zm = ld(M - p);
-      const T s = -tw2[p].y;
+      const T s = sn(p);
       const T hx = T(0.5) * (z.x - zm.x), hy = T(0.5) * (z.y - zm.y);
       h[m] = mkc<T>(s * (z.x + zm.x) + hx, s * (z.y + zm.y) + hy);
     }
   }
 }
+template <typename T, int M, int EP, class Ld>
+__device__ __forceinline__ void dst_pre(cx<T> *h, int t, const cx<T> *__restrict__ tw2, Ld ld) {
+  dst_pre_s<T, M, EP>(h, t, [&](int p) { return -tw2[p].y; }, ld);
+}
+// sin(pi p / M) = -Im e^{-2 pi i p / 2M} from the two-level table of length 2M
+template <typename T, int M> struct SinLds {
+  const TwTab<T, 2 * M> *tb;
+  __device__ __forceinline__ T operator()(int p) const {
+    const cx<T> a = tb->hi[p >> 6], b = tb->lo[p & 63];
+    return -(a.x * b.y + a.y * b.x);
+  }
+};
 
 // Post pass: from Y at the home positions to S^a + i S^b in buf, natural
 // order (entry k-1 holds S_k, k = 1..N, at rslot(k-1)).  Ends with a barrier.

@@ -101,10 +101,36 @@ template <typename T, int R> __device__ __forceinline__ void bfly(cx<T> *v) {
 // instead of stalling the stage on an L2 round trip after its barrier.
 template <int R> constexpr int tw_loads() { return R == 2 ? 1 : (R == 4 ? 2 : (R == 8 ? 3 : 4)); }
 
-template <typename T, int R>
-__device__ __forceinline__ void twiddle_load(cx<T> *wl, const cx<T> *__restrict__ tw, int base) {
+// Twiddle table accessors: entry i of the size-L table e^{-2 pi i k / L}.
+template <typename T> struct TwGlobal {  // the table in global memory
+  const cx<T> *__restrict__ p;
+  __device__ __forceinline__ cx<T> operator()(int i) const { return p[i]; }
+};
+// Two-level table in LDS: W^i = hi[i / 64] * lo[i % 64], hi[h] = W^(64 h),
+// lo[l] = W^l (L >= 64, a power of two; exact where either factor is 1).
+template <typename T, int L> struct TwTab {
+  static_assert(L >= 64 && (L & (L - 1)) == 0, "two-level table");
+  static constexpr int NHI = L / 64;
+  cx<T> lo[64], hi[NHI];
+};
+template <typename T, int L>
+__device__ __forceinline__ void twtab_fill(TwTab<T, L> &tb, const cx<T> *__restrict__ tw, int t, int nt) {
+  for (int i = t; i < 64 + TwTab<T, L>::NHI; i += nt) {
+    if (i < 64)
+      tb.lo[i] = tw[i];
+    else
+      tb.hi[i - 64] = tw[(i - 64) * 64];
+  }
+}
+template <typename T, int L> struct TwLds {
+  const TwTab<T, L> *tb;
+  __device__ __forceinline__ cx<T> operator()(int i) const { return cmul<T>(tb->hi[i >> 6], tb->lo[i & 63]); }
+};
+
+template <typename T, int R, class TW>
+__device__ __forceinline__ void twiddle_load(cx<T> *wl, const TW &tw, int base) {
 #pragma unroll
-  for (int b = 0; b < tw_loads<R>(); b++) wl[b] = tw[base << b];
+  for (int b = 0; b < tw_loads<R>(); b++) wl[b] = tw(base << b);
 }
 
 template <typename T, int R> __device__ __forceinline__ void twiddle_apply(cx<T> *v, const cx<T> *wl) {
@@ -150,8 +176,8 @@ template <typename T, int R> __device__ __forceinline__ void twiddle_apply(cx<T>
 // Table loads of every twiddle stage S of the plan needs (none for S = 0):
 // butterfly w of the thread uses wl[w * log2(R) ...], W = tw[k L/(Ns R)] with
 // k = (t + w NT) mod Ns.  At most E/2 entries.
-template <typename T, int L, int S, int EP>
-__device__ __forceinline__ void stage_tw_load(cx<T> *wl, const cx<T> *__restrict__ tw, int t) {
+template <typename T, int L, int S, int EP, class TW>
+__device__ __forceinline__ void stage_tw_load(cx<T> *wl, const TW &tw, int t) {
   using Pl = RegPlan<L, EP>;
   constexpr int R = Pl::radix(S), Ns = Pl::ns(S), NB = Pl::E / R, LG = tw_loads<R>();
   if constexpr (Ns > 1) {
@@ -166,9 +192,8 @@ __device__ __forceinline__ void stage_tw_load(cx<T> *wl, const cx<T> *__restrict
 // inputs; for the last stage v[w*R + r] is the output at position
 // t + (w + r*E/R)*NT.  The next stage's twiddle loads are issued before this
 // stage's butterflies.
-template <typename T, int L, int S, int EP = 0>
-__device__ __forceinline__ void reg_stage(cx<T> *v, cx<T> *buf, const cx<T> *__restrict__ tw, int t,
-                                          const cx<T> *wcur) {
+template <typename T, int L, int S, int EP, class TW>
+__device__ __forceinline__ void reg_stage(cx<T> *v, cx<T> *buf, const TW &tw, int t, const cx<T> *wcur) {
   using Pl = RegPlan<L, EP>;
   constexpr int E = Pl::E, NT = Pl::NT;
   constexpr int R = Pl::radix(S);
@@ -201,7 +226,7 @@ __device__ __forceinline__ void reg_stage(cx<T> *v, cx<T> *buf, const cx<T> *__r
 #pragma unroll
       for (int r = 0; r < R2; r++) v[w * R2 + r] = buf[rslot(j + r * (L / R2))];
     }
-    reg_stage<T, L, S + 1, EP>(v, buf, tw, t, wnext);
+    reg_stage<T, L, S + 1, EP, TW>(v, buf, tw, t, wnext);
   }
 }
 
@@ -221,16 +246,24 @@ template <int L, int EP = 0> constexpr int out_m(int slot) {
 // Forward FFT of the sequence held at the home positions.  h[m] is the value
 // at position t + m*NT, in and out.  buf: RegPlan<L>::slots entries (powers of
 // two) or lds_slots(L) (other lengths).
+// Power-of-two lengths with any table accessor (TwGlobal, TwLds).
+template <typename T, int L, int EP, class TW>
+__device__ __forceinline__ void fft_home_tw(cx<T> *h, cx<T> *buf, const TW &tw, int t) {
+  static_assert(is_pow2(L), "register plan");
+  constexpr int E = RegPlan<L, EP>::E;
+  cx<T> v[E];
+#pragma unroll
+  for (int s = 0; s < E; s++) v[s] = h[in_m<L, EP>(s)];
+  reg_stage<T, L, 0, EP, TW>(v, buf, tw, t, v);  // stage 0 has no twiddles
+#pragma unroll
+  for (int s = 0; s < E; s++) h[out_m<L, EP>(s)] = v[s];
+}
+
 template <typename T, int L, int EP = 0>
 __device__ __forceinline__ void fft_home(cx<T> *h, cx<T> *buf, const cx<T> *__restrict__ tw, int t) {
   constexpr int E = RegPlan<L, EP>::E, NT = RegPlan<L, EP>::NT;
   if constexpr (is_pow2(L)) {
-    cx<T> v[E];
-#pragma unroll
-    for (int s = 0; s < E; s++) v[s] = h[in_m<L, EP>(s)];
-    reg_stage<T, L, 0, EP>(v, buf, tw, t, v);  // stage 0 has no twiddles
-#pragma unroll
-    for (int s = 0; s < E; s++) h[out_m<L, EP>(s)] = v[s];
+    fft_home_tw<T, L, EP>(h, buf, TwGlobal<T>{tw}, t);
   } else {
     __syncthreads();
 #pragma unroll

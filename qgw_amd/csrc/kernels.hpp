@@ -488,6 +488,167 @@ k_p3h(Geo g, const T *__restrict__ what, T *__restrict__ psi, T *__restrict__ q,
 }
 
 // ===========================================================================
+// Pipelined half-length row transforms (power-of-two M): the same outputs as
+// k_p1h / k_p3h from persistent blocks.  A block walks the items
+// it = blockIdx.x, blockIdx.x + gridDim.x, ... (a row and layer pair).  The
+// rows of item it + gridDim.x are loaded into registers while item it is
+// transformed; the twiddles and the pre-pass sines come from two-level LDS
+// tables filled once per block, so the global traffic inside the loop is the
+// row loads and stores alone and no wait on a table load also waits on the
+// next item's rows.  Each item's row pair is staged in LDS (entry x - 1 holds
+// z_x), from where the pre pass reads z_p and z_{M-p}.  (The column pass P2
+// stays one block per item: its two transforms fill the register file, and
+// the prefetch registers would cost it a wavefront per SIMD.)
+// The thread index passes through an empty asm at the top of each item, so
+// that the index arithmetic of the item is formed inside the loop instead of
+// being hoisted out of it into registers held across all items.
+// ===========================================================================
+__device__ __forceinline__ int loop_tid(int t) {
+  asm volatile("" : "+v"(t));
+  return t;
+}
+template <typename T, int M, int EP> struct PipeRows {
+  static constexpr int E = DstHalf<M, EP>::E, NT = DstHalf<M, EP>::NT;
+  T za[E], zb[E];
+  // z_p (p = t + m NT >= 1) of two rows: element p - 1 of r0 and r1 (r1 is a
+  // valid row when has1 is false); unconditional loads, zeros selected after
+  __device__ __forceinline__ void fetch(const T *__restrict__ r0, const T *__restrict__ r1, bool has1, int t) {
+#pragma unroll
+    for (int m = 0; m < E; m++) {
+      const int p = t + m * NT;
+      const int e = (p >= 1) ? p - 1 : 0;
+      const T a = r0[e], b = r1[e];
+      za[m] = (p >= 1) ? a : T(0);
+      zb[m] = (p >= 1 && has1) ? b : T(0);
+    }
+  }
+  // publish the fetched rows in buf (entry p - 1 holds z_p); the caller's
+  // barrier before this call retires the previous item's readers of buf
+  __device__ __forceinline__ void stage(cx<T> *buf, int t) const {
+#pragma unroll
+    for (int m = 0; m < E; m++) {
+      const int p = t + m * NT;
+      if (p >= 1) buf[rslot(p - 1)] = mkc<T>(za[m], zb[m]);
+    }
+  }
+};
+
+template <typename T, int M, int EP>
+__global__ void __launch_bounds__((DstHalf<M, EP>::NT))
+k_p1hp(Geo g, const T *__restrict__ q, T *__restrict__ what, const cx<T> *__restrict__ twh,
+       const cx<T> *__restrict__ tw2) {
+  constexpr int E = DstHalf<M, EP>::E, NT = DstHalf<M, EP>::NT;
+  __shared__ cx<T> buf[DstHalf<M, EP>::slots];
+  __shared__ TwTab<T, M> tb;
+  __shared__ TwTab<T, 2 * M> tb2;
+  const int t = threadIdx.x;
+  const int npair = (g.nl + 1) / 2;
+  const int nitems = g.m * npair;
+  twtab_fill(tb, twh, t, NT);
+  twtab_fill(tb2, tw2, t, NT);
+  const TwLds<T, M> twa{&tb};
+  const SinLds<T, M> sna{&tb2};
+  PipeRows<T, M, EP> rows;
+  // storage column x = 1..n of row j holds x_1..x_N
+  auto fetch = [&](int it, int tf) {
+    const int lp = it % npair, j = it / npair + 1, l0 = 2 * lp;
+    const bool has1 = l0 + 1 < g.nl;
+    const T *r0 = q + l0 * g.LS + (long)j * g.P + 1;
+    const T *r1 = q + (has1 ? l0 + 1 : l0) * g.LS + (long)j * g.P + 1;
+    rows.fetch(r0, r1, has1, tf);
+  };
+  int it = blockIdx.x;
+  if (it < nitems) fetch(it, t);
+  for (; it < nitems; it += gridDim.x) {
+    const int tl = loop_tid(t);
+    __syncthreads();
+    rows.stage(buf, tl);
+    __syncthreads();
+    if (it + (int)gridDim.x < nitems) fetch(it + gridDim.x, tl);
+    cx<T> h[E];
+    dst_pre_s<T, M, EP>(h, tl, sna, [&](int x) { return buf[rslot(x - 1)]; });
+    fft_home_tw<T, M, EP>(h, buf, twa, tl);
+    dst_post<T, M, EP>(h, buf, tl);
+    const int lp = it % npair, j = it / npair + 1, l0 = 2 * lp, l1 = l0 + 1;
+    const bool has1 = l1 < g.nl;
+    T *o0 = what + l0 * g.WS + (long)(j - 1) * g.Pn;
+    T *o1 = what + l1 * g.WS + (long)(j - 1) * g.Pn;
+    for (int i = tl; i < M - 1; i += NT) {
+      const cx<T> v = buf[rslot(i)];
+      o0[i] = v.x;
+      if (has1) o1[i] = v.y;
+    }
+  }
+}
+
+template <typename T, int M, int EP>
+__global__ void __launch_bounds__((DstHalf<M, EP>::NT))
+k_p3hp(Geo g, const T *__restrict__ what, T *__restrict__ psi, T *__restrict__ q, T ring_c,
+       const cx<T> *__restrict__ twh, const cx<T> *__restrict__ tw2) {
+  constexpr int E = DstHalf<M, EP>::E, NT = DstHalf<M, EP>::NT;
+  __shared__ cx<T> buf[DstHalf<M, EP>::slots];
+  __shared__ TwTab<T, M> tb;
+  __shared__ TwTab<T, 2 * M> tb2;
+  const int t = threadIdx.x;
+  const int n = g.n, m_ = g.m;
+  const int npair = (g.nl + 1) / 2;
+  const int nitems = g.m * npair;
+  twtab_fill(tb, twh, t, NT);
+  twtab_fill(tb2, tw2, t, NT);
+  const TwLds<T, M> twa{&tb};
+  const SinLds<T, M> sna{&tb2};
+  PipeRows<T, M, EP> rows;
+  auto fetch = [&](int it, int tf) {
+    const int lp = it % npair, j = it / npair + 1, l0 = 2 * lp;
+    const bool has1 = l0 + 1 < g.nl;
+    const T *i0 = what + l0 * g.WS + (long)(j - 1) * g.Pn;
+    const T *i1 = what + (has1 ? l0 + 1 : l0) * g.WS + (long)(j - 1) * g.Pn;
+    rows.fetch(i0, i1, has1, tf);
+  };
+  int it = blockIdx.x;
+  if (it < nitems) fetch(it, t);
+  for (; it < nitems; it += gridDim.x) {
+    const int tl = loop_tid(t);
+    __syncthreads();
+    rows.stage(buf, tl);
+    __syncthreads();
+    if (it + (int)gridDim.x < nitems) fetch(it + gridDim.x, tl);
+    cx<T> h[E];
+    dst_pre_s<T, M, EP>(h, tl, sna, [&](int x) { return buf[rslot(x - 1)]; });
+    fft_home_tw<T, M, EP>(h, buf, twa, tl);
+    dst_post<T, M, EP>(h, buf, tl);
+    const int lp = it % npair, j = it / npair + 1, l0 = 2 * lp, l1 = l0 + 1;
+    const bool has1 = l1 < g.nl;
+#pragma unroll
+    for (int side = 0; side < 2; side++) {
+      const int l = side ? l1 : l0;
+      if (side && !has1) break;
+      T *prow = psi + l * g.LS + (long)j * g.P;
+      T *qr = q + l * g.LS;
+      for (int i = tl; i < n; i += NT) {
+        const int x = i + 1;
+        const cx<T> c = buf[rslot(i)];
+        const T v = side ? c.y : c.x;
+        prow[x] = v;
+        if (x == 1) qr[(long)j * g.P] = ring_c * v;
+        if (x == n) qr[(long)j * g.P + n + 1] = ring_c * v;
+        if (j == 1) qr[x] = ring_c * v;
+        if (j == m_) qr[(long)(m_ + 1) * g.P + x] = ring_c * v;
+      }
+      if (tl == 0) {
+        prow[0] = T(0);
+        prow[n + 1] = T(0);
+      }
+      if (j == 1 || j == m_) {
+        const long rr = (long)(j == 1 ? 0 : m_ + 1) * g.P;
+        T *pw = psi + l * g.LS + rr;
+        for (int x = tl; x < n + 2; x += NT) pw[x] = T(0);
+      }
+    }
+  }
+}
+
+// ===========================================================================
 // Periodic branch (bc_fac = -1, src/elliptic.h:32-47,103-118): the reference's
 // in-place r2c / c2r 2-D transform pair, as
 //   P1 (k_pf1): per (row, layer pair): FFT_N of a + i b, split into the two

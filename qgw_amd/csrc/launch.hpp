@@ -27,12 +27,13 @@ struct LaunchArgs {
   void *q = nullptr, *qn = nullptr, *psi = nullptr, *what = nullptr, *whatT = nullptr;
   void *cl2m = nullptr, *cm2l = nullptr, *twx = nullptr, *twy = nullptr;
   void *twxh = nullptr, *twyh = nullptr;  // length-(L/2) tables of the half-length transforms
-  int dst_half = 0;                       // 1: P1/P2/P3 through FFTs of length L/2
+  int dst_half = 0;                       // 1: P1/P2/P3 through FFTs of length L/2; 2: and P1/P3 pipelined (powers of two)
   int fft_e = 16;                         // elements per thread of those FFTs (8 or 16, lengths >= 1024)
   int per = 0;                            // periodic domain (bc_fac = -1): Lx = Ly = N, FFT transforms
   void *fnew = nullptr, *f1 = nullptr, *f2 = nullptr, *comp = nullptr, *topo = nullptr, *wind = nullptr;
   const double *La2 = nullptr, *Kb2 = nullptr, *iRd2 = nullptr;
   double norm = 1.0, ring_c = 0.0;
+  int cus = 256;    // compute units of the device (persistent grids)
   StepCoef cf{};
   int nlap = 0;
   int p4_tile = kP4TileCount;  // P4 variant: a tile index, or kP4TileCount for streaming

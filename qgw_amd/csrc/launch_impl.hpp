@@ -31,6 +31,56 @@ using T = QGW_T;
     QGW_GO(M, 0)                                          \
   }
 
+// Persistent grid of a pipelined kernel: its resident blocks on every compute
+// unit, at most one block per item.
+template <class K> static unsigned persist_grid(K kern, int nt, long nitems, int cus, int &occ) {
+  if (occ <= 0) {
+    int o = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, reinterpret_cast<const void *>(kern), nt, 0) != hipSuccess ||
+        o < 1)
+      o = 1;
+    occ = o;
+  }
+  const long g = (long)occ * (cus > 0 ? cus : 1);
+  return (unsigned)(g < nitems ? g : nitems);
+}
+
+// Half-length transforms of length M: for the row passes P1/P3 the pipelined
+// form for powers of two under dst_half == 2, else one block per item.
+template <int M, int EP> static void p1h_go(const LaunchArgs &a, dim3 grid) {
+  if constexpr (is_pow2(M)) {
+    if (a.dst_half == 2) {
+      static int occ = 0;
+      const unsigned pg = persist_grid(k_p1hp<T, M, EP>, DstHalf<M, EP>::NT, (long)grid.x, a.cus, occ);
+      hipLaunchKernelGGL((k_p1hp<T, M, EP>), dim3(pg), dim3(DstHalf<M, EP>::NT), 0, a.stream, a.g, (const T *)a.q,
+                         (T *)a.what, (const cx<T> *)a.twxh, (const cx<T> *)a.twx);
+      return;
+    }
+  }
+  hipLaunchKernelGGL((k_p1h<T, M, EP>), grid, dim3(DstHalf<M, EP>::NT), 0, a.stream, a.g, (const T *)a.q,
+                     (T *)a.what, (const cx<T> *)a.twxh, (const cx<T> *)a.twx);
+}
+
+template <int M, int EP> static void p3h_go(const LaunchArgs &a, dim3 grid) {
+  if constexpr (is_pow2(M)) {
+    if (a.dst_half == 2) {
+      static int occ = 0;
+      const unsigned pg = persist_grid(k_p3hp<T, M, EP>, DstHalf<M, EP>::NT, (long)grid.x, a.cus, occ);
+      hipLaunchKernelGGL((k_p3hp<T, M, EP>), dim3(pg), dim3(DstHalf<M, EP>::NT), 0, a.stream, a.g,
+                         (const T *)a.what, (T *)a.psi, (T *)a.q, (T)a.ring_c, (const cx<T> *)a.twxh,
+                         (const cx<T> *)a.twx);
+      return;
+    }
+  }
+  hipLaunchKernelGGL((k_p3h<T, M, EP>), grid, dim3(DstHalf<M, EP>::NT), 0, a.stream, a.g, (const T *)a.what,
+                     (T *)a.psi, (T *)a.q, (T)a.ring_c, (const cx<T> *)a.twxh, (const cx<T> *)a.twx);
+}
+
+template <int M, int EP> static void p2h_go(const LaunchArgs &a, dim3 grid) {
+  hipLaunchKernelGGL((k_p2h<T, M, EP>), grid, dim3(DstHalf<M, EP>::NT), 0, a.stream, a.g, (T *)a.whatT, a.La2,
+                     a.Kb2, a.iRd2, a.norm, (const cx<T> *)a.twyh, (const cx<T> *)a.twy);
+}
+
 template <> int launch_p1_t<T>(const LaunchArgs &a) {
   const dim3 grid(a.g.m * ((a.g.nl + 1) / 2));
   if (a.per) {
@@ -47,9 +97,7 @@ template <> int launch_p1_t<T>(const LaunchArgs &a) {
   }
   if (a.dst_half) {
     switch (a.Lx / 2) {
-#define QGW_GO(M, EP)                                                                                  \
-  hipLaunchKernelGGL((k_p1h<T, M, EP>), grid, dim3(DstHalf<M, EP>::NT), 0, a.stream, a.g, (const T *)a.q, \
-                     (T *)a.what, (const cx<T> *)a.twxh, (const cx<T> *)a.twx);
+#define QGW_GO(M, EP) p1h_go<M, EP>(a, grid);
 #define QGW_CASE(M) \
   case M:           \
     QGW_HALF_EP(M)  \
@@ -88,9 +136,7 @@ template <> int launch_p3_t<T>(const LaunchArgs &a) {
   }
   if (a.dst_half) {
     switch (a.Lx / 2) {
-#define QGW_GO(M, EP)                                                                                    \
-  hipLaunchKernelGGL((k_p3h<T, M, EP>), grid, dim3(DstHalf<M, EP>::NT), 0, a.stream, a.g, (const T *)a.what, \
-                     (T *)a.psi, (T *)a.q, (T)a.ring_c, (const cx<T> *)a.twxh, (const cx<T> *)a.twx);
+#define QGW_GO(M, EP) p3h_go<M, EP>(a, grid);
 #define QGW_CASE(M) \
   case M:           \
     QGW_HALF_EP(M)  \
@@ -131,9 +177,7 @@ template <> int launch_p2_t<T>(const LaunchArgs &a) {
   }
   if (a.dst_half) {
     switch (a.Ly / 2) {
-#define QGW_GO(M, EP)                                                                                \
-  hipLaunchKernelGGL((k_p2h<T, M, EP>), grid, dim3(DstHalf<M, EP>::NT), 0, a.stream, a.g, (T *)a.whatT, \
-                     a.La2, a.Kb2, a.iRd2, a.norm, (const cx<T> *)a.twyh, (const cx<T> *)a.twy);
+#define QGW_GO(M, EP) p2h_go<M, EP>(a, grid);
 #define QGW_CASE(M) \
   case M:           \
     QGW_HALF_EP(M)  \

@@ -64,7 +64,8 @@ struct qgw_ctx {
   int p4_tile;
   int t_kernel;  // projection kernel choice (QGW_TUNE_T_KERNEL)
   int p4_waves;  // streaming P4 wavefronts per SIMD (QGW_TUNE_P4_WAVES)
-  int dst_half;  // half-length sine transforms (QGW_TUNE_DST); -1: lengths not provided
+  int dst_half;  // sine transforms (QGW_TUNE_DST: 0, 1 half-length, 2 pipelined rows); -1: half lengths not provided
+  int cus;       // compute units of the device
   int fft_e;     // elements per thread of the half-length FFTs (QGW_TUNE_FFT_E)
   int per;       // periodic domain (bc_fac = -1)
   bool modes_set;
@@ -279,8 +280,9 @@ int run_invert(qgw_ctx *c) {
   a.t_kernel = c->t_kernel;
   a.twxh = c->twxh;
   a.twyh = c->twyh;
-  a.dst_half = c->dst_half > 0;
+  a.dst_half = c->dst_half > 0 ? c->dst_half : 0;
   a.fft_e = c->fft_e;
+  a.cus = c->cus;
   a.per = c->per;
   // normalisation of the transform pair (src/elliptic.h:36,51)
   a.norm = c->per ? (double)c->g.n * (double)c->g.m : 4.0 * (double)(c->g.n + 1) * (double)(c->g.m + 1);
@@ -832,6 +834,15 @@ qgw_ctx *create_group(const qgw_params *ps, int nb, int precision, hipStream_t e
       return nullptr;
     }
   }
+  {
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) {
+      release(c);
+      return nullptr;
+    }
+    c->cus = cus;
+  }
   const size_t field = c->esz * (size_t)g.LS * NL;
   int st = QGW_OK;
   st = st ? st : dalloc_field(c, &c->q[0], field);
@@ -930,7 +941,7 @@ qgw_ctx *create_group(const qgw_params *ps, int nb, int precision, hipStream_t e
   c->p4_waves = 32;
   c->fft_e = (precision == QGW_FP64) ? 16 : 8;
   c->per = per ? 1 : 0;
-  c->dst_half = (!per && qgw::launch_has_half_length(Lx / 2) && qgw::launch_has_half_length(Ly / 2)) ? 1 : -1;
+  c->dst_half = (!per && qgw::launch_has_half_length(Lx / 2) && qgw::launch_has_half_length(Ly / 2)) ? 2 : -1;
   c->modes_set = false;
   // the zero fills above ran on the null stream; the step runs on c->stream
   if (hipDeviceSynchronize() != hipSuccess) {
@@ -1071,11 +1082,12 @@ int autotune_group(qgw_ctx *c, int reps, int *p4_tile, double *best_ms) {
     c->t_kernel = targ;
   }
   if (!st && c->dst_half >= 0) {
-    // sine transforms: odd-extension FFTs, or half-length FFTs with each radix schedule
+    // sine transforms: odd-extension FFTs, or half-length FFTs (one block per
+    // item, or the row passes in pipelined persistent blocks) with each radix schedule
     double bt = 1e300;
     int darg = c->dst_half, earg = c->fft_e;
-    const int forms[3][2] = {{0, 16}, {1, 16}, {1, 8}};
-    for (int v = 0; v < 3 && !st; v++) {
+    const int forms[5][2] = {{0, 16}, {1, 16}, {1, 8}, {2, 16}, {2, 8}};
+    for (int v = 0; v < 5 && !st; v++) {
       c->dst_half = forms[v][0];
       c->fft_e = forms[v][1];
       double ms = 0;
@@ -1429,7 +1441,7 @@ int qgw_tune_set(qgw_ctx *c, int knob, int value) {
     return QGW_OK;
   }
   if (knob == QGW_TUNE_DST) {
-    if (value < 0 || value > 1 || (value == 1 && c->dst_half < 0)) return QGW_ERR_ARG;
+    if (value < 0 || value > 2 || (value >= 1 && c->dst_half < 0)) return QGW_ERR_ARG;
     if (c->dst_half >= 0) c->dst_half = value;
     return QGW_OK;
   }
@@ -1461,7 +1473,7 @@ int qgw_tune_get(qgw_ctx *c, int knob, int *value) {
     return QGW_OK;
   }
   if (knob == QGW_TUNE_DST) {
-    *value = c->dst_half > 0 ? 1 : 0;
+    *value = c->dst_half > 0 ? c->dst_half : 0;
     return QGW_OK;
   }
   if (knob == QGW_TUNE_FFT_E) {

@@ -251,15 +251,18 @@ def test_p4_stream_paths(prec, hyper):
 
 @pytest.mark.parametrize("shape", [(64, 64, 3), (96, 128, 2), (256, 192, 5), (384, 256, 2)])
 def test_dst_half_vs_full(shape):
-    """Sine transforms through half-length FFTs against the odd-extension
-    FFTs (FP64, 6 steps, and the oracle), on power-of-two and 3*2^k grids."""
+    """Sine transforms through half-length FFTs (one block per row pair, and
+    pipelined persistent blocks) against the odd-extension FFTs (FP64, 6
+    steps, and the oracle), on power-of-two and 3*2^k grids; a 3*2^k length
+    runs the one-block form under the pipelined setting, so the mixed grids
+    pair both kinds of kernels in one step."""
     NX, NY, nl = shape
     p = _small(NX, nl, NY=NY)
     _announce("dst half vs full", shape)
     modes = Q.eigmode(p)
     q0 = Q.make_ic(p, modes)[1]
     out = {}
-    for form in (0, 1):
+    for form in (0, 1, 2):
         m = Q.Model(p, Q.FP64, modes)
         m.tune_set(Q.TUNE_DST, form)
         assert m.tune_get(Q.TUNE_DST) == form
@@ -273,21 +276,24 @@ def test_dst_half_vs_full(shape):
     o.set_q(q0)
     o.run(6)
     o.invert_pv()
-    for form in (0, 1):
+    for form in (0, 1, 2):
         assert rel(out[form][0], o.field(O.Q)) < 1e-12, form
         assert rel(out[form][1], o.field(O.PSI)) < 1e-12, form
     assert rel(out[1][1], out[0][1]) < 1e-13
+    assert rel(out[2][1], out[0][1]) < 1e-13
 
 
+@pytest.mark.parametrize("form", [1, 2])
 @pytest.mark.parametrize("fft_e", [8, 16])
-def test_dst_half_radix_schedules(fft_e):
-    """Both radix schedules of the half-length FFTs (1024-point rows) against the oracle."""
+def test_dst_half_radix_schedules(fft_e, form):
+    """Both radix schedules of the half-length FFTs (1024-point rows), one
+    block per row pair and pipelined, against the oracle."""
     p = _small(1024, 2, NY=1024, Lx=4000e3)
-    _announce("dst half radix schedule", fft_e)
+    _announce("dst half radix schedule", fft_e, "form", form)
     modes = Q.eigmode(p)
     q0 = Q.make_ic(p, modes)[1]
     m = Q.Model(p, Q.FP64, modes)
-    m.tune_set(Q.TUNE_DST, 1)
+    m.tune_set(Q.TUNE_DST, form)
     m.tune_set(Q.TUNE_FFT_E, fft_e)
     m.set_q(q0)
     m.run(4)
