@@ -47,8 +47,8 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
 PASS_WORDS = {"P1": 2, "T1": 2, "P2": 2, "T2": 2, "P3": 2, "P4": 6}
 # Kernel name prefixes of each pass, in order of preference (the profile holds
 # whichever form ran).
-PASS_KERNELS = {"P1": ("k_p1h<", "k_p1<"), "T1": ("k_tproj_mfma<", "k_tproj<"), "P2": ("k_p2h<", "k_p2<"),
-                "T2": ("k_tproj_mfma<", "k_tproj<"), "P3": ("k_p3h<", "k_p3<"), "P4": ("k_p4s<", "k_p4<")}
+PASS_KERNELS = {"P1": ("k_p1hp<", "k_p1h<", "k_p1<"), "T1": ("k_tproj_mfma<", "k_tproj<"), "P2": ("k_p2h<", "k_p2<"),
+                "T2": ("k_tproj_mfma<", "k_tproj<"), "P3": ("k_p3hp<", "k_p3h<", "k_p3<"), "P4": ("k_p4s<", "k_p4<")}
 
 
 def log(*a):
