@@ -501,10 +501,12 @@ k_p3h(Geo g, const T *__restrict__ what, T *__restrict__ psi, T *__restrict__ q,
 // the prefetch registers would cost it a wavefront per SIMD.)
 // The thread index passes through an empty asm at the top of each item, so
 // that the index arithmetic of the item is formed inside the loop instead of
-// being hoisted out of it into registers held across all items.
+// being hoisted out of it into registers held across all items; its range
+// 0 <= t < NT is restated after the asm.
 // ===========================================================================
-__device__ __forceinline__ int loop_tid(int t) {
+template <int NT> __device__ __forceinline__ int loop_tid(int t) {
   asm volatile("" : "+v"(t));
+  __builtin_assume(t >= 0 && t < NT);  // the range survives, for the shifts and trip counts
   return t;
 }
 template <typename T, int M, int EP> struct PipeRows {
@@ -560,7 +562,7 @@ k_p1hp(Geo g, const T *__restrict__ q, T *__restrict__ what, const cx<T> *__rest
   int it = blockIdx.x;
   if (it < nitems) fetch(it, t);
   for (; it < nitems; it += gridDim.x) {
-    const int tl = loop_tid(t);
+    const int tl = loop_tid<NT>(t);
     __syncthreads();
     rows.stage(buf, tl);
     __syncthreads();
@@ -608,7 +610,7 @@ k_p3hp(Geo g, const T *__restrict__ what, T *__restrict__ psi, T *__restrict__ q
   int it = blockIdx.x;
   if (it < nitems) fetch(it, t);
   for (; it < nitems; it += gridDim.x) {
-    const int tl = loop_tid(t);
+    const int tl = loop_tid<NT>(t);
     __syncthreads();
     rows.stage(buf, tl);
     __syncthreads();
