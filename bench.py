@@ -250,6 +250,44 @@ def cpu_baseline(p, modes, q0, seconds):
                       "(gcc -O3, OpenMP), %.1f s; CPU: %s" % (nsteps, el, cpu_model())}
 
 
+def hbm_copy_rate(nbytes, reps=20):
+    """Device-to-device copy rate of the HIP runtime (hipMemcpyAsync of nbytes
+    between two device buffers, timed with HIP events): the bandwidth a plain
+    copy reaches on this GPU, beside the 8 TB/s peak the roofline is priced at.
+    Counts nbytes read + nbytes written per copy."""
+    import ctypes
+    try:
+        hip = ctypes.CDLL("libamdhip64.so")
+    except OSError:
+        hip = ctypes.CDLL("/opt/rocm/lib/libamdhip64.so")
+    vp, sz = ctypes.c_void_p, ctypes.c_size_t
+    a, b, e0, e1 = vp(), vp(), vp(), vp()
+    ms = ctypes.c_float()
+
+    def ok(rc, what):
+        if rc != 0:
+            raise RuntimeError("%s failed: hipError %d" % (what, rc))
+    ok(hip.hipMalloc(ctypes.byref(a), sz(nbytes)), "hipMalloc")
+    ok(hip.hipMalloc(ctypes.byref(b), sz(nbytes)), "hipMalloc")
+    try:
+        ok(hip.hipMemset(a, 0, sz(nbytes)), "hipMemset")
+        ok(hip.hipEventCreate(ctypes.byref(e0)), "hipEventCreate")
+        ok(hip.hipEventCreate(ctypes.byref(e1)), "hipEventCreate")
+        ok(hip.hipMemcpyAsync(b, a, sz(nbytes), 3, vp(0)), "hipMemcpyAsync")  # 3: device to device
+        ok(hip.hipEventRecord(e0, vp(0)), "hipEventRecord")
+        for _ in range(reps):
+            ok(hip.hipMemcpyAsync(b, a, sz(nbytes), 3, vp(0)), "hipMemcpyAsync")
+        ok(hip.hipEventRecord(e1, vp(0)), "hipEventRecord")
+        ok(hip.hipEventSynchronize(e1), "hipEventSynchronize")
+        ok(hip.hipEventElapsedTime(ctypes.byref(ms), e0, e1), "hipEventElapsedTime")
+        hip.hipEventDestroy(e0)
+        hip.hipEventDestroy(e1)
+    finally:
+        hip.hipFree(a)
+        hip.hipFree(b)
+    return 2.0 * nbytes * reps / (ms.value * 1e-3) / 1e9
+
+
 def kernel_traffic(dom, head):
     """HBM bytes per launch of the pass's kernel from the committed counter profile."""
     tpath = os.path.join(ROOT, "profiles", "traffic.json")
@@ -357,10 +395,15 @@ def main():
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "algorithmic_bytes": alg_bytes,
                      "traffic_source": "rocprofv3 FETCH_SIZE + WRITE_SIZE, profiles/traffic.json"},
+        "hbm_copy_GBs": None,
         "step_algorithmic_GBs": step_gbs,
         "step_roofline_frac": step_gbs / HBM_PEAK_GBS,
         "passes": pass_table,
     }
+    # a plain device copy of one FP32 field's size: the bandwidth a streaming
+    # kernel reaches in practice (reported beside the 8 TB/s peak, not used in frac)
+    out["hbm_copy_GBs"] = hbm_copy_rate(4 * p.nl * g.nyp2 * g.nxp2)
+    log("device copy: %.0f GB/s" % out["hbm_copy_GBs"])
     if "fp64" in res:
         ms64 = res["fp64"][0] / args.steps
         out["fp64_value"] = work / (ms64 * 1e-3)
