@@ -107,10 +107,10 @@ template <typename T> struct TwGlobal {  // the table in global memory
   __device__ __forceinline__ cx<T> operator()(int i) const { return p[i]; }
 };
 // Two-level table in LDS: W^i = hi[i / 64] * lo[i % 64], hi[h] = W^(64 h),
-// lo[l] = W^l (L >= 64, a power of two; exact where either factor is 1).
+// lo[l] = W^l (L >= 64; exact where either factor is 1).
 template <typename T, int L> struct TwTab {
-  static_assert(L >= 64 && (L & (L - 1)) == 0, "two-level table");
-  static constexpr int NHI = L / 64;
+  static_assert(L >= 64, "two-level table");
+  static constexpr int NHI = (L + 63) / 64;
   cx<T> lo[64], hi[NHI];
 };
 template <typename T, int L>

@@ -384,10 +384,11 @@ k_p1h(Geo g, const T *__restrict__ q, T *__restrict__ what, const cx<T> *__restr
 template <typename T, int M, int EP>
 __global__ void __launch_bounds__((DstHalf<M, EP>::NT))
 __attribute__((amdgpu_waves_per_eu((DstHalf<M, EP>::E == 8) ? (sizeof(T) == 4 ? 4 : 3) : 1)))
-k_p2h(Geo g, T *__restrict__ whatT, const double *__restrict__ La2, const double *__restrict__ Kb2,
-      const double *__restrict__ iRd2, double norm, const cx<T> *__restrict__ twh, const cx<T> *__restrict__ tw2) {
+k_p2h(Geo g, T *__restrict__ whatT, const double *__restrict__ La2, const double *__restrict__ iRd2, double dk,
+      double norm, const cx<T> *__restrict__ twh, const cx<T> *__restrict__ tw2) {
   constexpr int E = DstHalf<M, EP>::E, NT = DstHalf<M, EP>::NT;
   __shared__ cx<T> buf[DstHalf<M, EP>::slots];
+  __shared__ TwTab<T, 2 * M> tb2;  // sines of the inverse transform's pre pass
   const int t = threadIdx.x;
   const int n = g.n;
   const long rows = (long)g.nl * n;
@@ -400,23 +401,30 @@ k_p2h(Geo g, T *__restrict__ whatT, const double *__restrict__ La2, const double
   cx<T> h[E];
   // forward transform along y (row element y-1 holds x_y)
   dst_pre<T, M, EP>(h, t, tw2, [&](int y) { return mkc<T>(row0[y - 1], has1 ? row1[y - 1] : T(0)); });
+  twtab_fill(tb2, tw2, t, NT);  // published by the transform's barriers
   fft_home<T, M, EP>(h, buf, twh, t);
   dst_post<T, M, EP>(h, buf, t);
-  // Helmholtz division, f = 1 / (-(L_a^2 + K_b^2 + 1/Rd_k^2) * 4(n+1)(m+1)) (src/elliptic.h:120-138)
+  // Helmholtz division, f = 1 / (-(L_a^2 + K_b^2 + 1/Rd_k^2) * 4(n+1)(m+1)) (src/elliptic.h:120-138).
+  // K_b = pi b / Lx (src/domain.h:170-173) is formed here as b dk, dk = pi / Lx,
+  // within 2 ulp of the quotient, so that no table load stalls the division.
   const double d0 = La2[a0], d1 = La2[a1];
   const double ir0 = iRd2[k0], ir1 = iRd2[k1];
   const bool lo0 = (k0 == 0 && a0 <= 16), lo1 = (k1 == 0 && a1 <= 16);
   const T d0f = (T)(d0 + ir0), d1f = (T)(d1 + ir1), normf = (T)norm;
-  for (int i = t; i < M - 1; i += NT) {
+#pragma unroll
+  for (int s = 0; s < (M - 1 + NT - 1) / NT; s++) {
+    const int i = t + s * NT;
+    if (i >= M - 1) break;
     const int b = i + 1;
-    const double kb = Kb2[b];
     T f0, f1;
     if constexpr (sizeof(T) == 8) {
+      const double K = dk * (double)b, kb = K * K;
       f0 = (T)(1.0 / (-(d0 + kb + ir0) * norm));
       f1 = (T)(1.0 / (-(d1 + kb + ir1) * norm));
     } else {
       // FP32 forms: FP64 only for the lowest barotropic wavenumbers (mode 0, a, b <= 16);
       // elsewhere the hardware reciprocal (1 ulp), well inside the transforms' FP32 rounding
+      const double K = dk * (double)b, kb = K * K;
       const T kbf = (T)kb;
       f0 = -__builtin_amdgcn_rcpf((d0f + kbf) * normf);
       f1 = -__builtin_amdgcn_rcpf((d1f + kbf) * normf);
@@ -430,7 +438,7 @@ k_p2h(Geo g, T *__restrict__ whatT, const double *__restrict__ La2, const double
   }
   __syncthreads();
   // inverse transform (the same DST-I) of the divided spectra
-  dst_pre<T, M, EP>(h, t, tw2, [&](int y) { return buf[rslot(y - 1)]; });
+  dst_pre_s<T, M, EP>(h, t, SinLds<T, M>{&tb2}, [&](int y) { return buf[rslot(y - 1)]; });
   fft_home<T, M, EP>(h, buf, twh, t);
   dst_post<T, M, EP>(h, buf, t);
   for (int i = t; i < M - 1; i += NT) {

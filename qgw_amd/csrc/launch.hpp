@@ -33,6 +33,7 @@ struct LaunchArgs {
   void *fnew = nullptr, *f1 = nullptr, *f2 = nullptr, *comp = nullptr, *topo = nullptr, *wind = nullptr;
   const double *La2 = nullptr, *Kb2 = nullptr, *iRd2 = nullptr;
   double norm = 1.0, ring_c = 0.0;
+  double dk = 0.0;  // y wavenumber step of the bounded transforms, pi / Lx
   int cus = 256;    // compute units of the device (persistent grids)
   StepCoef cf{};
   int nlap = 0;

@@ -78,7 +78,7 @@ template <int M, int EP> static void p3h_go(const LaunchArgs &a, dim3 grid) {
 
 template <int M, int EP> static void p2h_go(const LaunchArgs &a, dim3 grid) {
   hipLaunchKernelGGL((k_p2h<T, M, EP>), grid, dim3(DstHalf<M, EP>::NT), 0, a.stream, a.g, (T *)a.whatT, a.La2,
-                     a.Kb2, a.iRd2, a.norm, (const cx<T> *)a.twyh, (const cx<T> *)a.twy);
+                     a.iRd2, a.dk, a.norm, (const cx<T> *)a.twyh, (const cx<T> *)a.twy);
 }
 
 template <> int launch_p1_t<T>(const LaunchArgs &a) {

@@ -66,6 +66,7 @@ struct qgw_ctx {
   int p4_waves;  // streaming P4 wavefronts per SIMD (QGW_TUNE_P4_WAVES)
   int dst_half;  // sine transforms (QGW_TUNE_DST: 0, 1 half-length, 2 pipelined rows); -1: half lengths not provided
   int cus;       // compute units of the device
+  double dk;     // y wavenumber step of the bounded transforms, pi / Lx (src/domain.h:170-173)
   int fft_e;     // elements per thread of the half-length FFTs (QGW_TUNE_FFT_E)
   int per;       // periodic domain (bc_fac = -1)
   bool modes_set;
@@ -283,6 +284,7 @@ int run_invert(qgw_ctx *c) {
   a.dst_half = c->dst_half > 0 ? c->dst_half : 0;
   a.fft_e = c->fft_e;
   a.cus = c->cus;
+  a.dk = c->dk;
   a.per = c->per;
   // normalisation of the transform pair (src/elliptic.h:36,51)
   a.norm = c->per ? (double)c->g.n * (double)c->g.m : 4.0 * (double)(c->g.n + 1) * (double)(c->g.m + 1);
@@ -843,6 +845,7 @@ qgw_ctx *create_group(const qgw_params *ps, int nb, int precision, hipStream_t e
     }
     c->cus = cus;
   }
+  c->dk = 3.141592653589793 / p->Lx;
   const size_t field = c->esz * (size_t)g.LS * NL;
   int st = QGW_OK;
   st = st ? st : dalloc_field(c, &c->q[0], field);
