@@ -81,15 +81,15 @@ __device__ __forceinline__ void dst_pre_s(cx<T> *h, int t, const Sn &sn, Ld ld) 
   constexpr int E = DstHalf<M, EP>::E, NT = DstHalf<M, EP>::NT;
 #pragma unroll
   for (int m = 0; m < E; m++) {
+    // branch-free: p = 0 (y_0 = 0) reads the valid pair of p = 1 and is zeroed
+    // after, so no wait is forced inside a divergent branch
     const int p = t + m * NT;
-    if (p == 0) {
-      h[m] = mkc<T>(T(0), T(0));
-    } else {
-      const cx<T> z = ld(p), zm = ld(M - p);
-      const T s = sn(p);
-      const T hx = T(0.5) * (z.x - zm.x), hy = T(0.5) * (z.y - zm.y);
-      h[m] = mkc<T>(s * (z.x + zm.x) + hx, s * (z.y + zm.y) + hy);
-    }
+    const int pc = (p == 0) ? 1 : p;
+    const cx<T> z = ld(pc), zm = ld(M - pc);
+    const T s = sn(pc);
+    const T hx = T(0.5) * (z.x - zm.x), hy = T(0.5) * (z.y - zm.y);
+    const cx<T> y = mkc<T>(s * (z.x + zm.x) + hx, s * (z.y + zm.y) + hy);
+    h[m] = (p == 0) ? mkc<T>(T(0), T(0)) : y;
   }
 }
 template <typename T, int M, int EP, class Ld>

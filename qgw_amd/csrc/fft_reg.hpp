@@ -122,6 +122,32 @@ __device__ __forceinline__ void twtab_fill(TwTab<T, L> &tb, const cx<T> *__restr
       tb.hi[i - 64] = tw[(i - 64) * 64];
   }
 }
+// The same fill split in two: the table loads into registers first, the LDS
+// stores later, so that the stores wait only for the table loads and work
+// issued in between (a transform's row loads) is not held up behind them.
+template <typename T, int L, int NT> struct TwTabFill {
+  static constexpr int CNT = 64 + TwTab<T, L>::NHI;
+  static constexpr int PER = (CNT + NT - 1) / NT;
+  cx<T> v[PER];
+  __device__ __forceinline__ void load(const cx<T> *__restrict__ tw, int t) {
+#pragma unroll
+    for (int r = 0; r < PER; r++) {
+      const int i = t + r * NT;
+      const int e = (i < 64) ? i : ((i < CNT) ? (i - 64) * 64 : 0);
+      v[r] = tw[e];
+    }
+  }
+  __device__ __forceinline__ void store(TwTab<T, L> &tb, int t) const {
+#pragma unroll
+    for (int r = 0; r < PER; r++) {
+      const int i = t + r * NT;
+      if (i < 64)
+        tb.lo[i] = v[r];
+      else if (i < CNT)
+        tb.hi[i - 64] = v[r];
+    }
+  }
+};
 template <typename T, int L> struct TwLds {
   const TwTab<T, L> *tb;
   __device__ __forceinline__ cx<T> operator()(int i) const { return cmul<T>(tb->hi[i >> 6], tb->lo[i & 63]); }

@@ -398,10 +398,14 @@ k_p2h(Geo g, T *__restrict__ whatT, const double *__restrict__ La2, const double
   const int k1 = has1 ? (int)(q1 / n) : k0, a1 = has1 ? (int)(q1 % n) + 1 : a0;
   T *row0 = whatT + k0 * g.WTS + (long)(a0 - 1) * g.Pm;
   T *row1 = whatT + k1 * g.WTS + (long)(a1 - 1) * g.Pm;
+  // the sine table's loads go first and its LDS stores after the pre pass, so
+  // that neither waits behind the row loads; published by the transform's barriers
+  TwTabFill<T, 2 * M, NT> tfill;
+  tfill.load(tw2, t);
   cx<T> h[E];
   // forward transform along y (row element y-1 holds x_y)
   dst_pre<T, M, EP>(h, t, tw2, [&](int y) { return mkc<T>(row0[y - 1], has1 ? row1[y - 1] : T(0)); });
-  twtab_fill(tb2, tw2, t, NT);  // published by the transform's barriers
+  tfill.store(tb2, t);
   fft_home<T, M, EP>(h, buf, twh, t);
   dst_post<T, M, EP>(h, buf, t);
   // Helmholtz division, f = 1 / (-(L_a^2 + K_b^2 + 1/Rd_k^2) * 4(n+1)(m+1)) (src/elliptic.h:120-138).
