@@ -122,16 +122,33 @@ k_tproj(const T *__restrict__ src, T *__restrict__ dst, const T *__restrict__ M,
   M += (long)blockIdx.z * nl * nl;
   const int c0 = blockIdx.x * TC, r0 = blockIdx.y * TR;
   const int tid = threadIdx.x;
-  // every layer of the thread's points, lanes along c (contiguous in src)
+  // every layer of the thread's points, lanes along c (contiguous in src):
+  // a uniform branch on the layer, then loads at clamped (valid) in-layer
+  // offsets from the layer's scalar base, zeroed outside the tile's R x C
   const int ci = tid % TC, rg = tid / TC;
+  int off[U];
+  bool ok[U];
+#pragma unroll
+  for (int u = 0; u < U; u++) {
+    const int r = r0 + rg + 8 * u, c = c0 + ci;
+    ok[u] = r < R && c < C;
+    off[u] = ((r < R) ? r : R - 1) * Ps + ((c < C) ? c : C - 1);
+  }
   T v[NLC][U];
 #pragma unroll
-  for (int l = 0; l < NLC; l++)
+  for (int l = 0; l < NLC; l++) {
+    if (l < nl) {
+      const T *lb = src + (long)l * Ss;
 #pragma unroll
-    for (int u = 0; u < U; u++) {
-      const int r = r0 + rg + 8 * u, c = c0 + ci;
-      v[l][u] = (l < nl && r < R && c < C) ? src[l * Ss + (long)r * Ps + c] : T(0);
+      for (int u = 0; u < U; u++) {
+        const T x = lb[off[u]];
+        v[l][u] = ok[u] ? x : T(0);
+      }
+    } else {
+#pragma unroll
+      for (int u = 0; u < U; u++) v[l][u] = T(0);
     }
+  }
   // store mapping: lanes along r (contiguous in dst)
   const int rl = tid % TR, cg = tid / TR;
   constexpr int CS = 256 / TR;  // columns per store pass
@@ -140,12 +157,17 @@ k_tproj(const T *__restrict__ src, T *__restrict__ dst, const T *__restrict__ M,
     for (int kb = 0; kb < KB; kb++) {
       const int k = k0 + kb;
       if (k >= nl) break;  // uniform
+      // row k of the matrix: NLC uniform (scalar) loads issued together and
+      // waited on once; entries past nl are clamped to a valid address and
+      // meet zero layers (v[l][u] = 0 for l >= nl), so their products vanish
+      T mk[NLC];
+#pragma unroll
+      for (int l = 0; l < NLC; l++) mk[l] = M[k * nl + ((l < nl) ? l : 0)];
 #pragma unroll
       for (int u = 0; u < U; u++) {
         T acc = T(0);
 #pragma unroll
-        for (int l = 0; l < NLC; l++)
-          if (l < nl) acc += M[k * nl + l] * v[l][u];  // uniform address: scalar loads
+        for (int l = 0; l < NLC; l++) acc += mk[l] * v[l][u];
         tile[kb][rg + 8 * u][ci] = acc;
       }
     }
@@ -155,10 +177,11 @@ k_tproj(const T *__restrict__ src, T *__restrict__ dst, const T *__restrict__ M,
     for (int kb = 0; kb < KB; kb++) {
       const int k = k0 + kb;
       if (k >= nl) break;
+      T *kd = dst + (long)k * Sd;
 #pragma unroll
       for (int cc = cg; cc < TC; cc += CS) {
         const int c = c0 + cc;
-        if (r < R && c < C) dst[k * Sd + (long)c * Pd + r] = tile[kb][rl][cc];
+        if (r < R && c < C) kd[c * Pd + r] = tile[kb][rl][cc];
       }
     }
     __syncthreads();
