@@ -283,6 +283,54 @@ def test_dst_half_vs_full(shape):
     assert rel(out[2][1], out[0][1]) < 1e-13
 
 
+_BIG = {}
+
+
+def _big_case():
+    """2048^2 x 4 layers: inputs and the oracle's inversion and 3-step state
+    (computed once, about 10 s on the host)."""
+    if not _BIG:
+        p = _small(2048, 4, Lx=4000e3, nu_hyper=1e4, ic_lambda0=200e3)
+        modes = Q.eigmode(p)
+        q0 = Q.make_ic(p, modes)[1]
+        o = O.Oracle(p)
+        o.set_modes(*modes)
+        o.set_q(q0)
+        o.invert_pv()
+        psi_o = o.field(O.PSI).copy()
+        o.run(3)
+        q_o = o.field(O.Q).copy()
+        o.close()
+        _BIG.update(p=p, modes=modes, q0=q0, psi_o=psi_o, q_o=q_o)
+    return _BIG
+
+
+@pytest.mark.parametrize("fft_e", [8, 16])
+def test_pipelined_rows_many_items(fft_e):
+    """Pipelined row transforms with several items per persistent block:
+    2048^2 x 4 layers has 4094 (row, layer pair) items against at most a few
+    per compute unit resident, so each block stages, prefetches and transforms
+    a run of items.  FP64 inversion and 3 steps against the oracle, and the
+    one-block form against the pipelined one."""
+    c = _big_case()
+    p, modes, q0, psi_o, q_o = c["p"], c["modes"], c["q0"], c["psi_o"], c["q_o"]
+    _announce("pipelined rows, many items per block", p.NX, p.nl, "fft_e", fft_e)
+    out = {}
+    for form in (1, 2):
+        m = Q.Model(p, Q.FP64, modes)
+        m.tune_set(Q.TUNE_DST, form)
+        m.tune_set(Q.TUNE_FFT_E, fft_e)
+        m.set_q(q0)
+        m.invert_pv()
+        psi = m.psi
+        m.run(3)
+        out[form] = (psi, m.q)
+        m.close()
+        assert rel(psi, psi_o) < 1e-12, form
+        assert rel(out[form][1], q_o) < 1e-12, form
+    assert rel(out[2][0], out[1][0]) < 1e-13
+
+
 @pytest.mark.parametrize("form", [1, 2])
 @pytest.mark.parametrize("fft_e", [8, 16])
 def test_dst_half_radix_schedules(fft_e, form):
