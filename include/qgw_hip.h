@@ -179,7 +179,7 @@ int qgw_profile_read(qgw_ctx *ctx, double *ms, int *count, int npass);
  * only). */
 #define QGW_TUNE_T_KERNEL 2
 /* QGW_TUNE_P4_WAVES sets the row chunk of the streaming P4 so that about
- * 2, 4, 8, 16 or 32 (default) wavefronts per SIMD are launched. */
+ * 2, 4, 8, 16, 32, 40 (default) or 48 wavefronts per SIMD are launched. */
 #define QGW_TUNE_P4_WAVES 3
 /* QGW_TUNE_DST selects the sine transforms of P1/P2/P3: 1 through complex
  * FFTs of length n+1 (m+1) with a pre/post pass (both half lengths n+1 and

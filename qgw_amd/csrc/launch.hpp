@@ -38,7 +38,7 @@ struct LaunchArgs {
   StepCoef cf{};
   int nlap = 0;
   int p4_tile = kP4TileCount;  // P4 variant: a tile index, or kP4TileCount for streaming
-  int p4_waves = 32;           // streaming P4: target wavefronts per SIMD (sets the row chunk)
+  int p4_waves = 40;           // streaming P4: target wavefronts per SIMD (sets the row chunk)
   int t_kernel = 0;            // projection: 0 auto (matrix cores for FP32 at >= 16 layers), 1 VALU, 2 matrix cores
   unsigned long long *cfl_max = nullptr;
 };

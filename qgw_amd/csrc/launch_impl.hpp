@@ -271,7 +271,7 @@ template <int NLAP, int KAHAN> static void p4s_go(const LaunchArgs &a) {
   const int nstrip = (a.g.n + ow - 1) / ow;
   // rows per wavefront: about p4_waves wavefronts per SIMD (256 CUs x 4 SIMDs)
   const long lanes = (long)nstrip * a.g.nl;
-  const long target = 1024L * (a.p4_waves > 0 ? a.p4_waves : 32);
+  const long target = 1024L * (a.p4_waves > 0 ? a.p4_waves : 40);
   int chunk = (int)(((long)a.g.m * lanes + target - 1) / target);
   chunk = chunk < 16 ? 16 : (chunk > 512 ? 512 : chunk);
   const int nchunk = (a.g.m + chunk - 1) / chunk;

@@ -941,7 +941,7 @@ qgw_ctx *create_group(const qgw_params *ps, int nb, int precision, hipStream_t e
   c->it = 0;
   c->qcur = 0;
   c->p4_tile = qgw::kP4TileCount;  // the register-streaming form
-  c->p4_waves = 32;
+  c->p4_waves = 40;
   c->fft_e = (precision == QGW_FP64) ? 16 : 8;
   c->per = per ? 1 : 0;
   c->dst_half = (!per && qgw::launch_has_half_length(Lx / 2) && qgw::launch_has_half_length(Ly / 2)) ? 2 : -1;
@@ -1051,10 +1051,11 @@ int autotune_group(qgw_ctx *c, int reps, int *p4_tile, double *best_ms) {
   if (st) return st;
   double best = 1e300;
   int arg = c->p4_tile, warg = c->p4_waves;
-  static const int kWaves[] = {4, 8, 16, 32};
+  static const int kWaves[] = {4, 8, 16, 32, 40, 48};
+  constexpr int kNW = sizeof(kWaves) / sizeof(kWaves[0]);
   for (int v = 0; v < (c->per ? 0 : qgw::kP4Variants) && !st; v++) {  // periodic: one P4 kernel
     const bool stream = (v == qgw::kP4TileCount);
-    for (int wi = 0; wi < (stream ? 4 : 1) && !st; wi++) {
+    for (int wi = 0; wi < (stream ? kNW : 1) && !st; wi++) {
       c->p4_tile = v;
       if (stream) c->p4_waves = kWaves[wi];
       double ms = 0;
@@ -1449,7 +1450,8 @@ int qgw_tune_set(qgw_ctx *c, int knob, int value) {
     return QGW_OK;
   }
   if (knob == QGW_TUNE_P4_WAVES) {
-    if (value != 2 && value != 4 && value != 8 && value != 16 && value != 32) return QGW_ERR_ARG;
+    if (value != 2 && value != 4 && value != 8 && value != 16 && value != 32 && value != 40 && value != 48)
+      return QGW_ERR_ARG;
     c->p4_waves = value;
     return QGW_OK;
   }
