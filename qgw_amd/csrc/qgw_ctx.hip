@@ -1136,6 +1136,12 @@ struct qgw_batch {
   };
   std::vector<Cached> cache;
   int graph_launches = 0;
+  // graph capture with more than one group: each group on a stream of its
+  // own, forked from and joined back to `stream` by events, so the captured
+  // graph holds one independent branch per group
+  std::vector<hipStream_t> gstreams;
+  std::vector<hipEvent_t> gjoin;
+  hipEvent_t gfork = nullptr;
   char err[256];
 };
 
@@ -1186,7 +1192,29 @@ void batch_release_graphs(qgw_batch *b) {
   b->cache.clear();
 }
 
+// Streams and events of a multi-group capture, made on first use.
+int batch_branch_streams(qgw_batch *b) {
+  const size_t ng = b->groups.size();
+  if (ng < 2 || b->gstreams.size() == ng) return QGW_OK;
+  hipError_t e = hipSuccess;
+  if (!b->gfork) e = hipEventCreateWithFlags(&b->gfork, hipEventDisableTiming);
+  while (e == hipSuccess && b->gstreams.size() < ng) {
+    hipStream_t st = nullptr;
+    e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+    if (e != hipSuccess) break;
+    b->gstreams.push_back(st);
+    hipEvent_t ev = nullptr;
+    e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+    if (e != hipSuccess) break;
+    b->gjoin.push_back(ev);
+  }
+  return (e == hipSuccess) ? QGW_OK : bfail(b, e, "batch branch streams");
+}
+
 // Capture kGraphSteps steps of every group from the current rotation state.
+// With several groups, each group's steps are captured on a stream of its own
+// (forked from the batch stream by an event and joined back by another), so
+// the groups' kernels are independent branches of the graph and may overlap.
 // Nothing runs during capture, so the host time state is put back after it.
 int batch_capture(qgw_batch *b, hipGraphExec_t *exec) {
   struct T {
@@ -1196,10 +1224,33 @@ int batch_capture(qgw_batch *b, hipGraphExec_t *exec) {
   std::vector<T> keep;
   for (auto *c : b->groups) keep.push_back({c->t, c->it});
   const auto sig0 = batch_sig(b);
+  int st = batch_branch_streams(b);
+  if (st) return st;
+  const bool branches = b->groups.size() > 1;
   hipError_t e = hipStreamBeginCapture(b->stream, hipStreamCaptureModeRelaxed);
   if (e != hipSuccess) return bfail(b, e, "hipStreamBeginCapture");
-  int st = QGW_OK;
-  for (int s = 0; s < kGraphSteps && !st; s++) st = batch_eager_step(b);
+  if (!branches) {
+    for (int s = 0; s < kGraphSteps && !st; s++) st = batch_eager_step(b);
+  } else {
+    e = hipEventRecord(b->gfork, b->stream);
+    if (e != hipSuccess) st = bfail(b, e, "hipEventRecord");
+    for (size_t i = 0; i < b->groups.size() && !st; i++) {
+      qgw_ctx *c = b->groups[i];
+      e = hipStreamWaitEvent(b->gstreams[i], b->gfork, 0);
+      if (e != hipSuccess) {
+        st = bfail(b, e, "hipStreamWaitEvent");
+        break;
+      }
+      hipStream_t own = c->stream;
+      c->stream = b->gstreams[i];
+      for (int s = 0; s < kGraphSteps && !st; s++) st = bpass(b, c, one_step(c));
+      c->stream = own;
+      if (st) break;
+      e = hipEventRecord(b->gjoin[i], b->gstreams[i]);
+      if (e == hipSuccess) e = hipStreamWaitEvent(b->stream, b->gjoin[i], 0);
+      if (e != hipSuccess) st = bfail(b, e, "branch join");
+    }
+  }
   hipGraph_t graph = nullptr;
   e = hipStreamEndCapture(b->stream, &graph);
   for (size_t i = 0; i < b->groups.size(); i++) {
@@ -1255,6 +1306,9 @@ void batch_free(qgw_batch *b) {
   if (b->stream) hipStreamSynchronize(b->stream);
   batch_release_graphs(b);
   for (auto *c : b->groups) release(c);
+  for (auto st : b->gstreams) hipStreamDestroy(st);
+  for (auto ev : b->gjoin) hipEventDestroy(ev);
+  if (b->gfork) hipEventDestroy(b->gfork);
   if (b->stream) hipStreamDestroy(b->stream);
   delete b;
 }
