@@ -494,30 +494,40 @@ k_p3h(Geo g, const T *__restrict__ what, T *__restrict__ psi, T *__restrict__ q,
   dst_pre<T, M, EP>(h, t, tw2, [&](int x) { return mkc<T>(i0[x - 1], has1 ? i1[x - 1] : T(0)); });
   fft_home<T, M, EP>(h, buf, twh, t);
   dst_post<T, M, EP>(h, buf, t);
-#pragma unroll
-  for (int side = 0; side < 2; side++) {
-    const int l = side ? l1 : l0;
-    if (side && !has1) break;
-    T *pr = psi + l * g.LS + (long)j * g.P;
-    T *qr = q + l * g.LS;
-    for (int i = t; i < n; i += NT) {
-      const int x = i + 1;
-      const cx<T> c = buf[rslot(i)];
-      const T v = side ? c.y : c.x;
-      pr[x] = v;
-      if (x == 1) qr[(long)j * g.P] = ring_c * v;
-      if (x == n) qr[(long)j * g.P + n + 1] = ring_c * v;
-      if (j == 1) qr[x] = ring_c * v;
-      if (j == m_) qr[(long)(m_ + 1) * g.P + x] = ring_c * v;
-    }
-    if (t == 0) {
-      pr[0] = T(0);
-      pr[n + 1] = T(0);
+  T *p0 = psi + l0 * g.LS + (long)j * g.P;
+  T *p1 = psi + l1 * g.LS + (long)j * g.P;
+  T *q0r = q + l0 * g.LS;
+  T *q1r = q + l1 * g.LS;
+  // both layers in one pass over buf, the wall-ring values of q beside psi
+  for (int i = t; i < n; i += NT) {
+    const int x = i + 1;
+    const cx<T> c = buf[rslot(i)];
+    p0[x] = c.x;
+    if (has1) p1[x] = c.y;
+    if (x == 1 || x == n) {
+      const long o = (long)j * g.P + (x == 1 ? 0 : n + 1);
+      q0r[o] = ring_c * c.x;
+      if (has1) q1r[o] = ring_c * c.y;
     }
     if (j == 1 || j == m_) {
-      const long rr = (long)(j == 1 ? 0 : m_ + 1) * g.P;
-      T *pw = psi + l * g.LS + rr;
-      for (int x = t; x < n + 2; x += NT) pw[x] = T(0);
+      const long o = (long)(j == 1 ? 0 : m_ + 1) * g.P + x;
+      q0r[o] = ring_c * c.x;
+      if (has1) q1r[o] = ring_c * c.y;
+    }
+  }
+  if (t == 0) {
+    p0[0] = T(0);
+    p0[n + 1] = T(0);
+    if (has1) {
+      p1[0] = T(0);
+      p1[n + 1] = T(0);
+    }
+  }
+  if (j == 1 || j == m_) {
+    const long rr = (long)(j == 1 ? 0 : m_ + 1) * g.P;
+    for (int x = t; x < n + 2; x += NT) {
+      psi[l0 * g.LS + rr + x] = T(0);
+      if (has1) psi[l1 * g.LS + rr + x] = T(0);
     }
   }
 }
@@ -530,8 +540,10 @@ k_p3h(Geo g, const T *__restrict__ what, T *__restrict__ psi, T *__restrict__ q,
 // transformed; the twiddles and the pre-pass sines come from two-level LDS
 // tables filled once per block, so the global traffic inside the loop is the
 // row loads and stores alone and no wait on a table load also waits on the
-// next item's rows.  Each item's row pair is staged in LDS (entry x - 1 holds
-// z_x), from where the pre pass reads z_p and z_{M-p}.  (The column pass P2
+// next item's rows.  Each item's row pair is staged in LDS (entry x holds z_x,
+// entry 0 unused), from where the pre pass reads z_p and z_{M-p}: both runs of
+// 16 lanes then start on a multiple of 16 entries, where the one-per-16 pad
+// keeps them on distinct banks.  (The column pass P2
 // stays one block per item: its two transforms fill the register file, and
 // the prefetch registers would cost it a wavefront per SIMD.)
 // The thread index passes through an empty asm at the top of each item, so
@@ -559,13 +571,13 @@ template <typename T, int M, int EP> struct PipeRows {
       zb[m] = (p >= 1 && has1) ? b : T(0);
     }
   }
-  // publish the fetched rows in buf (entry p - 1 holds z_p); the caller's
+  // publish the fetched rows in buf (entry p holds z_p); the caller's
   // barrier before this call retires the previous item's readers of buf
   __device__ __forceinline__ void stage(cx<T> *buf, int t) const {
 #pragma unroll
     for (int m = 0; m < E; m++) {
       const int p = t + m * NT;
-      if (p >= 1) buf[rslot(p - 1)] = mkc<T>(za[m], zb[m]);
+      if (p >= 1) buf[rslot(p)] = mkc<T>(za[m], zb[m]);
     }
   }
 };
@@ -603,7 +615,7 @@ k_p1hp(Geo g, const T *__restrict__ q, T *__restrict__ what, const cx<T> *__rest
     __syncthreads();
     if (it + (int)gridDim.x < nitems) fetch(it + gridDim.x, tl);
     cx<T> h[E];
-    dst_pre_s<T, M, EP>(h, tl, sna, [&](int x) { return buf[rslot(x - 1)]; });
+    dst_pre_s<T, M, EP>(h, tl, sna, [&](int x) { return buf[rslot(x)]; });
     fft_home_tw<T, M, EP>(h, buf, twa, tl);
     dst_post<T, M, EP>(h, buf, tl);
     const int lp = it % npair, j = it / npair + 1, l0 = 2 * lp, l1 = l0 + 1;
@@ -654,35 +666,47 @@ k_p3hp(Geo g, const T *__restrict__ what, T *__restrict__ psi, T *__restrict__ q
     __syncthreads();
     if (it + (int)gridDim.x < nitems) fetch(it + gridDim.x, tl);
     cx<T> h[E];
-    dst_pre_s<T, M, EP>(h, tl, sna, [&](int x) { return buf[rslot(x - 1)]; });
+    dst_pre_s<T, M, EP>(h, tl, sna, [&](int x) { return buf[rslot(x)]; });
     fft_home_tw<T, M, EP>(h, buf, twa, tl);
     dst_post<T, M, EP>(h, buf, tl);
     const int lp = it % npair, j = it / npair + 1, l0 = 2 * lp, l1 = l0 + 1;
     const bool has1 = l1 < g.nl;
-#pragma unroll
-    for (int side = 0; side < 2; side++) {
-      const int l = side ? l1 : l0;
-      if (side && !has1) break;
-      T *prow = psi + l * g.LS + (long)j * g.P;
-      T *qr = q + l * g.LS;
-      for (int i = tl; i < n; i += NT) {
-        const int x = i + 1;
-        const cx<T> c = buf[rslot(i)];
-        const T v = side ? c.y : c.x;
-        prow[x] = v;
-        if (x == 1) qr[(long)j * g.P] = ring_c * v;
-        if (x == n) qr[(long)j * g.P + n + 1] = ring_c * v;
-        if (j == 1) qr[x] = ring_c * v;
-        if (j == m_) qr[(long)(m_ + 1) * g.P + x] = ring_c * v;
-      }
-      if (tl == 0) {
-        prow[0] = T(0);
-        prow[n + 1] = T(0);
+    T *p0 = psi + l0 * g.LS + (long)j * g.P;
+    T *p1 = psi + l1 * g.LS + (long)j * g.P;
+    T *q0r = q + l0 * g.LS;
+    T *q1r = q + l1 * g.LS;
+    // both layers in one pass over buf (one LDS read per entry), the wall-ring
+    // values of q beside psi
+#pragma unroll 2
+    for (int i = tl; i < n; i += NT) {
+      const int x = i + 1;
+      const cx<T> c = buf[rslot(i)];
+      p0[x] = c.x;
+      if (has1) p1[x] = c.y;
+      if (x == 1 || x == n) {
+        const long o = (long)j * g.P + (x == 1 ? 0 : n + 1);
+        q0r[o] = ring_c * c.x;
+        if (has1) q1r[o] = ring_c * c.y;
       }
       if (j == 1 || j == m_) {
-        const long rr = (long)(j == 1 ? 0 : m_ + 1) * g.P;
-        T *pw = psi + l * g.LS + rr;
-        for (int x = tl; x < n + 2; x += NT) pw[x] = T(0);
+        const long o = (long)(j == 1 ? 0 : m_ + 1) * g.P + x;
+        q0r[o] = ring_c * c.x;
+        if (has1) q1r[o] = ring_c * c.y;
+      }
+    }
+    if (tl == 0) {
+      p0[0] = T(0);
+      p0[n + 1] = T(0);
+      if (has1) {
+        p1[0] = T(0);
+        p1[n + 1] = T(0);
+      }
+    }
+    if (j == 1 || j == m_) {
+      const long rr = (long)(j == 1 ? 0 : m_ + 1) * g.P;
+      for (int x = tl; x < n + 2; x += NT) {
+        psi[l0 * g.LS + rr + x] = T(0);
+        if (has1) psi[l1 * g.LS + rr + x] = T(0);
       }
     }
   }
