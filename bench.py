@@ -230,7 +230,8 @@ def forcing_rows(Q, p, modes, q0, steps):
 
 
 def cpu_baseline(p, modes, q0, seconds):
-    """The FP64 oracle (C restatement of the reference step) on all host cores."""
+    """The FP64 oracle (C restatement of the reference step) on all host cores,
+    and one step of it on a single core."""
     import oracle as O
     threads = O.threads(0)
     o = O.Oracle(p)
@@ -243,11 +244,19 @@ def cpu_baseline(p, modes, q0, seconds):
     t0 = time.perf_counter()
     o.run(nsteps)
     el = time.perf_counter() - t0
+    # one step on one core beside it: the single-threaded reference's own rate
+    O.threads(1)
+    t0 = time.perf_counter()
+    o.timestep()
+    el1 = time.perf_counter() - t0
+    O.threads(threads)
     o.close()
     work = p.NX * p.NY * p.nl
     return {"value": work * nsteps / el, "unit": "grid-pt*layer*steps/s", "cores": threads, "kind": "port",
             "sample": "%d steps of the same 2048^2 x 10 workload after 1 untimed step, FP64 oracle "
-                      "(gcc -O3, OpenMP), %.1f s; CPU: %s" % (nsteps, el, cpu_model())}
+                      "(gcc -O3, OpenMP), %.1f s; CPU: %s" % (nsteps, el, cpu_model()),
+            "serial": {"value": work / el1, "cores": 1,
+                       "sample": "1 step of the same workload on one thread, %.1f s" % el1}}
 
 
 def hbm_copy_rate(nbytes, reps=20):

@@ -190,15 +190,20 @@ int qgw_profile_read(qgw_ctx *ctx, double *ms, int *count, int npass);
 #define QGW_TUNE_DST 4
 /* QGW_TUNE_FFT_E: radix schedule of the half-length FFTs of power-of-two
  * lengths from 1024: 16 (radix-16 stages; default of QGW_FP64) or 8 (radix-8
- * stages, twice the threads per row; default of the FP32 forms). */
+ * stages, twice the threads per row; default of the FP32 forms), for the row
+ * passes P1 and P3. */
 #define QGW_TUNE_FFT_E 5
+/* QGW_TUNE_FFT_E_COL: the same choice for the column pass P2 (forward
+ * transform, Helmholtz division, inverse transform): 16 is the default of the
+ * FP32 forms, 8 of QGW_FP64. */
+#define QGW_TUNE_FFT_E_COL 6
 int qgw_tune_set(qgw_ctx *ctx, int knob, int value);
 int qgw_tune_get(qgw_ctx *ctx, int knob, int *value);
 
 /* Autotuner: times `reps` steps with each P4 variant (and, for the streaming
  * one, each QGW_TUNE_P4_WAVES value) and keeps the fastest (returned in
  * *p4_tile, its ms per step in *best_ms); it also chooses QGW_TUNE_DST and
- * QGW_TUNE_FFT_E, and QGW_TUNE_T_KERNEL for the FP32 forms.  The state (fields,
+ * QGW_TUNE_FFT_E and QGW_TUNE_FFT_E_COL, and QGW_TUNE_T_KERNEL for the FP32 forms.  The state (fields,
  * buffer rotation and time) is copied before the timing runs and restored
  * after them, so the run continues as if the call had not been made; the copy
  * takes one more state's worth of device memory while it runs. */

@@ -33,6 +33,7 @@ TUNE_T_KERNEL = 2
 TUNE_P4_WAVES = 3
 TUNE_DST = 4
 TUNE_FFT_E = 5
+TUNE_FFT_E_COL = 6
 P4_TILES = ["64x8", "64x16", "64x32", "128x16", "stream"]
 _TUNE_CACHE = {}
 
@@ -563,7 +564,7 @@ class Model:
         return {"p4": P4_TILES[self.tune_get(TUNE_P4_TILE)], "p4_waves": self.tune_get(TUNE_P4_WAVES),
                 "t_kernel": ["auto", "valu", "mfma32", "mfma16"][self.tune_get(TUNE_T_KERNEL)],
                 "dst": ["odd-extension", "half-length", "half-length pipelined"][self.tune_get(TUNE_DST)],
-                "fft_e": self.tune_get(TUNE_FFT_E)}
+                "fft_e": self.tune_get(TUNE_FFT_E), "fft_e_col": self.tune_get(TUNE_FFT_E_COL)}
 
     def autotune(self, reps=5):
         """Pick the tuning of this model by timing each variant on its own
@@ -576,7 +577,8 @@ class Model:
             _TUNE_CACHE[key] = {TUNE_P4_TILE: t.value, "ms_per_step": ms.value,
                                 TUNE_P4_WAVES: self.tune_get(TUNE_P4_WAVES),
                                 TUNE_T_KERNEL: self.tune_get(TUNE_T_KERNEL),
-                                TUNE_DST: self.tune_get(TUNE_DST), TUNE_FFT_E: self.tune_get(TUNE_FFT_E)}
+                                TUNE_DST: self.tune_get(TUNE_DST), TUNE_FFT_E: self.tune_get(TUNE_FFT_E),
+                                TUNE_FFT_E_COL: self.tune_get(TUNE_FFT_E_COL)}
         for knob, v in _TUNE_CACHE[key].items():
             if knob != "ms_per_step":
                 self.tune_set(knob, v)
@@ -719,7 +721,7 @@ class Batch:
                 "p4_waves": self.tune_get(group, TUNE_P4_WAVES),
                 "t_kernel": ["auto", "valu", "mfma32", "mfma16"][self.tune_get(group, TUNE_T_KERNEL)],
                 "dst": ["odd-extension", "half-length", "half-length pipelined"][self.tune_get(group, TUNE_DST)],
-                "fft_e": self.tune_get(group, TUNE_FFT_E)}
+                "fft_e": self.tune_get(group, TUNE_FFT_E), "fft_e_col": self.tune_get(group, TUNE_FFT_E_COL)}
 
     def profile(self, on=True):
         self._chk(self.lib.qgw_batch_profile_enable(self.b, 1 if on else 0), "profile_enable")

@@ -105,9 +105,76 @@ template <typename T, int M> struct SinLds {
   }
 };
 
+// Raw buffer loads of one element: the hardware range check returns 0 for a
+// byte offset at or beyond the record size, which also covers an offset of -1
+// element (it wraps to 2^32 - size).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t row_rsrc(const void *p, int bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, bytes, 0x00020000);
+}
+template <typename T> __device__ __forceinline__ T buf_ld(__amdgpu_buffer_rsrc_t r, int off) {
+  if constexpr (sizeof(T) == 8)
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+  else
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+// Pre pass from two rows in global memory, element x - 1 holding z_x (x = 1..N),
+// through buffer loads: the range check gives z_0 = z_M = 0 (offsets of -1 and
+// N elements), so p = 0 needs no clamp; b1 has a record size of 0 when the
+// second row does not exist.  The sines are the imaginary parts of the
+// length-2M table tw2, loaded alone.
+template <typename T, int M, int EP>
+__device__ __forceinline__ void dst_pre_rows(cx<T> *h, int t, __amdgpu_buffer_rsrc_t b0, __amdgpu_buffer_rsrc_t b1,
+                                             const cx<T> *__restrict__ tw2) {
+  constexpr int E = DstHalf<M, EP>::E, NT = DstHalf<M, EP>::NT, S = (int)sizeof(T);
+  const __amdgpu_buffer_rsrc_t bs = row_rsrc(tw2, 2 * M * 2 * S);
+#pragma unroll
+  for (int m = 0; m < E; m++) {
+    const int p = t + m * NT;
+    const cx<T> z = mkc<T>(buf_ld<T>(b0, (p - 1) * S), buf_ld<T>(b1, (p - 1) * S));
+    const cx<T> zm = mkc<T>(buf_ld<T>(b0, (M - 1 - p) * S), buf_ld<T>(b1, (M - 1 - p) * S));
+    const T s = -buf_ld<T>(bs, p * 2 * S + S);
+    const T hx = T(0.5) * (z.x - zm.x), hy = T(0.5) * (z.y - zm.y);
+    const cx<T> y = mkc<T>(s * (z.x + zm.x) + hx, s * (z.y + zm.y) + hy);
+    h[m] = (m == 0 && t == 0) ? mkc<T>(T(0), T(0)) : y;
+  }
+}
+
+// Pre pass from LDS: z_x at entry rslot(x) (x = 1..N), the sines from the
+// two-level table of length 2M.  With NT a multiple of 64 every address is a
+// per-thread base plus a compile-time offset: for u a multiple of 16,
+// rslot(u + t) = rslot(u) + rslot(t) and rslot(u - t) = rslot(u) - (t +
+// ceil(t / 16)); and sin(pi (t + m NT) / M) = -Im(hi[t/64 + m NT/64] lo[t mod 64])
+// takes the same lo entry for every m.  Lane t = 0 (p = 0) reads entries 0 and
+// rslot(M), whatever they hold, and y_0 = 0 is selected after.
+template <typename T, int M, int EP>
+__device__ __forceinline__ void dst_pre_lds(cx<T> *h, int t, const cx<T> *buf, const TwTab<T, 2 * M> &tb) {
+  constexpr int E = DstHalf<M, EP>::E, NT = DstHalf<M, EP>::NT;
+  if constexpr (NT % 64 == 0) {
+    const int rp = t + (t >> 4);
+    const int rn = t + ((t + 15) >> 4);
+    const cx<T> lo = tb.lo[t & 63];
+    const cx<T> *hib = tb.hi + (t >> 6);
+#pragma unroll
+    for (int m = 0; m < E; m++) {
+      const int up = m * NT, um = M - m * NT;
+      const cx<T> z = buf[rp + up + (up >> 4)];
+      const cx<T> zm = buf[um + (um >> 4) - rn];
+      const cx<T> a = hib[m * (NT / 64)];
+      const T s = -(a.x * lo.y + a.y * lo.x);
+      const T hx = T(0.5) * (z.x - zm.x), hy = T(0.5) * (z.y - zm.y);
+      const cx<T> y = mkc<T>(s * (z.x + zm.x) + hx, s * (z.y + zm.y) + hy);
+      h[m] = (m == 0 && t == 0) ? mkc<T>(T(0), T(0)) : y;
+    }
+  } else {
+    dst_pre_s<T, M, EP>(h, t, SinLds<T, M>{&tb}, [&](int x) { return buf[rslot(x)]; });
+  }
+}
+
 // Post pass: from Y at the home positions to S^a + i S^b in buf, natural
-// order (entry k-1 holds S_k, k = 1..N, at rslot(k-1)).  Ends with a barrier.
-template <typename T, int M, int EP> __device__ __forceinline__ void dst_post(cx<T> *h, cx<T> *buf, int t) {
+// order: entry k - 1 + SH holds S_k (k = 1..N) at rslot(k - 1 + SH); SH = 1
+// leaves entry 0 unwritten.  Ends with a barrier.
+template <typename T, int M, int EP, int SH = 0>
+__device__ __forceinline__ void dst_post(cx<T> *h, cx<T> *buf, int t) {
   constexpr int E = DstHalf<M, EP>::E, NT = DstHalf<M, EP>::NT, KPT = DstHalf<M, EP>::KPT;
   __syncthreads();  // the FFT's last readers of buf are done
 #pragma unroll
@@ -132,9 +199,9 @@ template <typename T, int M, int EP> __device__ __forceinline__ void dst_post(cx
 #pragma unroll
   for (int i = 0; i < KPT; i++) {
     const int k = t * KPT + i;
-    if (k >= 1) buf[rslot(2 * k - 1)] = ev[i];
+    if (k >= 1) buf[rslot(2 * k - 1 + SH)] = ev[i];
     const cx<double> o = cadd<double>(off, cum[i]);
-    buf[rslot(2 * k)] = mkc<T>((T)o.x, (T)o.y);
+    buf[rslot(2 * k + SH)] = mkc<T>((T)o.x, (T)o.y);
   }
   __syncthreads();
 }

@@ -427,10 +427,11 @@ k_p2h(Geo g, T *__restrict__ whatT, const double *__restrict__ La2, const double
   tfill.load(tw2, t);
   cx<T> h[E];
   // forward transform along y (row element y-1 holds x_y)
-  dst_pre<T, M, EP>(h, t, tw2, [&](int y) { return mkc<T>(row0[y - 1], has1 ? row1[y - 1] : T(0)); });
+  dst_pre_rows<T, M, EP>(h, t, row_rsrc(row0, (M - 1) * (int)sizeof(T)),
+                         row_rsrc(row1, has1 ? (M - 1) * (int)sizeof(T) : 0), tw2);
   tfill.store(tb2, t);
   fft_home<T, M, EP>(h, buf, twh, t);
-  dst_post<T, M, EP>(h, buf, t);
+  dst_post<T, M, EP, 1>(h, buf, t);  // entry b holds S_b
   // Helmholtz division, f = 1 / (-(L_a^2 + K_b^2 + 1/Rd_k^2) * 4(n+1)(m+1)) (src/elliptic.h:120-138).
   // K_b = pi b / Lx (src/domain.h:170-173) is formed here as b dk, dk = pi / Lx,
   // within 2 ulp of the quotient, so that no table load stalls the division.
@@ -438,11 +439,18 @@ k_p2h(Geo g, T *__restrict__ whatT, const double *__restrict__ La2, const double
   const double ir0 = iRd2[k0], ir1 = iRd2[k1];
   const bool lo0 = (k0 == 0 && a0 <= 16), lo1 = (k1 == 0 && a1 <= 16);
   const T d0f = (T)(d0 + ir0), d1f = (T)(d1 + ir1), normf = (T)norm;
+  // all of the thread's entries are read before any is written back, so that
+  // the LDS reads are in flight together
+  constexpr int NS = (M + NT - 1) / NT;
+  cx<T> hv[NS];
 #pragma unroll
-  for (int s = 0; s < (M - 1 + NT - 1) / NT; s++) {
-    const int i = t + s * NT;
-    if (i >= M - 1) break;
-    const int b = i + 1;
+  for (int s = 0; s < NS; s++) {
+    const int b = t + s * NT;
+    if (b < M) hv[s] = buf[rslot(b)];
+  }
+#pragma unroll
+  for (int s = 0; s < NS; s++) {
+    const int b = t + s * NT;
     T f0, f1;
     if constexpr (sizeof(T) == 8) {
       const double K = dk * (double)b, kb = K * K;
@@ -460,12 +468,11 @@ k_p2h(Geo g, T *__restrict__ whatT, const double *__restrict__ La2, const double
         if (lo1) f1 = (T)(1.0 / (-(d1 + kb + ir1) * norm));
       }
     }
-    const cx<T> v = buf[rslot(i)];
-    buf[rslot(i)] = mkc<T>(v.x * f0, v.y * f1);
+    if (b >= 1 && b < M) buf[rslot(b)] = mkc<T>(hv[s].x * f0, hv[s].y * f1);
   }
   __syncthreads();
   // inverse transform (the same DST-I) of the divided spectra
-  dst_pre_s<T, M, EP>(h, t, SinLds<T, M>{&tb2}, [&](int y) { return buf[rslot(y - 1)]; });
+  dst_pre_lds<T, M, EP>(h, t, buf, tb2);
   fft_home<T, M, EP>(h, buf, twh, t);
   dst_post<T, M, EP>(h, buf, t);
   for (int i = t; i < M - 1; i += NT) {
@@ -559,25 +566,29 @@ template <int NT> __device__ __forceinline__ int loop_tid(int t) {
 template <typename T, int M, int EP> struct PipeRows {
   static constexpr int E = DstHalf<M, EP>::E, NT = DstHalf<M, EP>::NT;
   T za[E], zb[E];
-  // z_p (p = t + m NT >= 1) of two rows: element p - 1 of r0 and r1 (r1 is a
-  // valid row when has1 is false); unconditional loads, zeros selected after
+  // z_p (p = t + m NT) of two rows of N = M - 1 elements, element p - 1 holding
+  // z_p: buffer loads, whose range check gives z_0 = 0 and, when has1 is
+  // false, zeros for the whole second row, with no select after the loads
   __device__ __forceinline__ void fetch(const T *__restrict__ r0, const T *__restrict__ r1, bool has1, int t) {
+    const __amdgpu_buffer_rsrc_t b0 = row_rsrc(r0, (M - 1) * (int)sizeof(T));
+    const __amdgpu_buffer_rsrc_t b1 = row_rsrc(r1, has1 ? (M - 1) * (int)sizeof(T) : 0);
+    const int o = (t - 1) * (int)sizeof(T);
 #pragma unroll
     for (int m = 0; m < E; m++) {
-      const int p = t + m * NT;
-      const int e = (p >= 1) ? p - 1 : 0;
-      const T a = r0[e], b = r1[e];
-      za[m] = (p >= 1) ? a : T(0);
-      zb[m] = (p >= 1 && has1) ? b : T(0);
+      za[m] = buf_ld<T>(b0, o + m * NT * (int)sizeof(T));
+      zb[m] = buf_ld<T>(b1, o + m * NT * (int)sizeof(T));
     }
   }
-  // publish the fetched rows in buf (entry p holds z_p); the caller's
-  // barrier before this call retires the previous item's readers of buf
+  // publish the fetched rows in buf (entry p holds z_p, entry 0 the zero z_0);
+  // the caller's barrier before this call retires the previous item's readers of buf
   __device__ __forceinline__ void stage(cx<T> *buf, int t) const {
+    const int rp = t + (t >> 4);
 #pragma unroll
     for (int m = 0; m < E; m++) {
-      const int p = t + m * NT;
-      if (p >= 1) buf[rslot(p)] = mkc<T>(za[m], zb[m]);
+      if constexpr (NT % 16 == 0)
+        buf[rp + m * NT + ((m * NT) >> 4)] = mkc<T>(za[m], zb[m]);  // = rslot(t + m NT)
+      else
+        buf[rslot(t + m * NT)] = mkc<T>(za[m], zb[m]);
     }
   }
 };
@@ -596,7 +607,6 @@ k_p1hp(Geo g, const T *__restrict__ q, T *__restrict__ what, const cx<T> *__rest
   twtab_fill(tb, twh, t, NT);
   twtab_fill(tb2, tw2, t, NT);
   const TwLds<T, M> twa{&tb};
-  const SinLds<T, M> sna{&tb2};
   PipeRows<T, M, EP> rows;
   // storage column x = 1..n of row j holds x_1..x_N
   auto fetch = [&](int it, int tf) {
@@ -615,7 +625,7 @@ k_p1hp(Geo g, const T *__restrict__ q, T *__restrict__ what, const cx<T> *__rest
     __syncthreads();
     if (it + (int)gridDim.x < nitems) fetch(it + gridDim.x, tl);
     cx<T> h[E];
-    dst_pre_s<T, M, EP>(h, tl, sna, [&](int x) { return buf[rslot(x)]; });
+    dst_pre_lds<T, M, EP>(h, tl, buf, tb2);
     fft_home_tw<T, M, EP>(h, buf, twa, tl);
     dst_post<T, M, EP>(h, buf, tl);
     const int lp = it % npair, j = it / npair + 1, l0 = 2 * lp, l1 = l0 + 1;
@@ -648,7 +658,6 @@ k_p3hp(Geo g, const T *__restrict__ what, T *__restrict__ psi, T *__restrict__ q
   twtab_fill(tb, twh, t, NT);
   twtab_fill(tb2, tw2, t, NT);
   const TwLds<T, M> twa{&tb};
-  const SinLds<T, M> sna{&tb2};
   PipeRows<T, M, EP> rows;
   auto fetch = [&](int it, int tf) {
     const int lp = it % npair, j = it / npair + 1, l0 = 2 * lp;
@@ -666,7 +675,7 @@ k_p3hp(Geo g, const T *__restrict__ what, T *__restrict__ psi, T *__restrict__ q
     __syncthreads();
     if (it + (int)gridDim.x < nitems) fetch(it + gridDim.x, tl);
     cx<T> h[E];
-    dst_pre_s<T, M, EP>(h, tl, sna, [&](int x) { return buf[rslot(x)]; });
+    dst_pre_lds<T, M, EP>(h, tl, buf, tb2);
     fft_home_tw<T, M, EP>(h, buf, twa, tl);
     dst_post<T, M, EP>(h, buf, tl);
     const int lp = it % npair, j = it / npair + 1, l0 = 2 * lp, l1 = l0 + 1;

@@ -29,6 +29,7 @@ struct LaunchArgs {
   void *twxh = nullptr, *twyh = nullptr;  // length-(L/2) tables of the half-length transforms
   int dst_half = 0;                       // 1: P1/P2/P3 through FFTs of length L/2; 2: and P1/P3 pipelined (powers of two)
   int fft_e = 16;                         // elements per thread of those FFTs (8 or 16, lengths >= 1024)
+  int fft_e_col = 16;                     // the same for the column pass P2
   int per = 0;                            // periodic domain (bc_fac = -1): Lx = Ly = N, FFT transforms
   void *fnew = nullptr, *f1 = nullptr, *f2 = nullptr, *comp = nullptr, *topo = nullptr, *wind = nullptr;
   const double *La2 = nullptr, *Kb2 = nullptr, *iRd2 = nullptr;

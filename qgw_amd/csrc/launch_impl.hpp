@@ -19,10 +19,12 @@ using T = QGW_T;
 // Lengths M of the half-length transforms (M = L/2 >= 64).
 #define QGW_HLENGTHS(X) X(64) X(128) X(256) X(512) X(1024) X(2048) X(4096) X(96) X(192) X(384) X(768) X(1536) X(3072)
 // Radix schedule of a half-length transform: powers of two from 1024 run with
-// 16 (radix-16 stages) or 8 (radix-8) elements per thread, a.fft_e.
-#define QGW_HALF_EP(M)                                    \
+// 16 (radix-16 stages) or 8 (radix-8) elements per thread: a.fft_e for the
+// row passes, a.fft_e_col for the column pass.
+#define QGW_HALF_EP(M) QGW_HALF_EP_E(M, a.fft_e)
+#define QGW_HALF_EP_E(M, EV)                              \
   if constexpr (is_pow2(M) && (M) >= 1024) {              \
-    if (a.fft_e == 8) {                                   \
+    if ((EV) == 8) {                                      \
       QGW_GO(M, 8)                                        \
     } else {                                              \
       QGW_GO(M, 16)                                       \
@@ -180,7 +182,7 @@ template <> int launch_p2_t<T>(const LaunchArgs &a) {
 #define QGW_GO(M, EP) p2h_go<M, EP>(a, grid);
 #define QGW_CASE(M) \
   case M:           \
-    QGW_HALF_EP(M)  \
+    QGW_HALF_EP_E(M, a.fft_e_col)  \
     return 0;
       QGW_HLENGTHS(QGW_CASE)
 #undef QGW_CASE

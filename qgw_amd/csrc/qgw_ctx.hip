@@ -67,7 +67,8 @@ struct qgw_ctx {
   int dst_half;  // sine transforms (QGW_TUNE_DST: 0, 1 half-length, 2 pipelined rows); -1: half lengths not provided
   int cus;       // compute units of the device
   double dk;     // y wavenumber step of the bounded transforms, pi / Lx (src/domain.h:170-173)
-  int fft_e;     // elements per thread of the half-length FFTs (QGW_TUNE_FFT_E)
+  int fft_e;     // elements per thread of the half-length FFTs of the row passes (QGW_TUNE_FFT_E)
+  int fft_e_col; // and of the column pass (QGW_TUNE_FFT_E_COL)
   int per;       // periodic domain (bc_fac = -1)
   bool modes_set;
   double t, dt, dt0, dt1, DT_max, t_out;
@@ -283,6 +284,7 @@ int run_invert(qgw_ctx *c) {
   a.twyh = c->twyh;
   a.dst_half = c->dst_half > 0 ? c->dst_half : 0;
   a.fft_e = c->fft_e;
+  a.fft_e_col = c->fft_e_col;
   a.cus = c->cus;
   a.dk = c->dk;
   a.per = c->per;
@@ -943,6 +945,7 @@ qgw_ctx *create_group(const qgw_params *ps, int nb, int precision, hipStream_t e
   c->p4_tile = qgw::kP4TileCount;  // the register-streaming form
   c->p4_waves = 40;
   c->fft_e = (precision == QGW_FP64) ? 16 : 8;
+  c->fft_e_col = (precision == QGW_FP64) ? 8 : 16;
   c->per = per ? 1 : 0;
   c->dst_half = (!per && qgw::launch_has_half_length(Lx / 2) && qgw::launch_has_half_length(Ly / 2)) ? 2 : -1;
   c->modes_set = false;
@@ -1104,6 +1107,21 @@ int autotune_group(qgw_ctx *c, int reps, int *p4_tile, double *best_ms) {
     }
     c->dst_half = darg;
     c->fft_e = earg;
+    // the column pass's radix schedule, with the row passes' chosen
+    if (c->dst_half > 0) {
+      int carg = c->fft_e_col;
+      double bc = 1e300;
+      for (int v = 0; v < 2 && !st; v++) {
+        c->fft_e_col = v ? 8 : 16;
+        double ms = 0;
+        st = time_candidate(c, reps, &ms);
+        if (!st && ms < bc) {
+          bc = ms;
+          carg = c->fft_e_col;
+        }
+      }
+      c->fft_e_col = carg;
+    }
   }
   // the chosen configuration, timed once more (the figure returned)
   if (!st) st = time_candidate(c, reps, &best);
@@ -1498,6 +1516,11 @@ int qgw_tune_set(qgw_ctx *c, int knob, int value) {
     c->fft_e = value;
     return QGW_OK;
   }
+  if (knob == QGW_TUNE_FFT_E_COL) {
+    if (value != 8 && value != 16) return QGW_ERR_ARG;
+    c->fft_e_col = value;
+    return QGW_OK;
+  }
   if (knob == QGW_TUNE_DST) {
     if (value < 0 || value > 2 || (value >= 1 && c->dst_half < 0)) return QGW_ERR_ARG;
     if (c->dst_half >= 0) c->dst_half = value;
@@ -1537,6 +1560,10 @@ int qgw_tune_get(qgw_ctx *c, int knob, int *value) {
   }
   if (knob == QGW_TUNE_FFT_E) {
     *value = c->fft_e;
+    return QGW_OK;
+  }
+  if (knob == QGW_TUNE_FFT_E_COL) {
+    *value = c->fft_e_col;
     return QGW_OK;
   }
   return QGW_ERR_ARG;

@@ -14,7 +14,7 @@ from conftest import rel
 pytestmark = pytest.mark.gpu
 
 # every knob the autotuner sets
-KNOBS = (Q.TUNE_P4_TILE, Q.TUNE_P4_WAVES, Q.TUNE_T_KERNEL, Q.TUNE_DST, Q.TUNE_FFT_E)
+KNOBS = (Q.TUNE_P4_TILE, Q.TUNE_P4_WAVES, Q.TUNE_T_KERNEL, Q.TUNE_DST, Q.TUNE_FFT_E, Q.TUNE_FFT_E_COL)
 
 
 def _announce(*a):

@@ -146,6 +146,24 @@ def test_diagnostics_match_oracle():
     assert Etm == pytest.approx(Eto, rel=1e-12) and Ztm == pytest.approx(Zto, rel=1e-12)
 
 
+@pytest.mark.parametrize("col", [8, 16])
+@pytest.mark.parametrize("fft_e", [8, 16])
+def test_fp32_radix_schedules(fft_e, col):
+    """The FP32 form with each radix schedule of the 1024-point row and column
+    transforms against the FP64 oracle: 6 steps within the FP32 budget of the
+    other FP32 checks (2e-5)."""
+    p = _small(1024, 2, NY=1024, Lx=4000e3)
+    _announce("fp32 radix schedule rows", fft_e, "columns", col)
+    m, o, _, _ = _pair(p, precision=Q.FP32)
+    m.tune_set(Q.TUNE_FFT_E, fft_e)
+    m.tune_set(Q.TUNE_FFT_E_COL, col)
+    m.run(6)
+    o.run(6)
+    assert rel(m.q, o.field(O.Q)) < 2e-5
+    m.close()
+    o.close()
+
+
 @pytest.mark.parametrize("prec,tol", [(Q.FP32, 2e-5), (Q.FP32_PLAIN, 2e-5)])
 def test_fp32_forms_c1(prec, tol):
     p = configs.c1()
@@ -320,6 +338,7 @@ def test_pipelined_rows_many_items(fft_e):
         m = Q.Model(p, Q.FP64, modes)
         m.tune_set(Q.TUNE_DST, form)
         m.tune_set(Q.TUNE_FFT_E, fft_e)
+        m.tune_set(Q.TUNE_FFT_E_COL, fft_e)
         m.set_q(q0)
         m.invert_pv()
         psi = m.psi
@@ -332,17 +351,19 @@ def test_pipelined_rows_many_items(fft_e):
 
 
 @pytest.mark.parametrize("form", [1, 2])
-@pytest.mark.parametrize("fft_e", [8, 16])
-def test_dst_half_radix_schedules(fft_e, form):
-    """Both radix schedules of the half-length FFTs (1024-point rows), one
-    block per row pair and pipelined, against the oracle."""
+@pytest.mark.parametrize("fft_e,col", [(8, 8), (16, 16), (8, 16), (16, 8)])
+def test_dst_half_radix_schedules(fft_e, col, form):
+    """Both radix schedules of the half-length FFTs (1024-point rows and
+    columns, each pass with either), one block per row pair and pipelined,
+    against the oracle."""
     p = _small(1024, 2, NY=1024, Lx=4000e3)
-    _announce("dst half radix schedule", fft_e, "form", form)
+    _announce("dst half radix schedule rows", fft_e, "columns", col, "form", form)
     modes = Q.eigmode(p)
     q0 = Q.make_ic(p, modes)[1]
     m = Q.Model(p, Q.FP64, modes)
     m.tune_set(Q.TUNE_DST, form)
     m.tune_set(Q.TUNE_FFT_E, fft_e)
+    m.tune_set(Q.TUNE_FFT_E_COL, col)
     m.set_q(q0)
     m.run(4)
     m.invert_pv()
