@@ -237,20 +237,39 @@ __device__ __forceinline__ void reg_stage(cx<T> *v, cx<T> *buf, const TW &tw, in
   if constexpr (more) {
     constexpr int R2 = Pl::radix(S + 1);
     constexpr int NB2 = E / R2;
+    // With the one-per-16 pad, rslot(B + c) = rslot(B) + c + c/16 whenever B mod 16
+    // plus c mod 16 stays below 16: so each butterfly's R writes are one
+    // per-thread base plus compile-time offsets when Ns = 1 (B = jR, R <= 16)
+    // or Ns R >= 16 (B mod 16 = j mod Ns, c = r Ns), and the reads are when NT
+    // and L/R2 are multiples of 16.
+    constexpr bool wfold = (Ns == 1 && R <= 16) || (Ns * R >= 16 && Ns <= 16) || Ns % 16 == 0;
+    constexpr bool rfold = NT % 16 == 0 && (L / R2) % 16 == 0;
     __syncthreads();  // previous readers of buf are done
 #pragma unroll
     for (int w = 0; w < NB; w++) {
       const int j = t + w * NT;
       const int base = (j / Ns) * Ns * R + (j % Ns);
+      const int rb = rslot(base);
 #pragma unroll
-      for (int r = 0; r < R; r++) buf[rslot(base + r * Ns)] = v[w * R + r];
+      for (int r = 0; r < R; r++) {
+        if constexpr (wfold)
+          buf[rb + r * Ns + ((r * Ns) >> 4)] = v[w * R + r];
+        else
+          buf[rslot(base + r * Ns)] = v[w * R + r];
+      }
     }
     __syncthreads();
+    const int rt = rslot(t);
 #pragma unroll
     for (int w = 0; w < NB2; w++) {
-      const int j = t + w * NT;
 #pragma unroll
-      for (int r = 0; r < R2; r++) v[w * R2 + r] = buf[rslot(j + r * (L / R2))];
+      for (int r = 0; r < R2; r++) {
+        const int c = w * NT + r * (L / R2);
+        if constexpr (rfold)
+          v[w * R2 + r] = buf[rt + c + (c >> 4)];
+        else
+          v[w * R2 + r] = buf[rslot(t + c)];
+      }
     }
     reg_stage<T, L, S + 1, EP, TW>(v, buf, tw, t, wnext);
   }

@@ -414,11 +414,12 @@ k_p2h(Geo g, T *__restrict__ whatT, const double *__restrict__ La2, const double
   __shared__ TwTab<T, 2 * M> tb2;  // sines of the inverse transform's pre pass
   const int t = threadIdx.x;
   const int n = g.n;
-  const long rows = (long)g.nl * n;
-  const long q0 = 2L * blockIdx.x, q1 = q0 + 1;
-  const bool has1 = q1 < rows;
-  const int k0 = (int)(q0 / n), a0 = (int)(q0 % n) + 1;
-  const int k1 = has1 ? (int)(q1 / n) : k0, a1 = has1 ? (int)(q1 % n) + 1 : a0;
+  // rows q0 = 2 blockIdx.x and q0 + 1 of nl n (< 2^31): mode k, column a (1-based);
+  // one 32-bit division, the second row following from the first
+  const unsigned q0 = 2u * blockIdx.x, un = (unsigned)n;
+  const bool has1 = q0 + 1 < (unsigned)g.nl * un;
+  const int k0 = (int)(q0 / un), a0 = (int)(q0 - (unsigned)k0 * un) + 1;
+  const int k1 = (has1 && a0 == n) ? k0 + 1 : k0, a1 = has1 ? (a0 == n ? 1 : a0 + 1) : a0;
   T *row0 = whatT + k0 * g.WTS + (long)(a0 - 1) * g.Pm;
   T *row1 = whatT + k1 * g.WTS + (long)(a1 - 1) * g.Pm;
   // the sine table's loads go first and its LDS stores after the pre pass, so
@@ -438,7 +439,8 @@ k_p2h(Geo g, T *__restrict__ whatT, const double *__restrict__ La2, const double
   const double d0 = La2[a0], d1 = La2[a1];
   const double ir0 = iRd2[k0], ir1 = iRd2[k1];
   const bool lo0 = (k0 == 0 && a0 <= 16), lo1 = (k1 == 0 && a1 <= 16);
-  const T d0f = (T)(d0 + ir0), d1f = (T)(d1 + ir1), normf = (T)norm;
+  // FP32: (d + K_b^2) norm = d norm + b^2 (dk^2 norm), b^2 exact in FP32 (b < 2^12)
+  const T d0n = (T)((d0 + ir0) * norm), d1n = (T)((d1 + ir1) * norm), c2 = (T)(dk * dk * norm);
   // all of the thread's entries are read before any is written back, so that
   // the LDS reads are in flight together
   constexpr int NS = (M + NT - 1) / NT;
@@ -458,12 +460,13 @@ k_p2h(Geo g, T *__restrict__ whatT, const double *__restrict__ La2, const double
       f1 = (T)(1.0 / (-(d1 + kb + ir1) * norm));
     } else {
       // FP32 forms: FP64 only for the lowest barotropic wavenumbers (mode 0, a, b <= 16);
-      // elsewhere the hardware reciprocal (1 ulp), well inside the transforms' FP32 rounding
-      const double K = dk * (double)b, kb = K * K;
-      const T kbf = (T)kb;
-      f0 = -__builtin_amdgcn_rcpf((d0f + kbf) * normf);
-      f1 = -__builtin_amdgcn_rcpf((d1f + kbf) * normf);
+      // elsewhere FP32 and the hardware reciprocal (1 ulp), well inside the
+      // transforms' FP32 rounding
+      const T bf = (T)b, bb = bf * bf;
+      f0 = -__builtin_amdgcn_rcpf(__builtin_fmaf(bb, c2, d0n));
+      f1 = -__builtin_amdgcn_rcpf(__builtin_fmaf(bb, c2, d1n));
       if (b <= 16) {
+        const double K = dk * (double)b, kb = K * K;
         if (lo0) f0 = (T)(1.0 / (-(d0 + kb + ir0) * norm));
         if (lo1) f1 = (T)(1.0 / (-(d1 + kb + ir1) * norm));
       }
