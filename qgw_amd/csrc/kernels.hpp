@@ -3,7 +3,8 @@
 // Device layout (per precision T):
 //   q[2], psi, f[3], comp   nl layers of (m+2) rows, row pitch P >= n+2; the
 //                           one-point wall ring is stored (reference idx
-//                           layout, src/domain.h:62, with a padded pitch)
+//                           layout, src/domain.h:62, with a padded pitch);
+//                           comp (FP32 forms) holds 16-bit elements, comp_t
 //   what                    nl layers of m rows (y) x n columns (x frequency), pitch Pn
 //   whatT                   nl modes of n rows (x frequency) x m columns (y), pitch Pm
 //
@@ -39,6 +40,37 @@ __device__ __forceinline__ int xcd_remap(int b, int total) {
   const int x = b & 7, i = b >> 3;
   const int q = total >> 3, r = total & 7;
   return x * q + (x < r ? x : r) + i;
+}
+
+// Kahan compensation of the FP32 forms: the rounding error c of the stored q
+// (q - c is the compensated value, |c| <= ulp(q)/2) kept as a 16-bit integer
+// multiple of ulp(q)/2^15, i.e. 15 bits below q's last place: q - c carries 39
+// significant bits, more than the FP32 increments added to it hold, in half
+// the bytes of an FP32 compensation.  The FP64 form keeps no compensation.
+template <typename T> struct CompOf {
+  using type = T;
+};
+template <> struct CompOf<float> {
+  using type = short;
+};
+template <typename T> using comp_t = typename CompOf<T>::type;
+__device__ __forceinline__ int comp_exp(float q) { return (int)((__float_as_uint(q) >> 23) & 0xffu); }
+// c in units of ulp(q)/2^15 = 2^(e - 165), e the biased exponent of q; clamped
+// to +-32767 (|c| above 2^15 ulp happens only under cancellation, where q's own
+// rounding dominates); q = 0 or subnormal decodes to 0
+template <typename T> __device__ __forceinline__ comp_t<T> comp_make(T c, T q) {
+  if constexpr (sizeof(T) == 4) {
+    const float v = __builtin_rintf(__builtin_ldexpf(c, 165 - comp_exp(q)));
+    return (short)__builtin_fminf(__builtin_fmaxf(v, -32767.f), 32767.f);
+  } else {
+    return c;
+  }
+}
+template <typename T> __device__ __forceinline__ T comp_val(comp_t<T> v, T q) {
+  if constexpr (sizeof(T) == 4)
+    return __builtin_ldexpf((float)v, comp_exp(q) - 165);
+  else
+    return v;
 }
 
 // Sizes shared by every kernel.
@@ -1049,8 +1081,8 @@ template <typename T, int NLAP, int KAHAN, int TX, int TY, bool FAST, bool PER =
 __device__ __forceinline__ void p4_body(P4Smem<T, TX, TY, NLAP> &sm, const Geo &g, const T *__restrict__ P,
                                         const T *__restrict__ Q, const T *__restrict__ topo, int k, int x0, int y0,
                                         int tx, int ty, const T (&hf1)[TY / 4], const T (&hf2)[TY / 4],
-                                        const T (&hcp)[TY / 4], T *__restrict__ qn, T *__restrict__ fnew,
-                                        T *__restrict__ comp, const T *__restrict__ wind, const StepCoef &cf) {
+                                        const comp_t<T> (&hcp)[TY / 4], T *__restrict__ qn, T *__restrict__ fnew,
+                                        comp_t<T> *__restrict__ comp, const T *__restrict__ wind, const StepCoef &cf) {
   using SM = P4Smem<T, TX, TY, NLAP>;
   constexpr int H = SM::H, PW = SM::PW, PH = SM::PH;
   (void)PW;
@@ -1156,9 +1188,9 @@ __device__ __forceinline__ void p4_body(P4Smem<T, TX, TY, NLAP> &sm, const Geo &
     if (!cf.update) continue;
     const T inc = c0 * f + c1 * hf1[r] + c2 * hf2[r];
     if constexpr (KAHAN) {
-      const T y = inc - hcp[r];
+      const T y = inc - comp_val<T>(hcp[r], qo);
       const T tq = qo + y;
-      comp[o] = (tq - qo) - y;
+      comp[o] = comp_make<T>((tq - qo) - y, tq);
       qn[o] = tq;
     } else {
       qn[o] = qo + inc;
@@ -1191,7 +1223,7 @@ __device__ __forceinline__ void p4_body(P4Smem<T, TX, TY, NLAP> &sm, const Geo &
 template <typename T, int NLAP, int KAHAN, int TX, int TY, bool PER = false>
 __global__ void __launch_bounds__(TX * 4)
 k_p4(Geo g, const T *__restrict__ psi, const T *__restrict__ q, T *__restrict__ qn, T *__restrict__ fnew,
-     const T *__restrict__ f1, const T *__restrict__ f2, T *__restrict__ comp, const T *__restrict__ topo,
+     const T *__restrict__ f1, const T *__restrict__ f2, comp_t<T> *__restrict__ comp, const T *__restrict__ topo,
      const T *__restrict__ wind, StepCoef cf) {
   using SM = P4Smem<T, TX, TY, NLAP>;
   constexpr int H = SM::H;
@@ -1213,7 +1245,8 @@ k_p4(Geo g, const T *__restrict__ psi, const T *__restrict__ q, T *__restrict__ 
   qn += lo;
   fnew += lo;
   if (KAHAN) comp += lo;
-  T hf1[TY / 4], hf2[TY / 4], hcp[TY / 4];
+  T hf1[TY / 4], hf2[TY / 4];
+  comp_t<T> hcp[TY / 4];
 #pragma unroll
   for (int r = 0; r < TY / 4; r++) {
     const int gx = x0 + tx, gy = y0 + ty + 4 * r;
@@ -1221,7 +1254,7 @@ k_p4(Geo g, const T *__restrict__ psi, const T *__restrict__ q, T *__restrict__ 
     const int o = gy * g.P + gx;
     hf1[r] = ok ? f1[o] : T(0);
     hf2[r] = ok ? f2[o] : T(0);
-    hcp[r] = (ok && KAHAN) ? comp[o] : T(0);
+    hcp[r] = (ok && KAHAN) ? comp[o] : comp_t<T>(0);
   }
   const T *P = psi + lo;
   const T *Q = q + lo;
@@ -1301,7 +1334,9 @@ __device__ __forceinline__ BufView buf_view(const void *base, unsigned bytes) {
 }
 // AUX: the cache-policy bits of the instruction (0: default caching).
 template <typename T, int AUX = 0> __device__ __forceinline__ T buf_ld(const BufView &b, unsigned xb, unsigned rowb) {
-  if constexpr (sizeof(T) == 4)
+  if constexpr (sizeof(T) == 2)
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b16(b.r, (int)xb, (int)rowb, AUX));
+  else if constexpr (sizeof(T) == 4)
     return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(b.r, (int)xb, (int)rowb, AUX));
   else
     return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(b.r, (int)xb, (int)rowb, AUX));
@@ -1316,6 +1351,9 @@ template <int AUX = 0> __device__ __forceinline__ void buf_st(const BufView &b, 
 template <typename T> __device__ __forceinline__ void st_at(T *rowp, unsigned xb, T v) {
   asm volatile("" : "+v"(xb));
   *(__attribute__((address_space(1))) T *)(uniform_gptr(rowp) + xb) = v;
+}
+__device__ __forceinline__ void buf_st(const BufView &b, unsigned xb, unsigned rowb, short v) {
+  __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, v), b.r, (int)xb, (int)rowb, 0);
 }
 
 // 12 Delta^2 J(p, q) in the 12-term form of src/domain.h:17-27, from the 3x3
@@ -1342,8 +1380,9 @@ template <int W> __device__ constexpr int wslot(int ph, int i) { return ((ph - i
 template <typename T, int NLAP, int KAHAN, bool EDGE>
 __device__ __forceinline__ void p4s_walk(const Geo &g, const T *__restrict__ P, const T *__restrict__ Qf,
                                          T *__restrict__ qn, T *__restrict__ fnew, const T *__restrict__ f1,
-                                         const T *__restrict__ f2, T *__restrict__ comp, const T *__restrict__ topo,
-                                         const T *__restrict__ wind, const StepCoef &cf, int x, int lane, int km,
+                                         const T *__restrict__ f2, comp_t<T> *__restrict__ comp,
+                                         const T *__restrict__ topo, const T *__restrict__ wind, const StepCoef &cf,
+                                         int x, int lane, int km,
                                          int y0, int y1, bool ringstrip) {
   constexpr int H = (NLAP > 1) ? NLAP : 1;       // halo, and the delay of the output row
   constexpr int NP = H + 2;                      // psi rows r, r-1, ..., r-H-1
@@ -1371,7 +1410,8 @@ __device__ __forceinline__ void p4s_walk(const Geo &g, const T *__restrict__ P, 
   T lv[NL][3];                                        // level a: rows r-a, r-a-1, r-a-2 (circular)
   T jpc[3], jpl[3], jpr[3], jqc[3], jql[3], jqr[3];  // rows y+1, y, y-1 of psi and q (circular)
   T jtc[3], jtl[3], jtr[3];                           // topography rows (bottom layer)
-  T bp[RING], bq[RING], bt[RING], b1[RING], b2[RING], bc[RING], bw[RING];  // loads of r .. r+D (D + 1 slots live)
+  T bp[RING], bq[RING], bt[RING], b1[RING], b2[RING], bw[RING];  // loads of r .. r+D (D + 1 slots live)
+  comp_t<T> bc[RING];
 #pragma unroll
   for (int i = 0; i < W; i++) pw[i] = nbw[i] = T(0);
 #pragma unroll
@@ -1385,13 +1425,20 @@ __device__ __forceinline__ void p4s_walk(const Geo &g, const T *__restrict__ P, 
     jtc[i] = jtl[i] = jtr[i] = T(0);
   }
 #pragma unroll
-  for (int i = 0; i < RING; i++) bp[i] = bq[i] = bt[i] = b1[i] = b2[i] = bc[i] = bw[i] = T(0);
+  for (int i = 0; i < RING; i++) {
+    bp[i] = bq[i] = bt[i] = b1[i] = b2[i] = bw[i] = T(0);
+    bc[i] = comp_t<T>(0);
+  }
 
   const int rbeg = y0 - H, rend = y1 + H;
   const unsigned xb = (unsigned)xc * sizeof(T);  // byte offset of this lane's column
   const unsigned lbytes = (unsigned)(g.LS * sizeof(T));
   const BufView vP = buf_view(P, lbytes), vQ = buf_view(Qf, lbytes), vT = buf_view(topo, lbytes);
-  const BufView v1 = buf_view(f1, lbytes), v2 = buf_view(f2, lbytes), vC = buf_view(comp, lbytes);
+  const BufView v1 = buf_view(f1, lbytes), v2 = buf_view(f2, lbytes);
+  // the compensation's element is sizeof(comp_t<T>) bytes: its offsets scale by CS / sizeof(T)
+  constexpr unsigned CS = sizeof(comp_t<T>);
+  const unsigned xbc = (unsigned)xc * CS, rowbc = (unsigned)g.P * CS;
+  const BufView vC = buf_view(comp, (unsigned)(g.LS * CS));
   const unsigned rowb = (unsigned)(g.P * sizeof(T));
   const BufView vW = buf_view(wind, (unsigned)((m + 2) * sizeof(T)));
   // loads of iteration r: psi row r, q and topography row r-H+1, AB3 history
@@ -1419,7 +1466,7 @@ __device__ __forceinline__ void p4s_walk(const Geo &g, const T *__restrict__ P, 
       if (upd) {
         b1[s] = ldout(f1, r - H);
         b2[s] = ldout(f2, r - H);
-        if (KAHAN) bc[s] = ldout(comp, r - H);
+        if (KAHAN) bc[s] = (outl && r - H >= y0 && r - H < y1) ? ld_at(comp + (r - H) * g.P, xbc) : comp_t<T>(0);
       }
     } else {
       // every row an interior walk touches lies inside the layer
@@ -1430,7 +1477,7 @@ __device__ __forceinline__ void p4s_walk(const Geo &g, const T *__restrict__ P, 
       if (r - H >= y0) {  // the AB3 history only for output rows (wavefront-uniform)
         b1[s] = buf_ld<T>(v1, xb, (unsigned)(r - H) * rowb);
         b2[s] = buf_ld<T>(v2, xb, (unsigned)(r - H) * rowb);
-        if (KAHAN) bc[s] = buf_ld<T>(vC, xb, (unsigned)(r - H) * rowb);
+        if (KAHAN) bc[s] = buf_ld<comp_t<T>>(vC, xbc, (unsigned)(r - H) * rowbc);
       }
     }
   };
@@ -1517,18 +1564,20 @@ __device__ __forceinline__ void p4s_walk(const Geo &g, const T *__restrict__ P, 
     const T qo = jqc[j1];
     const T inc = c0 * f + c1 * b1[cur] + c2 * b2[cur];
     T qv, cv = T(0);
+    comp_t<T> cs = comp_t<T>(0);
     if constexpr (KAHAN) {
-      const T yk = inc - bc[cur];
+      const T yk = inc - comp_val<T>(bc[cur], qo);
       qv = qo + yk;
       cv = (qv - qo) - yk;
+      cs = comp_make<T>(cv, qv);
     } else {
       qv = qo + inc;
     }
     if constexpr (EDGE) {
-      if constexpr (KAHAN) st_at(comp + ro, xb, cv);
+      if constexpr (KAHAN) st_at(comp + ro, xbc, cs);
       st_at(qn + ro, xb, qv);
     } else {
-      if constexpr (KAHAN) buf_st(vC, xb, rb, cv);
+      if constexpr (KAHAN) buf_st(vC, xbc, (unsigned)y * rowbc, cs);
       buf_st(vN, xb, rb, qv);
     }
     if constexpr (EDGE) {
@@ -1574,7 +1623,7 @@ __device__ __forceinline__ void p4s_walk(const Geo &g, const T *__restrict__ P, 
 template <typename T, int NLAP, int KAHAN>
 __global__ void __launch_bounds__(256)
 k_p4s(Geo g, const T *__restrict__ psi, const T *__restrict__ q, T *__restrict__ qn, T *__restrict__ fnew,
-      const T *__restrict__ f1, const T *__restrict__ f2, T *__restrict__ comp, const T *__restrict__ topo,
+      const T *__restrict__ f1, const T *__restrict__ f2, comp_t<T> *__restrict__ comp, const T *__restrict__ topo,
       const T *__restrict__ wind, StepCoef cf, int nstrip, int chunk, int nchunk) {
   constexpr int H = (NLAP > 1) ? NLAP : 1;  // halo
   constexpr int OW = 64 - 2 * H;            // output columns per strip
@@ -1649,7 +1698,7 @@ __global__ void __launch_bounds__(256) k_cfl(Geo g, const T *__restrict__ psi, u
 // ===========================================================================
 template <typename T>
 __global__ void __launch_bounds__(256)
-k_diag_partial(Geo g, const T *__restrict__ psi, const T *__restrict__ q, const T *__restrict__ comp,
+k_diag_partial(Geo g, const T *__restrict__ psi, const T *__restrict__ q, const comp_t<T> *__restrict__ comp,
                double *part, int nblk) {
   const int k = blockIdx.y;
   const long S = (long)(g.m + 2) * (g.n + 2);
@@ -1658,7 +1707,7 @@ k_diag_partial(Geo g, const T *__restrict__ psi, const T *__restrict__ q, const 
     const int j = (int)(idx / (g.n + 2)), i = (int)(idx % (g.n + 2));
     const long o = k * g.LS + (long)j * g.P + i;
     double qv = (double)q[o];
-    if (comp) qv -= (double)comp[o];
+    if (comp) qv -= (double)comp_val<T>(comp[o], q[o]);
     const double pv = (double)psi[o];
     e += pv * qv;
     z += qv * qv;
@@ -1745,7 +1794,7 @@ k_stoch_cols(int N, int nc, const int *__restrict__ col_start, const int *__rest
 template <typename T, int SR, bool KAHAN>
 __global__ void __launch_bounds__(256)
 k_stoch_rows(Geo g, int nc, const double2 *__restrict__ G, const double2 *__restrict__ E, double s_fac,
-             T *__restrict__ q, T *__restrict__ comp) {
+             T *__restrict__ q, comp_t<T> *__restrict__ comp) {
   constexpr int CC = 64, UE = 8;
   __shared__ double2 sg[SR][CC + UE];  // zero past the chunk: the unrolled steps need no branch
   const int j0 = blockIdx.x * SR;
@@ -1758,7 +1807,7 @@ k_stoch_rows(Geo g, int nc, const double2 *__restrict__ G, const double2 *__rest
   for (int r = 0; r < SR; r++) {
     const long o = (long)(j0 + (r < rows ? r : 0) + 1) * g.P + ipc + 1;
     qv[r] = q[o];
-    cv[r] = KAHAN ? comp[o] : T(0);
+    cv[r] = KAHAN ? comp_val<T>(comp[o], qv[r]) : T(0);
   }
   double acc[SR];
 #pragma unroll
@@ -1795,7 +1844,7 @@ k_stoch_rows(Geo g, int nc, const double2 *__restrict__ G, const double2 *__rest
     const T inc = (T)(acc[r] * s_fac);
     if constexpr (KAHAN) {
       const T y = inc - cv[r], t = qv[r] + y;
-      comp[o] = (t - qv[r]) - y;
+      comp[o] = comp_make<T>((t - qv[r]) - y, t);
       q[o] = t;
     } else {
       q[o] = qv[r] + inc;
@@ -1811,14 +1860,14 @@ k_stoch_rows(Geo g, int nc, const double2 *__restrict__ G, const double2 *__rest
 // ===========================================================================
 template <typename T, bool KAHAN>
 __global__ void __launch_bounds__(256)
-k_forcing(Geo g, T *__restrict__ q, T *__restrict__ comp, const T *__restrict__ F1, const T *__restrict__ F2,
+k_forcing(Geo g, T *__restrict__ q, comp_t<T> *__restrict__ comp, const T *__restrict__ F1, const T *__restrict__ F2,
           double w1, double w2, double dt) {
   const int k = blockIdx.x / g.m, j = blockIdx.x % g.m + 1;
   const int i = blockIdx.y * 256 + threadIdx.x + 1;
   if (i > g.n) return;
   const long o = (long)k * g.LS + (long)j * g.P + i;
   T v = q[o];
-  T c = KAHAN ? comp[o] : T(0);
+  T c = KAHAN ? comp_val<T>(comp[o], v) : T(0);
   auto add = [&](T inc) {
     if constexpr (KAHAN) {
       const T y = inc - c;
@@ -1832,7 +1881,7 @@ k_forcing(Geo g, T *__restrict__ q, T *__restrict__ comp, const T *__restrict__ 
   const double qf = w1 * (double)F1[o] + w2 * (double)F2[o];
   add((T)(qf * dt));
   q[o] = v;
-  if constexpr (KAHAN) comp[o] = c;
+  if constexpr (KAHAN) comp[o] = comp_make<T>(c, v);
 }
 
 // ===========================================================================
@@ -1840,7 +1889,7 @@ k_forcing(Geo g, T *__restrict__ q, T *__restrict__ comp, const T *__restrict__ 
 // ===========================================================================
 template <typename T>
 __global__ void __launch_bounds__(256)
-k_from_f64(Geo g, int layers, const double *__restrict__ src, T *__restrict__ dst, T *__restrict__ resid) {
+k_from_f64(Geo g, int layers, const double *__restrict__ src, T *__restrict__ dst, comp_t<T> *__restrict__ resid) {
   const long W = g.n + 2;
   const long S = W * (g.m + 2);
   const long tot = S * layers;
@@ -1851,13 +1900,13 @@ k_from_f64(Geo g, int layers, const double *__restrict__ src, T *__restrict__ ds
     const T tv = (T)v;
     const long o = k * g.LS + j * g.P + i;
     dst[o] = tv;
-    if (resid) resid[o] = (T)((double)tv - v);
+    if (resid) resid[o] = comp_make<T>((T)((double)tv - v), tv);
   }
 }
 
 template <typename T>
 __global__ void __launch_bounds__(256)
-k_to_f64(Geo g, int layers, const T *__restrict__ src, const T *__restrict__ resid, double *__restrict__ dst) {
+k_to_f64(Geo g, int layers, const T *__restrict__ src, const comp_t<T> *__restrict__ resid, double *__restrict__ dst) {
   const long W = g.n + 2;
   const long S = W * (g.m + 2);
   const long tot = S * layers;
@@ -1866,7 +1915,7 @@ k_to_f64(Geo g, int layers, const T *__restrict__ src, const T *__restrict__ res
     const long j = r / W, i = r % W;
     const long o = k * g.LS + j * g.P + i;
     double v = (double)src[o];
-    if (resid) v -= (double)resid[o];
+    if (resid) v -= (double)comp_val<T>(resid[o], src[o]);
     dst[e] = v;
   }
 }

@@ -263,7 +263,7 @@ template <> int launch_t2_t<T>(const LaunchArgs &a) {
 template <int NLAP, int KAHAN, int TX, int TY, bool PER = false> static void p4_go(const LaunchArgs &a) {
   const dim3 grid((a.g.n + TX - 1) / TX, (a.g.m + TY - 1) / TY, a.g.nl);
   hipLaunchKernelGGL((k_p4<T, NLAP, KAHAN, TX, TY, PER>), grid, dim3(TX * 4), 0, a.stream, a.g, (const T *)a.psi,
-                     (const T *)a.q, (T *)a.qn, (T *)a.fnew, (const T *)a.f1, (const T *)a.f2, (T *)a.comp,
+                     (const T *)a.q, (T *)a.qn, (T *)a.fnew, (const T *)a.f1, (const T *)a.f2, (comp_t<T> *)a.comp,
                      (const T *)a.topo, (const T *)a.wind, a.cf);
 }
 
@@ -280,7 +280,7 @@ template <int NLAP, int KAHAN> static void p4s_go(const LaunchArgs &a) {
   const long waves = lanes * nchunk;
   const dim3 grid((unsigned)((waves + 3) / 4));
   hipLaunchKernelGGL((k_p4s<T, NLAP, KAHAN>), grid, dim3(256), 0, a.stream, a.g, (const T *)a.psi, (const T *)a.q,
-                     (T *)a.qn, (T *)a.fnew, (const T *)a.f1, (const T *)a.f2, (T *)a.comp, (const T *)a.topo,
+                     (T *)a.qn, (T *)a.fnew, (const T *)a.f1, (const T *)a.f2, (comp_t<T> *)a.comp, (const T *)a.topo,
                      (const T *)a.wind, a.cf, nstrip, chunk, nchunk);
 }
 
@@ -328,7 +328,7 @@ template <> int launch_cfl_t<T>(const LaunchArgs &a) {
 
 template <> int launch_diag_t<T>(const LaunchArgs &a, double *part, int nblk, double *out) {
   hipLaunchKernelGGL((k_diag_partial<T>), dim3(nblk, a.g.nl), dim3(256), 0, a.stream, a.g, (const T *)a.psi,
-                     (const T *)a.q, (const T *)a.comp, part, nblk);
+                     (const T *)a.q, (const comp_t<T> *)a.comp, part, nblk);
   hipLaunchKernelGGL((k_diag_final<256>), dim3(a.g.nl), dim3(256), 0, a.stream, (const double *)part, nblk, out);
   return 0;
 }
@@ -339,7 +339,7 @@ int launch_from_f64_t<T>(const LaunchArgs &a, int layers, const double *src, voi
   long blocks = (tot + 255) / 256;
   if (blocks > 4096) blocks = 4096;
   hipLaunchKernelGGL((k_from_f64<T>), dim3((unsigned)blocks), dim3(256), 0, a.stream, a.g, layers, src, (T *)dst,
-                     (T *)resid);
+                     (comp_t<T> *)resid);
   return 0;
 }
 
@@ -354,10 +354,10 @@ template <> int launch_stoch_t<T>(const LaunchArgs &a, const StochArgs &s, doubl
   const dim3 grid((unsigned)((a.g.m + SR - 1) / SR), (unsigned)((a.g.n + 255) / 256));
   if (sizeof(T) == 4 && a.comp)
     hipLaunchKernelGGL((k_stoch_rows<T, SR, true>), grid, dim3(256), 0, a.stream, a.g, s.nc, (const double2 *)s.G,
-                       (const double2 *)s.E, s_fac, (T *)a.q, (T *)a.comp);
+                       (const double2 *)s.E, s_fac, (T *)a.q, (comp_t<T> *)a.comp);
   else
     hipLaunchKernelGGL((k_stoch_rows<T, SR, false>), grid, dim3(256), 0, a.stream, a.g, s.nc, (const double2 *)s.G,
-                       (const double2 *)s.E, s_fac, (T *)a.q, (T *)nullptr);
+                       (const double2 *)s.E, s_fac, (T *)a.q, (comp_t<T> *)nullptr);
   return 0;
 }
 
@@ -366,10 +366,10 @@ int launch_forcing_t<T>(const LaunchArgs &a, const void *F1, const void *F2, dou
   if (!F1 || !F2 || a.g.n < 1 || a.g.m < 1) return QGW_ERR_ARG;
   const dim3 grid((unsigned)(a.g.m * a.g.nl), (unsigned)((a.g.n + 255) / 256));
   if (sizeof(T) == 4 && a.comp)
-    hipLaunchKernelGGL((k_forcing<T, true>), grid, dim3(256), 0, a.stream, a.g, (T *)a.q, (T *)a.comp,
+    hipLaunchKernelGGL((k_forcing<T, true>), grid, dim3(256), 0, a.stream, a.g, (T *)a.q, (comp_t<T> *)a.comp,
                        (const T *)F1, (const T *)F2, w1, w2, dt);
   else
-    hipLaunchKernelGGL((k_forcing<T, false>), grid, dim3(256), 0, a.stream, a.g, (T *)a.q, (T *)nullptr,
+    hipLaunchKernelGGL((k_forcing<T, false>), grid, dim3(256), 0, a.stream, a.g, (T *)a.q, (comp_t<T> *)nullptr,
                        (const T *)F1, (const T *)F2, w1, w2, dt);
   return 0;
 }
@@ -380,7 +380,7 @@ int launch_to_f64_t<T>(const LaunchArgs &a, int layers, const void *src, const v
   long blocks = (tot + 255) / 256;
   if (blocks > 4096) blocks = 4096;
   hipLaunchKernelGGL((k_to_f64<T>), dim3((unsigned)blocks), dim3(256), 0, a.stream, a.g, layers, (const T *)src,
-                     (const T *)resid, dst);
+                     (const comp_t<T> *)resid, dst);
   return 0;
 }
 

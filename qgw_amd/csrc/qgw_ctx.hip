@@ -581,7 +581,8 @@ void *member_field(qgw_ctx *c, int member, int field, int *layers, void **resid)
   const size_t off = lay * (size_t)c->p.nl * member;
   switch (field) {
     case QGW_FIELD_Q:
-      if (c->prec == QGW_FP32) *resid = (char *)c->comp + off;
+      // the compensation has 16-bit elements at the same element offsets
+      if (c->prec == QGW_FP32) *resid = (char *)c->comp + off / c->esz * sizeof(short);
       return (char *)c->q[c->qcur] + off;
     case QGW_FIELD_PSI: return (char *)c->psi + off;
     case QGW_FIELD_TOPO: *layers = 1; return (char *)c->topo + lay * member;
@@ -854,6 +855,7 @@ qgw_ctx *create_group(const qgw_params *ps, int nb, int precision, hipStream_t e
   st = st ? st : dalloc_field(c, &c->q[1], field);
   st = st ? st : dalloc_field(c, &c->psi, field);
   for (int i = 0; i < 3; i++) st = st ? st : dalloc_field(c, &c->fa[i], field);
+  // (16-bit elements; a full field is allocated so that state copies treat it like the others)
   if (precision == QGW_FP32) st = st ? st : dalloc_field(c, &c->comp, field);
   st = st ? st : dalloc_field(c, &c->topo, c->esz * (size_t)g.LS * nb);
   st = st ? st : dalloc(c, &c->wind, c->esz * (size_t)(m + 2) * nb);

@@ -181,12 +181,18 @@ def test_fp32_forms_c1(prec, tol):
 
 
 def test_upload_download_roundtrip_fp32_keeps_residual():
+    """The FP32 form keeps q's FP64 rounding residual in its compensation (16
+    bits below q's last place): the download returns q to ~2^-39 relative,
+    where FP32 alone gives ~2^-24 (3e-8), and a dropped or mis-scaled residual
+    gives 1e-8 or worse."""
     p = configs.c1()
     _announce("fp32 upload residual")
     m = Q.Model(p, Q.FP32)
     _, q0 = Q.make_ic(p)
     m.set_q(q0)
-    assert rel(m.q, q0) < 1e-13
+    assert rel(m.q, q0) < 2e-12
+    # without the compensation: plain FP32 rounding
+    assert rel(q0.astype(np.float32).astype(np.float64), q0) > 1e-9
 
 
 @pytest.mark.parametrize("tile", [0, 1, 2, 3, 4])
