@@ -43,7 +43,8 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
 
 # Algorithmic HBM bytes per storage point.layer for each pass, in units of the
 # element size s (FP64: 8, FP32: 4).  P4 with Kahan adds a read and a write of
-# the compensation array.
+# the compensation array, whose elements are 16-bit (one word in all: 7 words of
+# 4 bytes, 28 bytes, per point.layer).
 PASS_WORDS = {"P1": 2, "T1": 2, "P2": 2, "T2": 2, "P3": 2, "P4": 6}
 # Kernel name prefixes of each pass, in order of preference (the profile holds
 # whichever form ran).
@@ -160,8 +161,8 @@ def forcing_rows(Q, p, modes, q0, steps):
     the pass after the AB3 update (QGW_PASS_FORC) timed with per-pass events.
     4-D file forcing: two records in a netCDF file, no record change inside the
     timed steps (k_forcing only; algorithmic bytes per interior point.layer:
-    q read+write and two records, plus the compensation read+write in the
-    FP32 form).  Stochastic: a ring of radius 32 grid wavenumbers; the pass
+    q read+write and two records, plus the 16-bit compensation read+write in
+    the FP32 form).  Stochastic: a ring of radius 32 grid wavenumbers; the pass
     holds the coefficient upload, the two synthesis kernels and k_forcing on
     layer 0.  Also the netCDF write of one vars.nc record (psi and q, float)."""
     import tempfile
@@ -186,7 +187,7 @@ def forcing_rows(Q, p, modes, q0, steps):
         for name, kw in legs.items():
             pf = Q.default_params(**dict(p.as_dict(), **kw))
             res = {}
-            for form, prec, words in (("fp64", Q.FP64, 4), ("fp32", Q.FP32, 6)):
+            for form, prec, words in (("fp64", Q.FP64, 4), ("fp32", Q.FP32, 5)):
                 m = Q.Model(pf, prec, modes)
                 m.set_q(q0)
                 if name == "forcing4d":
@@ -372,7 +373,7 @@ def main():
     s = 8 if head == "fp64" else 4
     words = dict(PASS_WORDS)
     if head == "fp32":
-        words["P4"] = 8
+        words["P4"] = 7
     per_launch = {k: v[0] / v[1] for k, v in passes.items() if v[1] and k in words}
     dom = max(per_launch, key=lambda k: per_launch[k] * passes[k][1])
     alg_bytes = words[dom] * s * work
