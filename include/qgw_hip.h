@@ -189,9 +189,8 @@ int qgw_profile_read(qgw_ctx *ctx, double *ms, int *count, int npass);
  * 1), 0 through complex FFTs of the odd extension, length 2(n+1) (2(m+1)). */
 #define QGW_TUNE_DST 4
 /* QGW_TUNE_FFT_E: radix schedule of the half-length FFTs of power-of-two
- * lengths from 1024: 16 (radix-16 stages; default of QGW_FP64) or 8 (radix-8
- * stages, twice the threads per row; default of the FP32 forms), for the row
- * passes P1 and P3. */
+ * lengths from 1024: 16 (radix-16 stages; the default) or 8 (radix-8 stages,
+ * twice the threads per row), for the row passes P1 and P3. */
 #define QGW_TUNE_FFT_E 5
 /* QGW_TUNE_FFT_E_COL: the same choice for the column pass P2 (forward
  * transform, Helmholtz division, inverse transform): 16 is the default of the

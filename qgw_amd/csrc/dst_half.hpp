@@ -171,23 +171,45 @@ __device__ __forceinline__ void dst_pre_lds(cx<T> *h, int t, const cx<T> *buf, c
 }
 
 // Post pass: from Y at the home positions to S^a + i S^b in buf, natural
-// order: entry k - 1 + SH holds S_k (k = 1..N) at rslot(k - 1 + SH); SH = 1
-// leaves entry 0 unwritten.  Ends with a barrier.
+// order: entry k - 1 + SH holds S_k (k = 1..N) at rslot(k - 1 + SH); with SH = 1
+// entry 0 holds no coefficient (zero or unwritten).  Ends with a barrier.
 template <typename T, int M, int EP, int SH = 0>
 __device__ __forceinline__ void dst_post(cx<T> *h, cx<T> *buf, int t) {
   constexpr int E = DstHalf<M, EP>::E, NT = DstHalf<M, EP>::NT, KPT = DstHalf<M, EP>::KPT;
+  // Addresses as a per-thread base plus compile-time offsets where the pad
+  // allows (rslot(B + c) = rslot(B) + c + c/16 while B mod 16 + c mod 16 < 16):
+  // the home positions t + m NT (NT a multiple of 16), the thread's KPT
+  // consecutive k and its 2 KPT consecutive outputs (KPT and 2 KPT dividing 16
+  // or multiples of it).
+  constexpr bool fold = NT % 16 == 0 && M % 16 == 0 && (16 % KPT == 0 || KPT % 16 == 0) &&
+                        (16 % (2 * KPT) == 0 || (2 * KPT) % 16 == 0);
+  const int rt = rslot(t);
   __syncthreads();  // the FFT's last readers of buf are done
 #pragma unroll
-  for (int m = 0; m < E; m++) buf[rslot(t + m * NT)] = h[m];
+  for (int m = 0; m < E; m++) {
+    if constexpr (fold)
+      buf[rt + m * NT + ((m * NT) >> 4)] = h[m];
+    else
+      buf[rslot(t + m * NT)] = h[m];
+  }
   __syncthreads();
   cx<T> ev[KPT];
   cx<double> cum[KPT];
   cx<double> run = mkc<double>(0.0, 0.0);
+  const int rk = rslot(t * KPT);                            // entry of k = KPT t
+  const int rm = rslot(M - KPT * (t + 1));                  // entry of M - KPT (t + 1)
+  const int rm0 = (t == 0) ? 0 : rslot(M - KPT * t);        // Y_{M-k} of i = 0 (Y_0 for k = 0)
 #pragma unroll
   for (int i = 0; i < KPT; i++) {
     const int k = t * KPT + i;
-    const cx<T> Yk = buf[rslot(k)];
-    const cx<T> Ym = buf[rslot(k == 0 ? 0 : M - k)];
+    cx<T> Yk, Ym;
+    if constexpr (fold) {
+      Yk = buf[rk + i + (i >> 4)];
+      Ym = buf[i == 0 ? rm0 : rm + (KPT - i) + ((KPT - i) >> 4)];
+    } else {
+      Yk = buf[rslot(k)];
+      Ym = buf[rslot(k == 0 ? 0 : M - k)];
+    }
     ev[i] = mkc<T>(Ym.y - Yk.y, Yk.x - Ym.x);
     cx<double> r = mkc<double>((double)Yk.x + (double)Ym.x, (double)Yk.y + (double)Ym.y);
     if (k == 0) r = mkc<double>(0.5 * r.x, 0.5 * r.y);
@@ -195,13 +217,22 @@ __device__ __forceinline__ void dst_post(cx<T> *h, cx<T> *buf, int t) {
     cum[i] = run;
   }
   const cx<double> off = block_excl_scan<NT>(run, t);
+  const int r2 = rslot(2 * KPT * t);  // entry of 2k for i = 0
   __syncthreads();  // every read of Y is done
 #pragma unroll
   for (int i = 0; i < KPT; i++) {
     const int k = t * KPT + i;
-    if (k >= 1) buf[rslot(2 * k - 1 + SH)] = ev[i];
-    const cx<double> o = cadd<double>(off, cum[i]);
-    buf[rslot(2 * k + SH)] = mkc<T>((T)o.x, (T)o.y);
+    if constexpr (fold) {
+      if (i > 0 || SH == 1)
+        buf[r2 + (2 * i - 1 + SH) + ((2 * i - 1 + SH) >> 4)] = ev[i];  // entry 2k - 1 + SH
+      else if (k >= 1)
+        buf[rslot(2 * k - 1)] = ev[i];
+      buf[r2 + (2 * i + SH) + ((2 * i + SH) >> 4)] = mkc<T>((T)(off.x + cum[i].x), (T)(off.y + cum[i].y));
+    } else {
+      if (k >= 1) buf[rslot(2 * k - 1 + SH)] = ev[i];
+      const cx<double> o = cadd<double>(off, cum[i]);
+      buf[rslot(2 * k + SH)] = mkc<T>((T)o.x, (T)o.y);
+    }
   }
   __syncthreads();
 }

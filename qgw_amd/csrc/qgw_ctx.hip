@@ -946,7 +946,7 @@ qgw_ctx *create_group(const qgw_params *ps, int nb, int precision, hipStream_t e
   c->qcur = 0;
   c->p4_tile = qgw::kP4TileCount;  // the register-streaming form
   c->p4_waves = 40;
-  c->fft_e = (precision == QGW_FP64) ? 16 : 8;
+  c->fft_e = 16;
   c->fft_e_col = (precision == QGW_FP64) ? 8 : 16;
   c->per = per ? 1 : 0;
   c->dst_half = (!per && qgw::launch_has_half_length(Lx / 2) && qgw::launch_has_half_length(Ly / 2)) ? 2 : -1;
