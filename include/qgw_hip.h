@@ -185,8 +185,8 @@ int qgw_profile_read(qgw_ctx *ctx, double *ms, int *count, int npass);
  * FFTs of length n+1 (m+1) with a pre/post pass (both half lengths n+1 and
  * m+1 in 64..4096 or 96..3072), 2 (default where 1 is provided) as 1 with
  * the row passes P1/P3 in persistent blocks that load the next row pair
- * while transforming the current one (power-of-two n+1; other lengths run as
- * 1), 0 through complex FFTs of the odd extension, length 2(n+1) (2(m+1)). */
+ * while transforming the current one, 0 through complex FFTs of the odd
+ * extension, length 2(n+1) (2(m+1)). */
 #define QGW_TUNE_DST 4
 /* QGW_TUNE_FFT_E: radix schedule of the half-length FFTs of power-of-two
  * lengths from 1024: 16 (radix-16 stages; the default) or 8 (radix-8 stages,

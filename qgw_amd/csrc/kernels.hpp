@@ -575,7 +575,7 @@ k_p3h(Geo g, const T *__restrict__ what, T *__restrict__ psi, T *__restrict__ q,
 }
 
 // ===========================================================================
-// Pipelined half-length row transforms (power-of-two M): the same outputs as
+// Pipelined half-length row transforms (every half length M): the same outputs as
 // k_p1h / k_p3h from persistent blocks.  A block walks the items
 // it = blockIdx.x, blockIdx.x + gridDim.x, ... (a row and layer pair).  The
 // rows of item it + gridDim.x are loaded into registers while item it is
@@ -628,6 +628,33 @@ template <typename T, int M, int EP> struct PipeRows {
   }
 };
 
+// Transform of the pipelined row passes: the register plan with the two-level
+// LDS table for powers of two; for 3 2^k the LDS Stockham of fft.hpp on buf,
+// with its full twiddle table in LDS (twf), so that no global twiddle load
+// waits behind the prefetched rows.
+template <typename T, int M, int EP>
+__device__ __forceinline__ void fft_pipe(cx<T> *h, cx<T> *buf, const TwLds<T, M> &twa, const cx<T> *twf, int t) {
+  if constexpr (is_pow2(M)) {
+    fft_home_tw<T, M, EP>(h, buf, twa, t);
+  } else {
+    constexpr int E = DstHalf<M, EP>::E, NT = DstHalf<M, EP>::NT;
+    __syncthreads();  // the pre pass's readers of buf are done
+#pragma unroll
+    for (int m = 0; m < E; m++) buf[lds_slot(t + m * NT)] = h[m];
+    __syncthreads();
+    Stages<T, M, NT, 0>::run(buf, twf, t);
+#pragma unroll
+    for (int m = 0; m < E; m++) h[m] = buf[lds_slot(t + m * NT)];
+  }
+}
+// the full length-M table of the LDS Stockham (3 2^k lengths; one entry otherwise)
+template <typename T, int M> constexpr int twf_len() { return is_pow2(M) ? 1 : M; }
+template <typename T, int M>
+__device__ __forceinline__ void twf_fill(cx<T> *twf, const cx<T> *__restrict__ tw, int t, int nt) {
+  if constexpr (!is_pow2(M))
+    for (int i = t; i < M; i += nt) twf[i] = tw[i];
+}
+
 template <typename T, int M, int EP>
 __global__ void __launch_bounds__((DstHalf<M, EP>::NT))
 k_p1hp(Geo g, const T *__restrict__ q, T *__restrict__ what, const cx<T> *__restrict__ twh,
@@ -635,11 +662,13 @@ k_p1hp(Geo g, const T *__restrict__ q, T *__restrict__ what, const cx<T> *__rest
   constexpr int E = DstHalf<M, EP>::E, NT = DstHalf<M, EP>::NT;
   __shared__ cx<T> buf[DstHalf<M, EP>::slots];
   __shared__ TwTab<T, M> tb;
+  __shared__ cx<T> twf[twf_len<T, M>()];
   __shared__ TwTab<T, 2 * M> tb2;
   const int t = threadIdx.x;
   const int npair = (g.nl + 1) / 2;
   const int nitems = g.m * npair;
   twtab_fill(tb, twh, t, NT);
+  twf_fill<T, M>(twf, twh, t, NT);
   twtab_fill(tb2, tw2, t, NT);
   const TwLds<T, M> twa{&tb};
   PipeRows<T, M, EP> rows;
@@ -661,7 +690,7 @@ k_p1hp(Geo g, const T *__restrict__ q, T *__restrict__ what, const cx<T> *__rest
     if (it + (int)gridDim.x < nitems) fetch(it + gridDim.x, tl);
     cx<T> h[E];
     dst_pre_lds<T, M, EP>(h, tl, buf, tb2);
-    fft_home_tw<T, M, EP>(h, buf, twa, tl);
+    fft_pipe<T, M, EP>(h, buf, twa, twf, tl);
     dst_post<T, M, EP>(h, buf, tl);
     const int lp = it % npair, j = it / npair + 1, l0 = 2 * lp, l1 = l0 + 1;
     const bool has1 = l1 < g.nl;
@@ -685,12 +714,14 @@ k_p3hp(Geo g, const T *__restrict__ what, T *__restrict__ psi, T *__restrict__ q
   constexpr int E = DstHalf<M, EP>::E, NT = DstHalf<M, EP>::NT;
   __shared__ cx<T> buf[DstHalf<M, EP>::slots];
   __shared__ TwTab<T, M> tb;
+  __shared__ cx<T> twf[twf_len<T, M>()];
   __shared__ TwTab<T, 2 * M> tb2;
   const int t = threadIdx.x;
   const int n = g.n, m_ = g.m;
   const int npair = (g.nl + 1) / 2;
   const int nitems = g.m * npair;
   twtab_fill(tb, twh, t, NT);
+  twf_fill<T, M>(twf, twh, t, NT);
   twtab_fill(tb2, tw2, t, NT);
   const TwLds<T, M> twa{&tb};
   PipeRows<T, M, EP> rows;
@@ -711,7 +742,7 @@ k_p3hp(Geo g, const T *__restrict__ what, T *__restrict__ psi, T *__restrict__ q
     if (it + (int)gridDim.x < nitems) fetch(it + gridDim.x, tl);
     cx<T> h[E];
     dst_pre_lds<T, M, EP>(h, tl, buf, tb2);
-    fft_home_tw<T, M, EP>(h, buf, twa, tl);
+    fft_pipe<T, M, EP>(h, buf, twa, twf, tl);
     dst_post<T, M, EP>(h, buf, tl);
     const int lp = it % npair, j = it / npair + 1, l0 = 2 * lp, l1 = l0 + 1;
     const bool has1 = l1 < g.nl;

@@ -48,31 +48,27 @@ template <class K> static unsigned persist_grid(K kern, int nt, long nitems, int
 }
 
 // Half-length transforms of length M: for the row passes P1/P3 the pipelined
-// form for powers of two under dst_half == 2, else one block per item.
+// form under dst_half == 2, else one block per item.
 template <int M, int EP> static void p1h_go(const LaunchArgs &a, dim3 grid) {
-  if constexpr (is_pow2(M)) {
-    if (a.dst_half == 2) {
-      static int occ = 0;
-      const unsigned pg = persist_grid(k_p1hp<T, M, EP>, DstHalf<M, EP>::NT, (long)grid.x, a.cus, occ);
-      hipLaunchKernelGGL((k_p1hp<T, M, EP>), dim3(pg), dim3(DstHalf<M, EP>::NT), 0, a.stream, a.g, (const T *)a.q,
-                         (T *)a.what, (const cx<T> *)a.twxh, (const cx<T> *)a.twx);
-      return;
-    }
+  if (a.dst_half == 2) {
+    static int occ = 0;
+    const unsigned pg = persist_grid(k_p1hp<T, M, EP>, DstHalf<M, EP>::NT, (long)grid.x, a.cus, occ);
+    hipLaunchKernelGGL((k_p1hp<T, M, EP>), dim3(pg), dim3(DstHalf<M, EP>::NT), 0, a.stream, a.g, (const T *)a.q,
+                       (T *)a.what, (const cx<T> *)a.twxh, (const cx<T> *)a.twx);
+    return;
   }
   hipLaunchKernelGGL((k_p1h<T, M, EP>), grid, dim3(DstHalf<M, EP>::NT), 0, a.stream, a.g, (const T *)a.q,
                      (T *)a.what, (const cx<T> *)a.twxh, (const cx<T> *)a.twx);
 }
 
 template <int M, int EP> static void p3h_go(const LaunchArgs &a, dim3 grid) {
-  if constexpr (is_pow2(M)) {
-    if (a.dst_half == 2) {
-      static int occ = 0;
-      const unsigned pg = persist_grid(k_p3hp<T, M, EP>, DstHalf<M, EP>::NT, (long)grid.x, a.cus, occ);
-      hipLaunchKernelGGL((k_p3hp<T, M, EP>), dim3(pg), dim3(DstHalf<M, EP>::NT), 0, a.stream, a.g,
-                         (const T *)a.what, (T *)a.psi, (T *)a.q, (T)a.ring_c, (const cx<T> *)a.twxh,
-                         (const cx<T> *)a.twx);
-      return;
-    }
+  if (a.dst_half == 2) {
+    static int occ = 0;
+    const unsigned pg = persist_grid(k_p3hp<T, M, EP>, DstHalf<M, EP>::NT, (long)grid.x, a.cus, occ);
+    hipLaunchKernelGGL((k_p3hp<T, M, EP>), dim3(pg), dim3(DstHalf<M, EP>::NT), 0, a.stream, a.g,
+                       (const T *)a.what, (T *)a.psi, (T *)a.q, (T)a.ring_c, (const cx<T> *)a.twxh,
+                       (const cx<T> *)a.twx);
+    return;
   }
   hipLaunchKernelGGL((k_p3h<T, M, EP>), grid, dim3(DstHalf<M, EP>::NT), 0, a.stream, a.g, (const T *)a.what,
                      (T *)a.psi, (T *)a.q, (T)a.ring_c, (const cx<T> *)a.twxh, (const cx<T> *)a.twx);
