@@ -579,11 +579,12 @@ k_p3h(Geo g, const T *__restrict__ what, T *__restrict__ psi, T *__restrict__ q,
 // k_p1h / k_p3h from persistent blocks.  A block walks the items
 // it = blockIdx.x, blockIdx.x + gridDim.x, ... (a row and layer pair).  The
 // rows of item it + gridDim.x are loaded into registers while item it is
-// transformed; the twiddles and the pre-pass sines come from two-level LDS
-// tables filled once per block, so the global traffic inside the loop is the
-// row loads and stores alone and no wait on a table load also waits on the
-// next item's rows.  Each item's row pair is staged in LDS (entry x holds z_x,
-// entry 0 unused), from where the pre pass reads z_p and z_{M-p}: both runs of
+// transformed; the twiddles (two-level tables for powers of two, the full
+// table for 3 2^k) and the pre-pass sines come from LDS tables filled once per
+// block, so the global traffic inside the loop is the row loads and stores
+// alone and no wait on a table load also waits on the next item's rows.  Each
+// item's row pair is staged in LDS (entry x holds z_x, entry 0 the zero z_0),
+// from where the pre pass reads z_p and z_{M-p}: both runs of
 // 16 lanes then start on a multiple of 16 entries, where the one-per-16 pad
 // keeps them on distinct banks.  (The column pass P2
 // stays one block per item: its two transforms fill the register file, and
