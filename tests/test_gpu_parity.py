@@ -277,9 +277,9 @@ def test_p4_stream_paths(prec, hyper):
 def test_dst_half_vs_full(shape):
     """Sine transforms through half-length FFTs (one block per row pair, and
     pipelined persistent blocks) against the odd-extension FFTs (FP64, 6
-    steps, and the oracle), on power-of-two and 3*2^k grids; a 3*2^k length
-    runs the one-block form under the pipelined setting, so the mixed grids
-    pair both kinds of kernels in one step."""
+    steps, and the oracle), on power-of-two and 3*2^k grids: the row passes
+    of a 3*2^k length run the pipelined blocks with the LDS Stockham, and the
+    mixed grids pair power-of-two and 3*2^k transforms in one step."""
     NX, NY, nl = shape
     p = _small(NX, nl, NY=NY)
     _announce("dst half vs full", shape)
@@ -327,6 +327,38 @@ def _big_case():
         o.close()
         _BIG.update(p=p, modes=modes, q0=q0, psi_o=psi_o, q_o=q_o)
     return _BIG
+
+
+def test_pipelined_rows_3x2k_many_items():
+    """Pipelined row transforms of a 3*2^k length (half length 1536, the LDS
+    Stockham with its twiddle table in LDS) with several items per persistent
+    block: 1536^2 x 3 layers (3070 items), FP64 inversion and 3 steps against
+    the oracle, and the one-block form against the pipelined one."""
+    p = _small(1536, 3, Lx=3000e3, nu_hyper=1e4, ic_lambda0=150e3)
+    _announce("pipelined 3*2^k rows, many items per block", p.NX, p.nl)
+    modes = Q.eigmode(p)
+    q0 = Q.make_ic(p, modes)[1]
+    o = O.Oracle(p)
+    o.set_modes(*modes)
+    o.set_q(q0)
+    o.invert_pv()
+    psi_o = o.field(O.PSI).copy()
+    o.run(3)
+    q_o = o.field(O.Q).copy()
+    o.close()
+    out = {}
+    for form in (1, 2):
+        m = Q.Model(p, Q.FP64, modes)
+        m.tune_set(Q.TUNE_DST, form)
+        m.set_q(q0)
+        m.invert_pv()
+        psi = m.psi
+        m.run(3)
+        out[form] = (psi, m.q)
+        m.close()
+        assert rel(psi, psi_o) < 1e-12, form
+        assert rel(out[form][1], q_o) < 1e-12, form
+    assert rel(out[2][0], out[1][0]) < 1e-13
 
 
 @pytest.mark.parametrize("fft_e", [8, 16])
