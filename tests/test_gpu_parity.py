@@ -329,12 +329,14 @@ def _big_case():
     return _BIG
 
 
-def test_pipelined_rows_3x2k_many_items():
-    """Pipelined row transforms of a 3*2^k length (half length 1536, the LDS
-    Stockham with its twiddle table in LDS) with several items per persistent
-    block: 1536^2 x 3 layers (3070 items), FP64 inversion and 3 steps against
-    the oracle, and the one-block form against the pipelined one."""
-    p = _small(1536, 3, Lx=3000e3, nu_hyper=1e4, ic_lambda0=150e3)
+@pytest.mark.parametrize("NX,nl", [(1536, 3), (3072, 2)])
+def test_pipelined_rows_3x2k_many_items(NX, nl):
+    """Pipelined row transforms of a 3*2^k length (half lengths 1536 and 3072,
+    the LDS Stockham with its twiddle table in LDS) with several items per
+    persistent block: 1536^2 x 3 layers (3070 items) and 3072^2 x 2 (3071),
+    FP64 inversion and 3 steps against the oracle, and the one-block form
+    against the pipelined one."""
+    p = _small(NX, nl, Lx=2e6 * NX / 1024, nu_hyper=1e4, ic_lambda0=150e3)
     _announce("pipelined 3*2^k rows, many items per block", p.NX, p.nl)
     modes = Q.eigmode(p)
     q0 = Q.make_ic(p, modes)[1]
