@@ -14,6 +14,12 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 NL_MAX = 1000
 Q, PSI, TOPO, F0, F1, F2, LAPN, FORC = 1, 2, 3, 4, 5, 6, 7, 8
 
+# planted mistakes of orc_mutate (qgw_oracle.h ORC_MUT_*), for tests that show
+# which mistake each pin catches
+MUT = dict(none=0, beta_sign=1, beta_axis=2, nu_psi=3, wind_layer=4, wind_dh=5, wind_lx=6, drag_dh=7,
+           drag_half=8, drag_q=9, topo_swap=10, topo_layer=11, bc_factor=12, bc_order=13, topo_lx=14,
+           shelf_width=15)
+
 _dp = ctypes.POINTER(ctypes.c_double)
 
 
@@ -81,6 +87,7 @@ def lib():
         L.orc_check_timestep.argtypes = [ctypes.POINTER(OrcParams), _dp, _dp]
         L.orc_jacobian_field.argtypes = [vp, _dp, _dp, _dp]
         L.orc_laplacian_field.argtypes = [vp, _dp, _dp]
+        L.orc_mutate.argtypes = [ctypes.c_int]
         L.orc_threads.argtypes = [ctypes.c_int]
         L.orc_threads.restype = ctypes.c_int
         _lib = L
@@ -184,6 +191,15 @@ class Oracle:
         if self.L.orc_set_forcing4d(self.s, r.shape[0], float(dt_forc), _p(r)) != 0:
             raise ValueError("bad forcing records")
 
+    def adjust_bc(self, field, field_type):
+        """orc_adjust_bc on a copy of `field` (storage shape), reading the
+        oracle's current psi; returns the copy."""
+        f = np.ascontiguousarray(field, dtype=np.float64).copy()
+        if f.shape != self.shape:
+            raise ValueError("field must have shape %s" % (self.shape,))
+        self.L.orc_adjust_bc(self.s, _p(f), int(field_type))
+        return f
+
     def calc_diff(self):
         self.L.orc_calc_diff(self.s)
 
@@ -272,6 +288,11 @@ def check_timestep(p):
     dt, DT = ctypes.c_double(), ctypes.c_double()
     lib().orc_check_timestep(ctypes.byref(o), ctypes.byref(dt), ctypes.byref(DT))
     return dt.value, DT.value
+
+
+def mutate(name):
+    """Plant one of the mistakes of MUT in the oracle ("none" restores it)."""
+    lib().orc_mutate(MUT[name])
 
 
 def threads(n=0):

@@ -24,6 +24,12 @@ static const double PI_REF = 3.141592653589793; /* src/qg.c:44 */
 static const double HUGE_REF = 1e30;            /* src/qg.c:43 */
 static const double TEPS = 1e-9;                /* src/timestep.h:16 */
 
+/* A planted mistake (ORC_MUT_*), for the tests that show which mistake each
+ * pin of the right-hand side, the wall ring and the topography catches.
+ * 0, the default, is the faithful restatement. */
+static int g_mut = ORC_MUT_NONE;
+void orc_mutate(int id) { g_mut = id; }
+
 int orc_threads(int n) {
 #ifdef _OPENMP
   if (n > 0) omp_set_num_threads(n);
@@ -416,10 +422,11 @@ orc *orc_create(const orc_params *p, int *status) {
   s->topo = (double *)calloc((size_t)s->Nxp2 * s->Nyp2, sizeof(double));
   for (int j = 0; j < s->Nyp2; j++)
     for (int i = 0; i < s->Nxp2; i++) {
-#define SHELF(x, d) (1 - exp(-((x) * (x)) / (2 * ((d) * (d)))))
+#define SHELF(x, d) (1 - exp(-((x) * (x)) / ((g_mut == ORC_MUT_SHELF_WIDTH ? 1 : 2) * ((d) * (d)))))
+      double Ly_top = g_mut == ORC_MUT_TOPO_LX ? p->Lx : s->Ly;
       s->topo[IDX(s, i, j, 0)] =
           p->h_topo * (1 - SHELF(s->X[i], p->w_topo) * SHELF(p->Lx - s->X[i], p->w_topo) *
-                               SHELF(s->Y[j], p->w_topo) * SHELF(s->Ly - s->Y[j], p->w_topo));
+                               SHELF(s->Y[j], p->w_topo) * SHELF(Ly_top - s->Y[j], p->w_topo));
 #undef SHELF
     }
   s->nu_hyper = pow(-1, p->n_hyper / 2 - 1) * fabs(p->nu_hyper);
@@ -500,23 +507,30 @@ void orc_adjust_bc(orc *s, double *field, int field_type) {
   }
   double psi_bc = 0.;
   double *psi = s->psi;
-  double c = 2 * s->p.bc_fac / (s->Delta * s->Delta);
+  double c = (g_mut == ORC_MUT_BC_FACTOR ? 1 : 2) * s->p.bc_fac / (s->Delta * s->Delta);
   for (int k = 0; k < s->p.nl; k++) {
-    for (int i = 0; i < s->Nxp2; i++) {
-      int j = 0;
-      field[IDX(s, i, j, k)] = (field_type ? c * (psi[IDX(s, i, j + 1, k)] - psi_bc) : psi_bc);
-    }
-    for (int i = 0; i < s->Nxp2; i++) {
-      int j = s->Nyp1;
-      field[IDX(s, i, j, k)] = (field_type ? c * (psi[IDX(s, i, j - 1, k)] - psi_bc) : psi_bc);
-    }
-    for (int j = 0; j < s->Nyp2; j++) {
-      int i = 0;
-      field[IDX(s, i, j, k)] = (field_type ? c * (psi[IDX(s, i + 1, j, k)] - psi_bc) : psi_bc);
-    }
-    for (int j = 0; j < s->Nyp2; j++) {
-      int i = s->Nxp1;
-      field[IDX(s, i, j, k)] = (field_type ? c * (psi[IDX(s, i - 1, j, k)] - psi_bc) : psi_bc);
+    for (int pass = 0; pass < 2; pass++) {
+      /* S and N first, then W and E, which therefore own the corners */
+      int sn = (pass == 0) != (g_mut == ORC_MUT_BC_ORDER);
+      if (sn) {
+        for (int i = 0; i < s->Nxp2; i++) {
+          int j = 0;
+          field[IDX(s, i, j, k)] = (field_type ? c * (psi[IDX(s, i, j + 1, k)] - psi_bc) : psi_bc);
+        }
+        for (int i = 0; i < s->Nxp2; i++) {
+          int j = s->Nyp1;
+          field[IDX(s, i, j, k)] = (field_type ? c * (psi[IDX(s, i, j - 1, k)] - psi_bc) : psi_bc);
+        }
+      } else {
+        for (int j = 0; j < s->Nyp2; j++) {
+          int i = 0;
+          field[IDX(s, i, j, k)] = (field_type ? c * (psi[IDX(s, i + 1, j, k)] - psi_bc) : psi_bc);
+        }
+        for (int j = 0; j < s->Nyp2; j++) {
+          int i = s->Nxp1;
+          field[IDX(s, i, j, k)] = (field_type ? c * (psi[IDX(s, i - 1, j, k)] - psi_bc) : psi_bc);
+        }
+      }
     }
   }
 }
@@ -679,32 +693,45 @@ void orc_rhs(orc *s, int which_out) {
   orc_invert_pv(s);
   orc_calc_diff(s);
   double Delta = s->Delta;
+  /* planted mistakes (g_mut): the faithful terms are the ones with g_mut = 0 */
+  const double bsign = g_mut == ORC_MUT_BETA_SIGN ? -1 : 1;
+  const int bdi = g_mut == ORC_MUT_BETA_AXIS ? 0 : 1, bdj = 1 - bdi;
+  const double *nuf = g_mut == ORC_MUT_NU_PSI ? psi : q;
   for (int k = 0; k < nl; k++) {
 #pragma omp parallel for schedule(static)
     for (int j = 1; j < s->Nyp1; j++)
       for (int i = 1; i < s->Nxp1; i++) {
         f1[IDX(s, i, j, k)] = -jac(s, psi, q, i, j, k, k) -
-                              (p->beta * (psi[IDX(s, i + 1, j, k)] - psi[IDX(s, i - 1, j, k)]) / (2 * Delta)) +
-                              p->nu * (q[IDX(s, i + 1, j, k)] + q[IDX(s, i - 1, j, k)] + q[IDX(s, i, j + 1, k)] +
-                                       q[IDX(s, i, j - 1, k)] - 4 * q[IDX(s, i, j, k)]) /
+                              bsign * (p->beta * (psi[IDX(s, i + bdi, j + bdj, k)] - psi[IDX(s, i - bdi, j - bdj, k)]) /
+                                       (2 * Delta)) +
+                              p->nu * (nuf[IDX(s, i + 1, j, k)] + nuf[IDX(s, i - 1, j, k)] + nuf[IDX(s, i, j + 1, k)] +
+                                       nuf[IDX(s, i, j - 1, k)] - 4 * nuf[IDX(s, i, j, k)]) /
                                   (Delta * Delta) +
                               s->nu_hyper * s->lapn[IDX(s, i, j, k)];
       }
   }
   double dhb = p->dh[nl - 1];
+  const int wl = g_mut == ORC_MUT_WIND_LAYER ? nl - 1 : 0;
+  const double wdh = g_mut == ORC_MUT_WIND_DH ? p->dh[nl - 1] : p->dh[0];
+  const double wLy = g_mut == ORC_MUT_WIND_LX ? p->Lx : s->Ly;
+  const double ddh = g_mut == ORC_MUT_DRAG_DH ? p->dh[0] : dhb;
+  const double dhalf = g_mut == ORC_MUT_DRAG_HALF ? 1 : 2;
+  const double *dfield = g_mut == ORC_MUT_DRAG_Q ? q : psi;
+  const int tk = g_mut == ORC_MUT_TOPO_LAYER ? 0 : nl - 1;
 #pragma omp parallel for schedule(static)
   for (int j = 1; j < s->Nyp1; j++)
     for (int i = 1; i < s->Nxp1; i++) {
       /* forcing_q (src/domain.h:33) */
-      f1[IDX(s, i, j, 0)] += (-p->tau0 / p->dh[0] * p->forc_mode * PI_REF / s->Ly *
-                              sin(p->forc_mode * PI_REF * (s->Y[j]) / s->Ly));
+      f1[IDX(s, i, j, wl)] += (-p->tau0 / wdh * p->forc_mode * PI_REF / wLy *
+                               sin(p->forc_mode * PI_REF * (s->Y[j]) / wLy));
       int k = nl - 1;
+      double jt = g_mut == ORC_MUT_TOPO_SWAP ? jac(s, s->topo, psi, i, j, 0, tk) : jac(s, psi, s->topo, i, j, tk, 0);
       f1[IDX(s, i, j, nl - 1)] +=
-          -p->hEkb * p->f0 / (2 * dhb) *
-              (psi[IDX(s, i + 1, j, k)] + psi[IDX(s, i - 1, j, k)] + psi[IDX(s, i, j + 1, k)] +
-               psi[IDX(s, i, j - 1, k)] - 4 * psi[IDX(s, i, j, k)]) /
+          -p->hEkb * p->f0 / (dhalf * ddh) *
+              (dfield[IDX(s, i + 1, j, k)] + dfield[IDX(s, i - 1, j, k)] + dfield[IDX(s, i, j + 1, k)] +
+               dfield[IDX(s, i, j - 1, k)] - 4 * dfield[IDX(s, i, j, k)]) /
               (Delta * Delta) -
-          jac(s, psi, s->topo, i, j, nl - 1, 0) * p->f0 / dhb;
+          jt * p->f0 / dhb;
     }
 }
 

@@ -42,7 +42,15 @@
  * the dynamics vanish exactly; the stochastic ring against a hand-counted
  * ring and the integer geometry of the selection rule, its c2r against
  * numpy.fft.irfft2, its energy injection against the sigma^2 rate of
- * doc/forcing.org.  No function is unpinned.
+ * doc/forcing.org.  Each term of orc_rhs alone against a closed form on a
+ * sine mode (beta as the centred x-difference of f = f0 + beta*y, nu lap q
+ * through the 5-point eigenvalue, the wind curl of doc/usage.org at worked
+ * points, the drag as rate r = hEkb*f0/(2 dh[nl-1]) on the bottom vorticity,
+ * J(psi, topo) of a linear ramp as a topographic beta), their superposition,
+ * the bc_fac > 0 wall ring against Thom's no-slip wall vorticity 2 psi_1/Delta^2
+ * and the W/E corner ownership, and the bathtub topography at worked points;
+ * each of these pins is shown to fail under the planted mistakes ORC_MUT_*.
+ * No function is unpinned.
  */
 #ifndef QGW_ORACLE_H
 #define QGW_ORACLE_H
@@ -76,6 +84,26 @@ typedef struct orc orc;
 #define ORC_F2 6
 #define ORC_LAPN 7 /* lap_n_diff */
 #define ORC_FORC 8 /* stochastic forcing field of the last step (layer 0) */
+
+/* Planted mistakes for orc_mutate (tests only): each names one plausible
+ * misreading of the reference that a pin in tests/ must catch. */
+#define ORC_MUT_NONE 0
+#define ORC_MUT_BETA_SIGN 1    /* +beta*psi_x instead of -beta*psi_x */
+#define ORC_MUT_BETA_AXIS 2    /* beta term differenced along y */
+#define ORC_MUT_NU_PSI 3       /* nu * lap(psi) instead of nu * lap(q) */
+#define ORC_MUT_WIND_LAYER 4   /* wind forcing on the bottom layer */
+#define ORC_MUT_WIND_DH 5      /* wind forcing divided by dh[nl-1] */
+#define ORC_MUT_WIND_LX 6      /* wind forcing with Lx in place of Ly */
+#define ORC_MUT_DRAG_DH 7      /* drag divided by dh[0] */
+#define ORC_MUT_DRAG_HALF 8    /* drag without the factor 1/2 */
+#define ORC_MUT_DRAG_Q 9       /* drag on lap(q) instead of lap(psi) */
+#define ORC_MUT_TOPO_SWAP 10   /* J(topo, psi) instead of J(psi, topo) */
+#define ORC_MUT_TOPO_LAYER 11  /* J(psi[0], topo) instead of J(psi[nl-1], topo) */
+#define ORC_MUT_BC_FACTOR 12   /* wall ring bc_fac/Delta^2 instead of 2 bc_fac/Delta^2 */
+#define ORC_MUT_BC_ORDER 13    /* S/N written after W/E (S/N own the corners) */
+#define ORC_MUT_TOPO_LX 14     /* bathtub shelf at Lx - y instead of Ly - y */
+#define ORC_MUT_SHELF_WIDTH 15 /* shelf exp(-x^2/d^2) instead of exp(-x^2/(2 d^2)) */
+void orc_mutate(int id);  /* set before orc_create for the topography ones */
 
 orc *orc_create(const orc_params *p, int *status);
 void orc_destroy(orc *s);
