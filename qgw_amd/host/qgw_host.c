@@ -307,62 +307,77 @@ int qgwh_gamma(const qgw_params *p, double *G) {
   return QGW_OK;
 }
 
-/* Symmetric tridiagonal eigen-solver: implicit QL with Wilkinson shifts
- * (Golub & Van Loan, Matrix Computations, sec. 8.3; the classic tql2 scheme).
- * d[0..n-1] diagonal, e[0..n-2] off-diagonal; on return d holds the
- * eigenvalues and column k of Z (row-major n*n, Z[i*n+k]) the eigenvector. */
-static int tridiag_ql(int n, double *d, double *e_in, double *Z) {
+/* Symmetric tridiagonal eigen-solver: the implicit symmetric QR algorithm
+ * with Wilkinson shifts (Golub & Van Loan, Matrix Computations, 4th ed.,
+ * Algorithms 8.3.2 and 8.3.3), written from the mathematics below.
+ *
+ * T has diagonal d[0..n-1] and sub-diagonal e[0..n-2] (e[k] = T[k+1][k]).
+ * Each sweep works on the lowest unreduced block p..q: the shift mu is the
+ * eigenvalue of the block's trailing 2x2 closer to its last entry; a plane
+ * rotation R_k = [[c, s], [-s, c]] in rows/columns (k, k+1) is applied as
+ * T <- R_k T R_k^T, the first one chosen to zero the second entry of
+ * (d[p] - mu, e[p]), each later one to zero the bulge T[k+1][k-1] left by
+ * the one before.  The rotations accumulate into Z <- Z R_k^T, so on return
+ * d holds the eigenvalues and column k of Z (row-major, Z[i*n+k]) the unit
+ * eigenvector of d[k].  An off-diagonal entry below eps*(|d_k| + |d_k+1|)
+ * is set to zero (deflation). */
+static void rot_cols(double *Z, int n, int k, double c, double s) {
+  for (int r = 0; r < n; r++) {
+    double a = Z[r * n + k], b = Z[r * n + k + 1];
+    Z[r * n + k] = c * a + s * b;
+    Z[r * n + k + 1] = -s * a + c * b;
+  }
+}
+
+static int tridiag_eig(int n, double *d, const double *e_in, double *Z) {
+  const double eps = 2.2204460492503131e-16;
   double *e = (double *)calloc((size_t)n + 1, sizeof(double));
   if (!e) return QGW_ERR_NOMEM;
-  for (int i = 0; i < n - 1; i++) e[i] = e_in[i];
-  e[n - 1] = 0.0;
+  for (int k = 0; k < n - 1; k++) e[k] = e_in[k];
   for (int i = 0; i < n * n; i++) Z[i] = 0.0;
   for (int i = 0; i < n; i++) Z[i * n + i] = 1.0;
-  for (int l = 0; l < n; l++) {
-    int iter = 0;
-    for (;;) {
-      int m;
-      for (m = l; m < n - 1; m++) {
-        double dd = fabs(d[m]) + fabs(d[m + 1]);
-        if (fabs(e[m]) <= 1e-300 + 2.2204460492503131e-16 * dd) break;
+  int q = n - 1, sweeps = 0;
+  while (q > 0) {
+    /* deflate at the bottom */
+    if (fabs(e[q - 1]) <= eps * (fabs(d[q - 1]) + fabs(d[q])) || e[q - 1] == 0.0) {
+      e[q - 1] = 0.0;
+      q--;
+      continue;
+    }
+    /* top of the unreduced block ending at q */
+    int p = q - 1;
+    while (p > 0 && fabs(e[p - 1]) > eps * (fabs(d[p - 1]) + fabs(d[p]))) p--;
+    if (p > 0) e[p - 1] = 0.0;
+    if (++sweeps > 60 * n) {
+      free(e);
+      return QGW_ERR_NUMERIC;
+    }
+    /* Wilkinson shift from [[d[q-1], e[q-1]], [e[q-1], d[q]]] */
+    double half = 0.5 * (d[q - 1] - d[q]);
+    double b2 = e[q - 1] * e[q - 1];
+    double mu = d[q] - b2 / (half + (half >= 0 ? 1.0 : -1.0) * hypot(half, e[q - 1]));
+    double x = d[p] - mu, z = e[p], bulge = 0.0;
+    for (int k = p; k < q; k++) {
+      if (k > p) {
+        x = e[k - 1];
+        z = bulge;
       }
-      if (m == l) break;
-      if (++iter > 200) {
-        free(e);
-        return QGW_ERR_NUMERIC;
+      double r = hypot(x, z);
+      double c = 1.0, s = 0.0;
+      if (r != 0.0) {
+        c = x / r;
+        s = z / r;
       }
-      double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
-      double r = hypot(g, 1.0);
-      g = d[m] - d[l] + e[l] / (g + (g >= 0 ? fabs(r) : -fabs(r)));
-      double s = 1.0, c = 1.0, pp = 0.0;
-      int i;
-      for (i = m - 1; i >= l; i--) {
-        double f = s * e[i];
-        double b = c * e[i];
-        r = hypot(f, g);
-        e[i + 1] = r;
-        if (r == 0.0) {
-          d[i + 1] -= pp;
-          e[m] = 0.0;
-          break;
-        }
-        s = f / r;
-        c = g / r;
-        g = d[i + 1] - pp;
-        r = (d[i] - g) * s + 2.0 * c * b;
-        pp = s * r;
-        d[i + 1] = g + pp;
-        g = c * r - b;
-        for (int k = 0; k < n; k++) {
-          double zf = Z[k * n + i + 1];
-          Z[k * n + i + 1] = s * Z[k * n + i] + c * zf;
-          Z[k * n + i] = c * Z[k * n + i] - s * zf;
-        }
+      if (k > p) e[k - 1] = r; /* the bulge is annihilated */
+      double a = d[k], b = e[k], cc = d[k + 1];
+      d[k] = c * c * a + 2.0 * c * s * b + s * s * cc;
+      d[k + 1] = s * s * a - 2.0 * c * s * b + c * c * cc;
+      e[k] = c * s * (cc - a) + (c * c - s * s) * b;
+      if (k + 1 < q) {
+        bulge = s * e[k + 1];
+        e[k + 1] = c * e[k + 1];
       }
-      if (r == 0.0 && i >= l) continue;
-      d[l] -= pp;
-      e[l] = g;
-      e[m] = 0.0;
+      rot_cols(Z, n, k, c, s);
     }
   }
   free(e);
@@ -408,7 +423,7 @@ int qgwh_eigmode(const qgw_params *p, double *cl2m, double *cm2l, double *iRd2) 
     double s = (G[nl * l + l + 1] < 0) ? -1.0 : 1.0;
     e[l] = s * sqrt(fabs(prod));
   }
-  st = tridiag_ql(nl, d, e, Z);
+  st = tridiag_eig(nl, d, e, Z);
   if (st) goto done;
   /* ascending order (src/eigmode.h:154-185); a stable insertion sort */
   for (int i = 0; i < nl; i++) ord[i] = i;

@@ -143,9 +143,9 @@ def test_host_modes_match_oracle_and_golden(mk):
     assert np.abs(cl2m.reshape(nl, nl) @ cm2l.reshape(nl, nl) - np.eye(nl)).max() < 1e-12
 
 
-def test_host_modes_many_layers_and_single_layer():
+@pytest.mark.parametrize("nl", [32, 200])
+def test_host_modes_many_layers_and_single_layer(nl):
     rng = np.random.default_rng(9)
-    nl = 32
     dh = list(rng.uniform(50, 400, nl))
     N2 = list(rng.uniform(1e-6, 1e-4, nl - 1))
     p = Q.default_params(NX=8, NY=8, nl=nl, Lx=1e6, dh=dh, N2=N2, f0=1e-4)
@@ -154,9 +154,14 @@ def test_host_modes_many_layers_and_single_layer():
     assert np.abs(A @ B - np.eye(nl)).max() < 1e-10
     G = Q.gamma(p)
     lam = np.concatenate([[0.0], iRd2[1:]])
-    assert np.abs(G @ B - B * lam).max() < 1e-8 * np.abs(G).max() * np.abs(B).max()
+    assert np.abs(G.reshape(nl, nl) @ B - B * lam).max() < 1e-8 * np.abs(G).max() * np.abs(B).max()
     big = np.abs(B[0]) > 1e-8  # bottom-trapped modes can have a vanishing surface value
     assert (np.diff(iRd2) > 0).all() and (B[0][big] > 0).all()
+    # eigenvalues against LAPACK's symmetric solver (numpy) on D^1/2 Gamma D^-1/2
+    sq = np.sqrt(np.asarray(dh))
+    S = (sq[:, None] * G.reshape(nl, nl)) / sq[None, :]
+    w = np.linalg.eigvalsh(0.5 * (S + S.T))
+    assert np.allclose(iRd2[1:], w[1:], rtol=1e-9, atol=1e-12 * np.abs(w).max())
     p1 = Q.default_params(NX=8, NY=8, nl=1, Lx=1e6, dh=[1000.0], Ld=30e3)
     c1, m1, i1 = Q.eigmode(p1)
     assert (c1[0], m1[0]) == (1.0, 1.0) and i1[0] == pytest.approx(1 / 30e3 ** 2)
