@@ -403,7 +403,10 @@ def main():
                    "interior_points": g.n * g.m, "graph_replays": replays, "tuning": tunes},
         "roofline": {"bound": "hbm", "kernel": dom, "kernel_name": tkernel, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "algorithmic_bytes": alg_bytes,
+                     "algorithmic_bytes": alg_bytes, "ms_per_launch": per_launch[dom],
+                     "timing_source": "HIP events around each launch of the kernel on the stream it runs on, "
+                                      "averaged over the %d launches of --profile-steps %d eager steps run "
+                                      "after the timed steps" % (passes[dom][1], args.profile_steps),
                      "traffic_source": "rocprofv3 FETCH_SIZE + WRITE_SIZE, profiles/traffic.json"},
         "hbm_copy_GBs": None,
         "step_algorithmic_GBs": step_gbs,

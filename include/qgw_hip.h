@@ -241,8 +241,14 @@ int qgw_batch_download(qgw_batch *b, int member, int field, double *host);
 int qgw_batch_invert_pv(qgw_batch *b);
 /* nsteps of timestep (src/timestep.h:114) for every member. */
 int qgw_batch_step(qgw_batch *b, int nsteps);
-/* As qgw_batch_step, timed with HIP events on the batch stream. */
+/* As qgw_batch_step, timed with HIP events on the batch stream.  The graph of
+ * the current state is captured and instantiated before the start event, so
+ * the timed region holds only launches. */
 int qgw_batch_time_step(qgw_batch *b, int nsteps, double *ms);
+/* Capture and instantiate the graph for the current state now, instead of at
+ * the first qgw_batch_step that replays it (no step is run; nothing is done
+ * when the batch cannot replay a graph, e.g. while profiling). */
+int qgw_batch_prepare(qgw_batch *b);
 int qgw_batch_synchronize(qgw_batch *b);
 int qgw_batch_diagnostics(qgw_batch *b, int member, double *E_layer, double *Z_layer, double *E_tot,
                           double *Z_tot);

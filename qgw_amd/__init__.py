@@ -35,6 +35,7 @@ TUNE_DST = 4
 TUNE_FFT_E = 5
 TUNE_FFT_E_COL = 6
 P4_TILES = ["64x8", "64x16", "64x32", "128x16", "stream"]
+TUNE_KNOBS = (TUNE_P4_TILE, TUNE_P4_WAVES, TUNE_T_KERNEL, TUNE_DST, TUNE_FFT_E, TUNE_FFT_E_COL)
 _TUNE_CACHE = {}
 
 QGW_OK = 0
@@ -189,7 +190,7 @@ def hip_lib():
             ("qgw_batch_set_modes", [vp, ci, _dp, _dp, _dp]),
             ("qgw_batch_upload", [vp, ci, ci, _dp]), ("qgw_batch_download", [vp, ci, ci, _dp]),
             ("qgw_batch_invert_pv", [vp]), ("qgw_batch_step", [vp, ci]),
-            ("qgw_batch_time_step", [vp, ci, _dp]), ("qgw_batch_synchronize", [vp]),
+            ("qgw_batch_time_step", [vp, ci, _dp]), ("qgw_batch_synchronize", [vp]), ("qgw_batch_prepare", [vp]),
             ("qgw_batch_diagnostics", [vp, ci, _dp, _dp, _dp, _dp]),
             ("qgw_batch_get_time", [vp, _dp, ip]), ("qgw_batch_use_graph", [vp, ci]),
             ("qgw_batch_graph_launches", [vp, ip]), ("qgw_batch_autotune", [vp, ci, ip]),
@@ -204,6 +205,72 @@ def hip_lib():
         lib.qgw_batch_last_error.restype = ctypes.c_char_p
         _hip = lib
     return _hip
+
+
+# ---------------------------------------------------------------------------
+# autotuner choices, persisted
+# ---------------------------------------------------------------------------
+
+def tune_cache_path():
+    """JSON file of the autotuner's choices: $QGW_TUNE_CACHE, else
+    ~/.cache/qgw_amd/tune.json."""
+    p = os.environ.get("QGW_TUNE_CACHE")
+    if p:
+        return p
+    return os.path.join(os.path.expanduser("~"), ".cache", "qgw_amd", "tune.json")
+
+
+def device_name():
+    """Name of HIP device 0 (hipDeviceGetName), part of every tuning key."""
+    for name in ("libamdhip64.so", "/opt/rocm/lib/libamdhip64.so"):
+        try:
+            hip = ctypes.CDLL(name)
+            break
+        except OSError:
+            hip = None
+    if hip is None:
+        return "unknown"
+    buf = ctypes.create_string_buffer(256)
+    if hip.hipDeviceGetName(buf, 256, 0) != 0:
+        return "unknown"
+    return buf.value.decode(errors="replace").strip() or "unknown"
+
+
+def tune_key(precision, n, m, layers, n_hyper, hyper_on):
+    """Key of one group's tuning: the kernels it launches depend only on the
+    device, the precision form, the grid, the stacked layer count and the
+    depth of the nested Laplacians."""
+    return "%s|prec%d|%dx%d|nl%d|nh%d|%s" % (device_name(), precision, n, m, layers, n_hyper,
+                                             "hv" if hyper_on else "nohv")
+
+
+def tune_cache_load():
+    """The persisted choices {key: {knob: value, ...}} (empty when absent or unreadable)."""
+    import json
+    try:
+        with open(tune_cache_path()) as f:
+            d = json.load(f)
+        return d if isinstance(d, dict) else {}
+    except (OSError, ValueError):
+        return {}
+
+
+def tune_cache_store(entries):
+    """Merge {key: choice} into the JSON cache (written to a temporary file and
+    renamed, so a reader never sees a partial file)."""
+    import json
+    import tempfile
+    path = tune_cache_path()
+    d = tune_cache_load()
+    d.update(entries)
+    try:
+        os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
+        fd, tmp = tempfile.mkstemp(dir=os.path.dirname(os.path.abspath(path)), suffix=".tmp")
+        with os.fdopen(fd, "w") as f:
+            json.dump(d, f, indent=1, sort_keys=True)
+        os.replace(tmp, path)
+    except OSError:
+        pass  # an unwritable cache only costs a retune
 
 
 # ---------------------------------------------------------------------------
@@ -566,23 +633,24 @@ class Model:
                 "dst": ["odd-extension", "half-length", "half-length pipelined"][self.tune_get(TUNE_DST)],
                 "fft_e": self.tune_get(TUNE_FFT_E), "fft_e_col": self.tune_get(TUNE_FFT_E_COL)}
 
-    def autotune(self, reps=5):
+    def autotune(self, reps=5, cache=True):
         """Pick the tuning of this model by timing each variant on its own
         state (qgw_autotune copies the state first and restores it after).
-        Choices are cached per shape and form."""
-        key = (self.precision, self.n, self.m, self.nl, self.p.n_hyper, self.p.nu_hyper != 0)
-        if key not in _TUNE_CACHE:
+        Choices are kept per device, form and shape in the JSON file of
+        tune_cache_path(), so later processes reuse them; cache=False retunes."""
+        key = tune_key(self.precision, self.n, self.m, self.nl, self.p.n_hyper, self.p.nu_hyper != 0)
+        choice = tune_cache_load().get(key) if cache else None
+        if choice is None:
             t, ms = ctypes.c_int(), ctypes.c_double()
             self._chk(self.lib.qgw_autotune(self.ctx, reps, ctypes.byref(t), ctypes.byref(ms)), "autotune")
-            _TUNE_CACHE[key] = {TUNE_P4_TILE: t.value, "ms_per_step": ms.value,
-                                TUNE_P4_WAVES: self.tune_get(TUNE_P4_WAVES),
-                                TUNE_T_KERNEL: self.tune_get(TUNE_T_KERNEL),
-                                TUNE_DST: self.tune_get(TUNE_DST), TUNE_FFT_E: self.tune_get(TUNE_FFT_E),
-                                TUNE_FFT_E_COL: self.tune_get(TUNE_FFT_E_COL)}
-        for knob, v in _TUNE_CACHE[key].items():
-            if knob != "ms_per_step":
-                self.tune_set(knob, v)
-        return dict(_TUNE_CACHE[key])
+            choice = {str(k): self.tune_get(k) for k in TUNE_KNOBS}
+            choice["ms_per_step"] = ms.value
+            tune_cache_store({key: choice})
+        else:
+            for k in TUNE_KNOBS:
+                if str(k) in choice:
+                    self.tune_set(k, int(choice[str(k)]))
+        return dict(choice, cached=choice is not None and cache)
 
     def device_bytes(self):
         b = ctypes.c_size_t()
@@ -680,6 +748,10 @@ class Batch:
     def synchronize(self):
         self._chk(self.lib.qgw_batch_synchronize(self.b), "synchronize")
 
+    def prepare(self):
+        """Capture the graph of the current state now (no step is run)."""
+        self._chk(self.lib.qgw_batch_prepare(self.b), "prepare")
+
     def diagnostics(self, member):
         nl = self.shapes[member][0]
         El, Zl = np.zeros(nl), np.zeros(nl)
@@ -701,11 +773,32 @@ class Batch:
         self._chk(self.lib.qgw_batch_graph_launches(self.b, ctypes.byref(c)), "graph_launches")
         return c.value
 
-    def autotune(self, reps=3):
-        ng = self.groups()[0]
-        v = (ctypes.c_int * ng)()
-        self._chk(self.lib.qgw_batch_autotune(self.b, reps, v), "autotune")
-        return [P4_TILES[x] for x in v]
+    def _group_keys(self):
+        ng, gof = self.groups()
+        keys = []
+        for g in range(ng):
+            mem = [i for i in range(len(gof)) if gof[i] == g]
+            p = self.params[mem[0]]
+            nl, m2, n2 = self.shapes[mem[0]]
+            keys.append(tune_key(self.precision, n2 - 2, m2 - 2, nl * len(mem), p.n_hyper, p.nu_hyper != 0))
+        return keys
+
+    def autotune(self, reps=3, cache=True):
+        """Tune every group (as Model.autotune); choices come from and go to
+        the JSON cache of tune_cache_path() unless cache=False."""
+        keys = self._group_keys()
+        known = tune_cache_load() if cache else {}
+        if all(k in known for k in keys):
+            for g, k in enumerate(keys):
+                for knob in TUNE_KNOBS:
+                    if str(knob) in known[k]:
+                        self.tune_set(g, knob, int(known[k][str(knob)]))
+        else:
+            v = (ctypes.c_int * len(keys))()
+            self._chk(self.lib.qgw_batch_autotune(self.b, reps, v), "autotune")
+            tune_cache_store({k: {str(knob): self.tune_get(g, knob) for knob in TUNE_KNOBS}
+                              for g, k in enumerate(keys)})
+        return [self.tuning(g)["p4"] for g in range(len(keys))]
 
     def tune_set(self, group, knob, value):
         self._chk(self.lib.qgw_batch_tune_set(self.b, group, knob, value), "tune_set")

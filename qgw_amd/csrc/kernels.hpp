@@ -1700,6 +1700,11 @@ k_p4s(Geo g, const T *__restrict__ psi, const T *__restrict__ q, T *__restrict__
 // |psi(i+1,j) - psi(i,j)| and |psi(i,j+1) - psi(i,j)| (src/timestep.h:74-87).
 // The maximum of a non-negative double is reduced through its bit pattern.
 // ===========================================================================
+// A maximum that keeps NaN (fmax drops it): a NaN anywhere in psi reaches
+// the reduced value, whose bits (above those of +inf as unsigned integers)
+// then win the atomicMax, so the host sees a non-finite maximum.
+__device__ inline double nan_max(double a, double b) { return (b > a || b != b) ? b : a; }
+
 template <typename T>
 __global__ void __launch_bounds__(256) k_cfl(Geo g, const T *__restrict__ psi, unsigned long long *out) {
   const long I = (long)g.n * g.m * g.nl;
@@ -1712,13 +1717,13 @@ __global__ void __launch_bounds__(256) k_cfl(Geo g, const T *__restrict__ psi, u
     const double c = (double)p[0];
     const double du = fabs((double)p[1] - c);
     const double dv = fabs((double)p[g.P] - c);
-    mx = fmax(mx, fmax(du, dv));
+    mx = nan_max(mx, nan_max(du, dv));
   }
   __shared__ double red[256];
   red[threadIdx.x] = mx;
   __syncthreads();
   for (int s = 128; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+    if ((int)threadIdx.x < s) red[threadIdx.x] = nan_max(red[threadIdx.x], red[threadIdx.x + s]);
     __syncthreads();
   }
   if (threadIdx.x == 0) atomicMax(out, (unsigned long long)__double_as_longlong(red[0]));

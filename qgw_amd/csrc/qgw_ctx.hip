@@ -1207,7 +1207,9 @@ int batch_eager_step(qgw_batch *b) {
   return QGW_OK;
 }
 
+// Destroys the cached graphs; a replay still in flight finishes first.
 void batch_release_graphs(qgw_batch *b) {
+  if (!b->cache.empty() && b->stream) hipStreamSynchronize(b->stream);
   for (auto &e : b->cache) hipGraphExecDestroy(e.exec);
   b->cache.clear();
 }
@@ -1293,17 +1295,24 @@ int batch_capture(qgw_batch *b, hipGraphExec_t *exec) {
   return QGW_OK;
 }
 
+// The graph of the current rotation state, captured on first use.
+int batch_exec(qgw_batch *b, hipGraphExec_t *exec) {
+  const auto sig = batch_sig(b);
+  *exec = nullptr;
+  for (auto &e : b->cache)
+    if (e.sig == sig) *exec = e.exec;
+  if (*exec) return QGW_OK;
+  int st = batch_capture(b, exec);
+  if (st) return st;
+  b->cache.push_back({sig, *exec});
+  return QGW_OK;
+}
+
 int batch_run(qgw_batch *b, int nsteps) {
   while (nsteps >= kGraphSteps && graph_ok(b)) {
-    const auto sig = batch_sig(b);
     hipGraphExec_t exec = nullptr;
-    for (auto &e : b->cache)
-      if (e.sig == sig) exec = e.exec;
-    if (!exec) {
-      int st = batch_capture(b, &exec);
-      if (st) return st;
-      b->cache.push_back({sig, exec});
-    }
+    int st = batch_exec(b, &exec);
+    if (st) return st;
     hipError_t e = hipGraphLaunch(exec, b->stream);
     if (e != hipSuccess) return bfail(b, e, "hipGraphLaunch");
     b->graph_launches++;
@@ -1694,7 +1703,13 @@ int qgw_batch_time_step(qgw_batch *b, int nsteps, double *ms) {
     return bfail(b, e, "hipEventCreate");
   }
   int st = QGW_OK;
-  if ((e = hipEventRecord(e0, b->stream)) != hipSuccess) st = bfail(b, e, "hipEventRecord");
+  // capture (and instantiate) the graph before the timed region: a replay
+  // returns to the rotation state it started from, so one graph serves the run
+  if (nsteps >= kGraphSteps && graph_ok(b)) {
+    hipGraphExec_t exec = nullptr;
+    st = batch_exec(b, &exec);
+  }
+  if (!st && (e = hipEventRecord(e0, b->stream)) != hipSuccess) st = bfail(b, e, "hipEventRecord");
   if (!st) st = batch_run(b, nsteps);
   if (!st && (e = hipEventRecord(e1, b->stream)) != hipSuccess) st = bfail(b, e, "hipEventRecord");
   if (!st && (e = hipEventSynchronize(e1)) != hipSuccess) st = bfail(b, e, "hipEventSynchronize");
@@ -1704,6 +1719,13 @@ int qgw_batch_time_step(qgw_batch *b, int nsteps, double *ms) {
   hipEventDestroy(e0);
   hipEventDestroy(e1);
   return st;
+}
+
+int qgw_batch_prepare(qgw_batch *b) {
+  if (!b) return QGW_ERR_ARG;
+  if (!graph_ok(b)) return QGW_OK;
+  hipGraphExec_t exec = nullptr;
+  return batch_exec(b, &exec);
 }
 
 int qgw_batch_synchronize(qgw_batch *b) {

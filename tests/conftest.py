@@ -9,6 +9,10 @@ if ROOT not in sys.path:
 
 
 def pytest_configure(config):
+    # the autotuner's JSON cache of this test session, apart from the user's
+    if "QGW_TUNE_CACHE" not in os.environ:
+        import tempfile
+        os.environ["QGW_TUNE_CACHE"] = os.path.join(tempfile.mkdtemp(prefix="qgw_tune_"), "tune.json")
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs through libqgw_hip.so)")
     config.addinivalue_line("markers", "slow: longer CPU test")
 

@@ -133,7 +133,7 @@ def test_batch_autotune_preserves_state():
     b2, _, _ = _batch(params, Q.FP32)
     b1.step(4)
     b2.step(4)
-    choice = b1.autotune(reps=2)
+    choice = b1.autotune(reps=2, cache=False)
     assert len(choice) == 3
     for g in range(3):
         for knob in KNOBS:
@@ -166,5 +166,27 @@ def test_model_autotune_preserves_state():
     m1.run(5)
     m2.run(5)
     assert np.array_equal(m1.q, m2.q)
+    m1.close()
+    m2.close()
+
+
+def test_autotune_choices_persist_across_objects():
+    # a second model of the same shape takes the first one's choices from the
+    # JSON cache instead of retuning
+    p = _member(96, 3, 0, 4e-5)
+    _announce("autotune cache", p.NX, p.nl)
+    modes = Q.eigmode(p)
+    q0 = Q.make_ic(p, modes)[1]
+    m1 = Q.Model(p, Q.FP32, modes)
+    m1.set_q(q0)
+    first = m1.autotune(reps=1, cache=False)
+    key = Q.tune_key(Q.FP32, m1.n, m1.m, m1.nl, p.n_hyper, p.nu_hyper != 0)
+    assert key in Q.tune_cache_load()
+    m2 = Q.Model(p, Q.FP32, modes)
+    m2.set_q(q0)
+    again = m2.autotune(reps=1)
+    assert again["cached"]
+    for knob in KNOBS:
+        assert m2.tune_get(knob) == m1.tune_get(knob) == int(first[str(knob)])
     m1.close()
     m2.close()

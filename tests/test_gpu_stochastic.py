@@ -152,7 +152,7 @@ def test_stochastic_autotune_keeps_the_stream():
     b.set_q(q0)
     a.run(3)
     b.run(3)
-    b.autotune(reps=2)
+    b.autotune(reps=2, cache=False)
     a.run(5)
     b.run(5)
     assert rel(b.q, a.q) < 1e-12

@@ -215,3 +215,17 @@ def test_diagnostics_host_matches_oracle():
     assert np.allclose(a[0], b[0], rtol=1e-13) and np.allclose(a[1], b[1], rtol=1e-13)
     assert a[2] == pytest.approx(b[2], rel=1e-13) and a[3] == pytest.approx(b[3], rel=1e-13)
     assert a[2] > 0 and a[3] > 0
+
+
+def test_tune_cache_round_trip(tmp_path, monkeypatch):
+    monkeypatch.setenv("QGW_TUNE_CACHE", str(tmp_path / "sub" / "tune.json"))
+    assert Q.tune_cache_load() == {}
+    k1 = Q.tune_key(Q.FP32, 2047, 2047, 10, 4, True)
+    k2 = Q.tune_key(Q.FP64, 2047, 2047, 10, 4, True)
+    assert k1 != k2 and "2047x2047" in k1 and "nl10" in k1
+    Q.tune_cache_store({k1: {"1": 4, "3": 40}})
+    Q.tune_cache_store({k2: {"1": 2}})
+    d = Q.tune_cache_load()
+    assert d[k1] == {"1": 4, "3": 40} and d[k2] == {"1": 2}
+    (tmp_path / "sub" / "tune.json").write_text("{not json")
+    assert Q.tune_cache_load() == {}  # a damaged cache only costs a retune
