@@ -156,6 +156,99 @@ def ensemble(Q, configs, steps):
     return out
 
 
+def pass_table(passes, work, s, kahan):
+    """Per-pass ms per launch and algorithmic GB/s from per-pass event timings."""
+    words = dict(PASS_WORDS)
+    if kahan:
+        words["P4"] = 7
+    out = {}
+    for k, (ms, cnt) in passes.items():
+        if cnt and k in words:
+            per = ms / cnt
+            gbs = words[k] * s * work / (per * 1e-3) / 1e9
+            out[k] = {"ms_per_launch": round(per, 5), "GBs_algorithmic": round(gbs, 1),
+                      "roofline_frac": round(gbs / HBM_PEAK_GBS, 3)}
+    return out
+
+
+def c5_leg(Q, configs, steps, acc_steps):
+    """4096^2 x 32 layers (L = 8000 km, dt = 600 s): the FP32 + Kahan form with
+    the layer<->mode projections on the matrix cores (32 x 32 FP32 contraction
+    over 16.8 M columns, k_tproj_mfma), batched through the graph, and the FP64
+    form launch by launch; per-pass GB/s, the contraction's achieved FLOP/s
+    against the FP32 matrix peak, and the FP32 error against the FP64 form
+    after acc_steps steps (the FP64 form matches the oracle to 1e-12 at this
+    size: tests/test_gpu_fullsize.py)."""
+    p = configs.c5()
+    t0 = time.perf_counter()
+    modes = Q.eigmode(p)
+    q0 = Q.make_ic(p, modes)[1]
+    t_ic = time.perf_counter() - t0
+    work = p.NX * p.NY * p.nl
+    g = Q.grid(p)[0]
+    storage = g.nxp2 * g.nyp2 * p.nl
+    out = {"workload": "4096^2 x 32-layer QG step (L=8000 km, dt=600 s, nu4=2e8, double-gyre wind)",
+           "initial_condition_s": t_ic}
+    # FP32 batched, matrix-core projections (the default at >= 16 layers in FP32)
+    b = Q.Batch([p], Q.FP32, [modes])
+    b.set_q(0, q0)
+    b.step(6)
+    b.synchronize()
+    ms32 = b.time_step(steps) / steps
+    b.profile(True)
+    b.step(2)
+    b.profile(False)
+    p32 = b.profile_read()
+    tuning32 = b.tuning(0)
+    b.close()
+    m = Q.Model(p, Q.FP64, modes)
+    m.set_q(q0)
+    m.run(2)
+    m.synchronize()
+    ms64 = m.time_run(steps) / steps
+    m.profile(True)
+    m.run(2)
+    m.profile(False)
+    p64 = m.profile_read()
+    m.close()
+    t32 = pass_table(p32, storage, 4, True)
+    t64 = pass_table(p64, storage, 8, False)
+    # the contraction: 2 nl^2 flops per column per projection
+    flops = 2.0 * p.nl * p.nl * g.nxp2 * g.nyp2
+    for k in ("T1", "T2"):
+        if k in t32:
+            tf = flops / (t32[k]["ms_per_launch"] * 1e-3) / 1e12
+            t32[k]["TFLOPs_matrix"] = round(tf, 2)
+            t32[k]["matrix_peak_frac"] = round(tf / 157.3, 3)
+    # accuracy: acc_steps steps of each form from the same state
+    m = Q.Model(p, Q.FP64, modes)
+    m.set_q(q0)
+    m.run(acc_steps)
+    m.invert_pv()
+    E64, Z64 = m.diagnostics()[2:]
+    q64 = m.q
+    m.close()
+    b = Q.Batch([p], Q.FP32, [modes])
+    b.set_q(0, q0)
+    del q0
+    b.step(acc_steps)
+    b.invert_pv()
+    E32, Z32 = b.diagnostics(0)[2:]
+    q32 = b.q(0)
+    b.close()
+    out.update({
+        "fp32": {"ms_per_step": ms32, "value": work / (ms32 * 1e-3), "passes": t32, "tuning": tuning32,
+                 "form": "fp32+kahan, batched (hipGraph replay), projections on the matrix cores"},
+        "fp64": {"ms_per_step": ms64, "value": work / (ms64 * 1e-3), "passes": t64},
+        "fp32_over_fp64": ms64 / ms32,
+        "accuracy_vs_fp64": {"steps": acc_steps, "energy_rel_err": abs(E32 - E64) / abs(E64),
+                             "enstrophy_rel_err": abs(Z32 - Z64) / abs(Z64),
+                             "q_l2_rel_err": float(np.linalg.norm(q32 - q64) / np.linalg.norm(q64))},
+        "unit": "grid-pt*layer*steps/s", "steps": steps,
+    })
+    return out
+
+
 def forcing_rows(Q, p, modes, q0, steps):
     """The forcing rows at the same workload, one run per form and forcing:
     the pass after the AB3 update (QGW_PASS_FORC) timed with per-pass events.
@@ -324,6 +417,10 @@ def main():
     ap.add_argument("--ensemble-steps", type=int, default=24)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--no-c5", action="store_true", help="skip the 4096^2 x 32 leg")
+    ap.add_argument("--c5-steps", type=int, default=12)
+    ap.add_argument("--c5-accuracy-steps", type=int, default=24)
+    ap.add_argument("--c5-only", action="store_true", help="run only the 4096^2 x 32 leg (profiling)")
     ap.add_argument("--no-rows", action="store_true", help="skip the forcing and netCDF measurements")
     ap.add_argument("--rows-steps", type=int, default=20)
     ap.add_argument("--forms", default="fp64,fp32")
@@ -337,6 +434,9 @@ def main():
     import qgw_amd as Q
     from qgw_amd import configs
 
+    if args.c5_only:
+        print(json.dumps({"c5": c5_leg(Q, configs, args.c5_steps, args.c5_accuracy_steps)}), flush=True)
+        return
     p = configs.c3()
     if args.nx != 2048:
         p = configs.scaled(p, args.nx)
@@ -428,6 +528,9 @@ def main():
     if not args.no_ensemble:
         log("ensemble leg")
         out["ensemble"] = ensemble(Q, configs, args.ensemble_steps)
+    if not args.no_c5:
+        log("4096^2 x 32 leg")
+        out["c5"] = c5_leg(Q, configs, args.c5_steps, args.c5_accuracy_steps)
     if not args.no_rows:
         log("forcing and netCDF rows")
         out["rows"] = forcing_rows(Q, p, modes, q0, args.rows_steps)

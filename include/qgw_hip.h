@@ -60,7 +60,17 @@ typedef struct qgw_ctx qgw_ctx;
  * 2(m+1) in {2^a: 8..8192} or {3*2^a: 24..6144}; periodic (bc_fac = -1):
  * NX = NY even, in the same length sets; n_hyper even and <= 8; with
  * stochastic forcing (k_f != 0), NX = NY; with file forcing, dt_forc_period a
- * positive multiple of dt_forc > 0). */
+ * positive multiple of dt_forc > 0).  When the environment variable
+ * QGW_MAX_DEVICE_BYTES holds a positive byte count, a context (or batch group)
+ * that would allocate more device memory fails with QGW_ERR_NOMEM and frees
+ * what it had allocated.
+ *
+ * Failure detection: every step's update kernel raises a device flag when a
+ * new q value (or, in qgw_rhs, a tendency) is NaN or Inf.  The flag is
+ * checked, and QGW_ERR_NUMERIC returned, at the end of qgw_run, qgw_time_run,
+ * qgw_synchronize and qgw_diagnostics (and of their batch forms); on the
+ * adaptive step the CFL reduction also fails on a non-finite psi.  The flag
+ * stays raised until q is uploaded again (a single run). */
 qgw_ctx *qgw_create(const qgw_params *p, int precision, int *status);
 
 /* Free the run (clean_fft, clean_timestep, clean_eigmode: src/elliptic.h:157,
@@ -100,8 +110,9 @@ int qgw_timestep(qgw_ctx *ctx);
 
 /* nsteps consecutive qgw_timestep calls.  With a fixed dt and no 4-D forcing
  * record change the whole sequence is enqueued without host synchronisation
- * (the tendency buffers rotate by pointer).  The graph-replayed step is the
- * batch form (qgw_batch_run). */
+ * (the tendency buffers rotate by pointer); the call then waits for the
+ * stream once, to check the non-finite flag (QGW_ERR_NUMERIC).  The
+ * graph-replayed step is the batch form (qgw_batch_step). */
 int qgw_run(qgw_ctx *ctx, int nsteps);
 
 /* Energy and enstrophy of the current (psi, q) pair, accumulated in FP64:
@@ -145,7 +156,8 @@ int qgw_set_forcing_file(qgw_ctx *ctx, const char *path);
  * is private to the context.  This call restarts it from another seed. */
 int qgw_set_stoch_seed(qgw_ctx *ctx, unsigned seed);
 
-/* Wait for all work queued on the context's stream. */
+/* Wait for all work queued on the context's stream; QGW_ERR_NUMERIC if a
+ * step has produced a non-finite q. */
 int qgw_synchronize(qgw_ctx *ctx);
 
 /* Time nsteps steps of qgw_run between two HIP events recorded on the
@@ -239,7 +251,9 @@ int qgw_batch_upload(qgw_batch *b, int member, int field, const double *host);
 int qgw_batch_download(qgw_batch *b, int member, int field, double *host);
 /* invert_pv (src/elliptic.h:76) of every member. */
 int qgw_batch_invert_pv(qgw_batch *b);
-/* nsteps of timestep (src/timestep.h:114) for every member. */
+/* nsteps of timestep (src/timestep.h:114) for every member; waits for the
+ * stream at the end and returns QGW_ERR_NUMERIC if any group's q has become
+ * non-finite (the flag of a group of several members stays raised). */
 int qgw_batch_step(qgw_batch *b, int nsteps);
 /* As qgw_batch_step, timed with HIP events on the batch stream.  The graph of
  * the current state is captured and instantiated before the start event, so

@@ -1034,7 +1034,17 @@ struct StepCoef {
   double topo_fac;       // f0/dh_bottom
   double ring_c;         // 2*bc_fac/Delta^2
   int update;            // 0: write the tendency only (rhs), 1: also the AB3 update
+  unsigned *nonfinite;   // set to 1 when an output (new q, or the tendency) is NaN or Inf; may be null
 };
+
+// Non-finite detection at one FMA per output: v * 0 is +-0 for finite v and
+// NaN for NaN or Inf, so the running sum stays zero until a non-finite value
+// enters it.  The flag is raised once per thread at the end of its work.
+template <typename T> __device__ __forceinline__ T nf_acc(T acc, T v) { return __builtin_fma(v, T(0), acc); }
+__device__ __forceinline__ float nf_acc(float acc, float v) { return __builtin_fmaf(v, 0.f, acc); }
+template <typename T> __device__ __forceinline__ void nf_flag(T acc, unsigned *flag) {
+  if (flag && acc != acc) atomicOr(flag, 1u);
+}
 
 // Visit every point (ly, lx) of a (TX + 2R) x HH region with the 4 x TX
 // threads of a P4 workgroup: the TX middle columns one column per thread, the
@@ -1174,6 +1184,7 @@ __device__ __forceinline__ void p4_body(P4Smem<T, TX, TY, NLAP> &sm, const Geo &
   const T drag = (T)cf.drag, tfac = (T)cf.topo_fac;
   const bool use_nu = cf.nu != 0.0;
   const int gx = x0 + tx;
+  T nfa = T(0);
 #pragma unroll
   for (int r = 0; r < TY / 4; r++) {
     const int tyr = ty + 4 * r;
@@ -1217,15 +1228,21 @@ __device__ __forceinline__ void p4_body(P4Smem<T, TX, TY, NLAP> &sm, const Geo &
 #undef TS
     const int o = gy * g.P + gx;  // offset inside layer k (the pointers carry the layer)
     fnew[o] = f;
-    if (!cf.update) continue;
+    if (!cf.update) {
+      nfa = nf_acc(nfa, f);
+      continue;
+    }
     const T inc = c0 * f + c1 * hf1[r] + c2 * hf2[r];
     if constexpr (KAHAN) {
       const T y = inc - comp_val<T>(hcp[r], qo);
       const T tq = qo + y;
       comp[o] = comp_make<T>((tq - qo) - y, tq);
       qn[o] = tq;
+      nfa = nf_acc(nfa, tq);
     } else {
-      qn[o] = qo + inc;
+      const T tq = qo + inc;
+      qn[o] = tq;
+      nfa = nf_acc(nfa, tq);
     }
     // carry q's wall ring into the next buffer (bounded: corners stay zero)
     if (gy == 1) qn[gx] = sm.Q[qy - 1][qx];
@@ -1239,6 +1256,7 @@ __device__ __forceinline__ void p4_body(P4Smem<T, TX, TY, NLAP> &sm, const Geo &
       if (gy == m && gx == n) qn[(m + 1) * g.P + n + 1] = sm.Q[qy + 1][qx + 1];
     }
   }
+  nf_flag(nfa, cf.nonfinite);
 }
 
 // ===========================================================================
@@ -1514,6 +1532,7 @@ __device__ __forceinline__ void p4s_walk(const Geo &g, const T *__restrict__ P, 
     }
   };
   const BufView vF = buf_view(fnew, lbytes), vN = buf_view(qn, lbytes);
+  T nfa = T(0);  // non-finite detection (nf_acc)
   auto body = [&](auto PHc, int r) {
     constexpr int ph = decltype(PHc)::value;
     constexpr int cur = ph % RING;
@@ -1592,7 +1611,10 @@ __device__ __forceinline__ void p4s_walk(const Geo &g, const T *__restrict__ P, 
       st_at(fnew + ro, xb, f);
     else
       buf_st(vF, xb, rb, f);
-    if (!upd) return;
+    if (!upd) {
+      nfa = nf_acc(nfa, f);
+      return;
+    }
     const T qo = jqc[j1];
     const T inc = c0 * f + c1 * b1[cur] + c2 * b2[cur];
     T qv, cv = T(0);
@@ -1605,6 +1627,7 @@ __device__ __forceinline__ void p4s_walk(const Geo &g, const T *__restrict__ P, 
     } else {
       qv = qo + inc;
     }
+    nfa = nf_acc(nfa, qv);
     if constexpr (EDGE) {
       if constexpr (KAHAN) st_at(comp + ro, xbc, cs);
       st_at(qn + ro, xb, qv);
@@ -1650,6 +1673,7 @@ __device__ __forceinline__ void p4s_walk(const Geo &g, const T *__restrict__ P, 
     QGW_P4S_PHASE(11)
 #undef QGW_P4S_PHASE
   }
+  nf_flag(nfa, cf.nonfinite);
 }
 
 template <typename T, int NLAP, int KAHAN>

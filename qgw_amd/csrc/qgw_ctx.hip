@@ -60,6 +60,8 @@ struct qgw_ctx {
   void *psi, *fa[3], *comp, *topo, *wind, *what, *whatT, *cl2m, *cm2l, *twx, *twy, *twxh, *twyh;
   double *iRd2_d, *La2, *Kb2, *diag_part, *diag_out, *stage;
   unsigned long long *cfl_max;
+  unsigned *nonfinite;  // raised by P4 when a new q (or a tendency) is NaN or Inf; sticky until q is uploaded
+  size_t byte_limit;    // device bytes the context may allocate (QGW_MAX_DEVICE_BYTES; 0: no limit)
   int nblk_diag;
   int p4_tile;
   int t_kernel;  // projection kernel choice (QGW_TUNE_T_KERNEL)
@@ -122,6 +124,11 @@ int fail_hip(qgw_ctx *c, hipError_t e, const char *what) {
 int dalloc(qgw_ctx *c, void **ptr, size_t bytes) {
   *ptr = nullptr;
   if (bytes == 0) bytes = 16;
+  if (c->byte_limit && c->bytes + bytes > c->byte_limit) {
+    snprintf(c->err, sizeof c->err, "device memory limit QGW_MAX_DEVICE_BYTES=%zu reached (%zu + %zu bytes)",
+             c->byte_limit, c->bytes, bytes);
+    return QGW_ERR_NOMEM;
+  }
   hipError_t e = hipMalloc(ptr, bytes);
   if (e != hipSuccess) {
     fail_hip(c, e, "hipMalloc");
@@ -350,6 +357,7 @@ int run_p4(qgw_ctx *c, const double *cab3, int update) {
   cf.topo_fac = c->p.f0 / dhb;
   cf.ring_c = 2.0 * c->p.bc_fac / (D * D);
   cf.update = update;
+  cf.nonfinite = c->nonfinite;
   a.nlap = (c->nu_hyper != 0.0) ? (c->p.n_hyper / 2 + 1) : 0;
   a.p4_tile = c->p4_tile;
   a.p4_waves = c->p4_waves;
@@ -361,6 +369,19 @@ int run_p4(qgw_ctx *c, const double *cab3, int update) {
   }
   if (st || (st = check_launch(c, "p4"))) return st;
   if (update) c->qcur ^= 1;
+  return QGW_OK;
+}
+
+// The P4 non-finite flag: QGW_ERR_NUMERIC once a step has produced a NaN or
+// Inf in q (or in the tendency of qgw_rhs).  Waits for the stream.
+int check_finite(qgw_ctx *c) {
+  unsigned h = 0;
+  HIPCHK(c, hipMemcpyAsync(&h, c->nonfinite, sizeof h, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (h) {
+    snprintf(c->err, sizeof c->err, "non-finite q (NaN or Inf) produced by a step at or before step %d", c->it);
+    return QGW_ERR_NUMERIC;
+  }
   return QGW_OK;
 }
 
@@ -619,6 +640,8 @@ int member_upload(qgw_ctx *c, int member, int field, const double *host) {
   a.stream = c->stream;
   st = qgw::launch_from_f64(a, layers, c->stage, dst, resid);
   if (st || (st = check_launch(c, "from_f64"))) return st;
+  // a new state: the non-finite flag of the previous one no longer applies
+  if (field == QGW_FIELD_Q && c->nb == 1) HIPCHK(c, hipMemsetAsync(c->nonfinite, 0, sizeof(unsigned), c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return QGW_OK;
 }
@@ -803,6 +826,11 @@ qgw_ctx *create_group(const qgw_params *ps, int nb, int precision, hipStream_t e
 
   qgw_ctx *c = new qgw_ctx();
   memset(c->err, 0, sizeof c->err);
+  {
+    // an optional ceiling on the device memory of one context (callers that share the GPU)
+    const char *lim = getenv("QGW_MAX_DEVICE_BYTES");
+    c->byte_limit = lim ? strtoull(lim, nullptr, 10) : 0;
+  }
   c->p = *p;
   c->nb = nb;
   c->mp.assign(ps, ps + nb);
@@ -870,6 +898,7 @@ qgw_ctx *create_group(const qgw_params *ps, int nb, int precision, hipStream_t e
   st = st ? st : dalloc(c, (void **)&c->diag_part, sizeof(double) * 2 * c->nblk_diag * NL);
   st = st ? st : dalloc(c, (void **)&c->diag_out, sizeof(double) * 2 * NL);
   st = st ? st : dalloc(c, (void **)&c->cfl_max, sizeof(unsigned long long));
+  st = st ? st : dalloc(c, (void **)&c->nonfinite, sizeof(unsigned));
   c->nrec4 = (p->dt_forc_period != 0) ? forcing_records(p) : 0;
   c->irec4[0] = c->irec4[1] = -1;
   c->forc4[0] = c->forc4[1] = nullptr;
@@ -1399,8 +1428,7 @@ int qgw_timestep(qgw_ctx *c) {
   return one_step(c);
 }
 
-int qgw_run(qgw_ctx *c, int nsteps) {
-  if (!c || nsteps < 0) return QGW_ERR_ARG;
+int run_steps(qgw_ctx *c, int nsteps) {
   for (int i = 0; i < nsteps; i++) {
     int st = one_step(c);
     if (st) return st;
@@ -1408,11 +1436,18 @@ int qgw_run(qgw_ctx *c, int nsteps) {
   return QGW_OK;
 }
 
+int qgw_run(qgw_ctx *c, int nsteps) {
+  if (!c || nsteps < 0) return QGW_ERR_ARG;
+  int st = run_steps(c, nsteps);
+  return (st || nsteps == 0) ? st : check_finite(c);
+}
+
 int qgw_diagnostics(qgw_ctx *c, double *E_layer, double *Z_layer, double *E_tot, double *Z_tot) {
   if (!c) return QGW_ERR_ARG;
   std::vector<double> h;
   int st = group_diag(c, h);
   if (st) return st;
+  if ((st = check_finite(c))) return st;
   member_energy(c, 0, h.data(), E_layer, Z_layer, E_tot, Z_tot);
   return QGW_OK;
 }
@@ -1470,7 +1505,7 @@ int qgw_set_forcing_file(qgw_ctx *c, const char *path) {
 int qgw_synchronize(qgw_ctx *c) {
   if (!c) return QGW_ERR_ARG;
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  return QGW_OK;
+  return check_finite(c);
 }
 
 int qgw_time_run(qgw_ctx *c, int nsteps, double *ms) {
@@ -1479,13 +1514,14 @@ int qgw_time_run(qgw_ctx *c, int nsteps, double *ms) {
   HIPCHK(c, hipEventCreate(&a));
   HIPCHK(c, hipEventCreate(&b));
   HIPCHK(c, hipEventRecord(a, c->stream));
-  int st = qgw_run(c, nsteps);
+  int st = run_steps(c, nsteps);
   if (!st) {
     HIPCHK(c, hipEventRecord(b, c->stream));
     HIPCHK(c, hipEventSynchronize(b));
     float f = 0.f;
     HIPCHK(c, hipEventElapsedTime(&f, a, b));
     *ms = f;
+    st = check_finite(c);
   }
   hipEventDestroy(a);
   hipEventDestroy(b);
@@ -1596,6 +1632,7 @@ const char *qgw_strerror(int status) { return qgwh_strerror(status); }
 const char *qgw_last_error(qgw_ctx *c) { return c ? c->err : ""; }
 
 
+
 qgw_batch *qgw_batch_create(int nmembers, const qgw_params *params, int precision, int *status) {
   int dummy;
   if (!status) status = &dummy;
@@ -1687,9 +1724,19 @@ int qgw_batch_invert_pv(qgw_batch *b) {
   return QGW_OK;
 }
 
+// The non-finite flags of every group (waits for the batch stream).
+int batch_check_finite(qgw_batch *b) {
+  for (auto *c : b->groups) {
+    int st = check_finite(c);
+    if (st) return bpass(b, c, st);
+  }
+  return QGW_OK;
+}
+
 int qgw_batch_step(qgw_batch *b, int nsteps) {
   if (!b || nsteps < 0) return QGW_ERR_ARG;
-  return batch_run(b, nsteps);
+  int st = batch_run(b, nsteps);
+  return (st || nsteps == 0) ? st : batch_check_finite(b);
 }
 
 int qgw_batch_time_step(qgw_batch *b, int nsteps, double *ms) {
@@ -1718,7 +1765,7 @@ int qgw_batch_time_step(qgw_batch *b, int nsteps, double *ms) {
   *ms = f;
   hipEventDestroy(e0);
   hipEventDestroy(e1);
-  return st;
+  return st ? st : batch_check_finite(b);
 }
 
 int qgw_batch_prepare(qgw_batch *b) {
@@ -1731,7 +1778,7 @@ int qgw_batch_prepare(qgw_batch *b) {
 int qgw_batch_synchronize(qgw_batch *b) {
   if (!b) return QGW_ERR_ARG;
   hipError_t e = hipStreamSynchronize(b->stream);
-  return e == hipSuccess ? QGW_OK : bfail(b, e, "hipStreamSynchronize");
+  return e == hipSuccess ? batch_check_finite(b) : bfail(b, e, "hipStreamSynchronize");
 }
 
 int qgw_batch_diagnostics(qgw_batch *b, int member, double *E_layer, double *Z_layer, double *E_tot,
@@ -1741,6 +1788,7 @@ int qgw_batch_diagnostics(qgw_batch *b, int member, double *E_layer, double *Z_l
   std::vector<double> h;
   int st = group_diag(c, h);
   if (st) return bpass(b, c, st);
+  if ((st = check_finite(c))) return bpass(b, c, st);
   member_energy(c, b->iof[member], h.data(), E_layer, Z_layer, E_tot, Z_tot);
   return QGW_OK;
 }

@@ -9,6 +9,7 @@
  *   seeded initial condition ..... no reference counterpart (the reference reads restart.nc)
  *   diagnostics .................. test/test_energy_injection.py:24-25
  */
+#define _GNU_SOURCE /* sincos */
 #include <ctype.h>
 #include <math.h>
 #include <stddef.h>
@@ -628,6 +629,14 @@ static uint64_t splitmix64(uint64_t *s) {
 /* uniform in (0, 1] */
 static double unif(uint64_t *s) { return ((double)(splitmix64(s) >> 11) + 1.0) * 0x1.0p-53; }
 
+/* draw k (0-based) of the stream seeded with s0: SplitMix64 advances its
+ * state by a constant, so any draw is reached directly (parallel fills give
+ * the sequence of the serial stream) */
+static double unif_at(uint64_t s0, uint64_t k) {
+  uint64_t s = s0 + k * 0x9e3779b97f4a7c15ULL;
+  return unif(&s);
+}
+
 /* In-place 2-D complex FFT of an m x n array (rows of n), forward or inverse
  * (unnormalised), rows and columns in parallel. */
 static int fft2_complex(cplx *a, int n, int m, int inverse) {
@@ -781,34 +790,31 @@ int qgwh_make_ic(const qgw_params *p, const double *cl2m, const double *cm2l, co
   }
   double kappa0 = 2.0 * QGW_PI / p->ic_lambda0;
   double k06 = pow(kappa0, 6);
-  uint64_t s = seed;
-  int have = 0;
-  double spare = 0.0;
-  for (int l = 0; l < nl; l++)
-    for (int b = 1; b <= m; b++)
-      for (int a = 1; a <= n; a++) {
-        double gauss;
-        if (have) {
-          gauss = spare;
-          have = 0;
-        } else {
-          double u1 = unif(&s), u2 = unif(&s);
-          double r = sqrt(-2.0 * log(u1));
-          gauss = r * cos(2.0 * QGW_PI * u2);
-          spare = r * sin(2.0 * QGW_PI * u2);
-          have = 1;
-        }
-        double ka = QGW_PI * a / p->Lx, kb = QGW_PI * b / p->Lx;
-        double kap = sqrt(ka * ka + kb * kb);
-        double amp = 1.0 / sqrt(k06 + pow(kap, 6));
-        X[(size_t)l * NI + (size_t)(b - 1) * n + (a - 1)] = gauss * amp;
-      }
+  /* coefficient e (layer-major, then b, then a) takes the cosine (e even) or
+   * the sine (e odd) of the Box-Muller pair made of draws e-e%2 and e-e%2+1 */
+  const long long NT = (long long)NI * nl;
+#pragma omp parallel for schedule(static)
+  for (long long e = 0; e < NT; e++) {
+    const uint64_t d = (uint64_t)(e - (e & 1));
+    const double u1 = unif_at(seed, d), u2 = unif_at(seed, d + 1);
+    const double r = sqrt(-2.0 * log(u1));
+    double sn, cs;
+    sincos(2.0 * QGW_PI * u2, &sn, &cs);
+    const double gauss = (e & 1) ? r * sn : r * cs;
+    const size_t o = (size_t)e % NI;
+    const int a = (int)(o % (size_t)n) + 1, b = (int)(o / (size_t)n) + 1;
+    double ka = QGW_PI * a / p->Lx, kb = QGW_PI * b / p->Lx;
+    double kap = sqrt(ka * ka + kb * kb);
+    double amp = 1.0 / sqrt(k06 + pow(kap, 6));
+    X[e] = gauss * amp;
+  }
   double norm = 4.0 * (n + 1) * (double)(m + 1);
   /* psi per layer */
   memcpy(W, X, sizeof(double) * NI * nl);
   for (int l = 0; l < nl && !st; l++) st = qgwh_dst2d(W + (size_t)l * NI, n, m);
   if (st) goto out;
-  for (size_t t = 0; t < NI * nl; t++) W[t] /= norm;
+#pragma omp parallel for schedule(static)
+  for (long long t = 0; t < NT; t++) W[t] /= norm;
   /* RMS of the forward-difference velocity over the interior (zero ring) */
   double sum = 0.0;
   for (int l = 0; l < nl; l++)
@@ -822,25 +828,30 @@ int qgwh_make_ic(const qgw_params *p, const double *cl2m, const double *cm2l, co
       }
   double rms = sqrt(sum / ((double)NI * nl));
   double scale = (rms > 0.0 && p->ic_urms > 0.0) ? p->ic_urms / rms : 1.0;
-  for (size_t t = 0; t < NI * nl; t++) {
+#pragma omp parallel for schedule(static)
+  for (long long t = 0; t < NT; t++) {
     X[t] *= scale;
     W[t] *= scale;
   }
-  for (size_t t = 0; t < S * nl; t++) psi[t] = 0.0;
+  memset(psi, 0, sizeof(double) * S * nl);
+#pragma omp parallel for collapse(2) schedule(static)
   for (int l = 0; l < nl; l++)
     for (int j = 1; j <= m; j++)
       for (int i = 1; i <= n; i++)
         psi[(size_t)l * S + (size_t)j * g.nxp2 + i] = W[(size_t)l * NI + (size_t)(j - 1) * n + (i - 1)];
   /* q: spectral modal Helmholtz operator applied to the coefficients */
+  int nomem = 0;
+#pragma omp parallel
   {
     double *Xm = (double *)malloc(sizeof(double) * nl);
     double *Qm = (double *)malloc(sizeof(double) * nl);
     if (!Xm || !Qm) {
-      free(Xm);
-      free(Qm);
-      st = QGW_ERR_NOMEM;
-      goto out;
+#pragma omp atomic write
+      nomem = 1;
     }
+#pragma omp barrier
+    if (!nomem) {
+#pragma omp for schedule(static)
     for (int b = 1; b <= m; b++)
       for (int a = 1; a <= n; a++) {
         size_t o = (size_t)(b - 1) * n + (a - 1);
@@ -857,12 +868,18 @@ int qgwh_make_ic(const qgw_params *p, const double *cl2m, const double *cm2l, co
           W[(size_t)l * NI + o] = acc;
         }
       }
+    }
     free(Xm);
     free(Qm);
   }
+  if (nomem) {
+    st = QGW_ERR_NOMEM;
+    goto out;
+  }
   for (int l = 0; l < nl && !st; l++) st = qgwh_dst2d(W + (size_t)l * NI, n, m);
   if (st) goto out;
-  for (size_t t = 0; t < S * nl; t++) q[t] = 0.0;
+  memset(q, 0, sizeof(double) * S * nl);
+#pragma omp parallel for collapse(2) schedule(static)
   for (int l = 0; l < nl; l++)
     for (int j = 1; j <= m; j++)
       for (int i = 1; i <= n; i++)

@@ -123,6 +123,43 @@ def test_4096x32_inversion():
     assert rel(psi32, psi64) < 1e-3
 
 
+@pytest.mark.timeout(600)
+def test_4096x32_steps_against_oracle():
+    """4096^2 x 32 (16.8 M columns, 537 M storage point.layers): two FP64
+    steps against the oracle (1e-12 on q, like the smaller sizes; the oracle
+    takes tens of seconds per step on the host's cores), and the FP32 form
+    with matrix-core projections two steps on: q within 1e-4 of FP64 and the
+    energy and enstrophy within 1e-5."""
+    p = configs.c5()
+    _announce("full size 4096x32 steps vs oracle", p.NX, p.nl)
+    modes = Q.eigmode(p)
+    q0 = Q.make_ic(p, modes)[1]
+    m = Q.Model(p, Q.FP64, modes)
+    m.set_q(q0)
+    m.run(2)
+    m.invert_pv()
+    q64 = m.q
+    E64, Z64 = m.diagnostics()[2:]
+    m.close()
+    f = Q.Batch([p], Q.FP32, [modes])
+    f.tune_set(0, Q.TUNE_T_KERNEL, 2)
+    f.set_q(0, q0)
+    f.step(2)
+    f.invert_pv()
+    q32 = f.q(0)
+    E32, Z32 = f.diagnostics(0)[2:]
+    f.close()
+    assert rel(q32, q64) < 1e-4
+    assert abs(E32 - E64) / abs(E64) < 1e-5 and abs(Z32 - Z64) / abs(Z64) < 1e-5
+    o = O.Oracle(p)
+    o.set_modes(*modes)
+    o.set_q(q0)
+    del q0
+    o.run(2)
+    assert rel(q64, o.field(O.Q)) < 1e-12
+    o.close()
+
+
 @pytest.mark.timeout(300)
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
 def test_ensemble_members_against_single_runs(prec):

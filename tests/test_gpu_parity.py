@@ -416,3 +416,60 @@ def test_dst_half_radix_schedules(fft_e, col, form):
     assert rel(m.psi, o.field(O.PSI)) < 1e-12
     m.close()
     o.close()
+
+
+@pytest.mark.parametrize("nl", [16, 20, 32])
+def test_projection_matrix_cores_vs_oracle(nl):
+    """The FP32 matrix-core projections (32- and 16-column tiles) against the
+    FP64 oracle directly: the inversion of the seeded state and 8 steps, at
+    the FP32 bar of the other FP32 checks (2e-5); a transposed or misplaced
+    coefficient of the 32 x 32 contraction gives O(1)."""
+    p = _small(64, nl)
+    _announce("projection MFMA vs oracle", p.NX, nl)
+    modes = Q.eigmode(p)
+    q0 = Q.make_ic(p, modes)[1]
+    o = O.Oracle(p)
+    o.set_modes(*modes)
+    o.set_q(q0)
+    o.invert_pv()
+    psi_o = o.field(O.PSI).copy()
+    o.run(8)
+    q_o = o.field(O.Q).copy()
+    o.close()
+    for tk in (2, 3):
+        m = Q.Model(p, Q.FP32, modes)
+        m.tune_set(Q.TUNE_T_KERNEL, tk)
+        m.set_q(q0)
+        m.invert_pv()
+        assert rel(m.psi, psi_o) < 2e-5, tk
+        m.run(8)
+        assert rel(m.q, q_o) < 2e-5, tk
+        m.close()
+
+
+@pytest.mark.parametrize("NX,nl", [(96, 2), (384, 3), (1536, 3)])
+def test_pipelined_rows_3x2k_fp32(NX, nl):
+    """FP32 row transforms of 3*2^k half lengths (96, 384, 1536): the
+    pipelined persistent blocks against the one-block form (the same Stockham
+    code and twiddle values: 1e-6), and both against the FP64 oracle within
+    the FP32 bar (2e-5)."""
+    p = _small(NX, nl, Lx=2e6 * NX / 1024, nu_hyper=1e4, ic_lambda0=max(150e3, 8 * 2e6 / 1024))
+    _announce("pipelined 3*2^k rows FP32", p.NX, p.nl)
+    modes = Q.eigmode(p)
+    q0 = Q.make_ic(p, modes)[1]
+    o = O.Oracle(p)
+    o.set_modes(*modes)
+    o.set_q(q0)
+    o.run(3)
+    q_o = o.field(O.Q).copy()
+    o.close()
+    out = {}
+    for form in (1, 2):
+        m = Q.Model(p, Q.FP32, modes)
+        m.tune_set(Q.TUNE_DST, form)
+        m.set_q(q0)
+        m.run(3)
+        out[form] = m.q
+        m.close()
+        assert rel(out[form], q_o) < 2e-5, form
+    assert rel(out[2], out[1]) < 1e-6
