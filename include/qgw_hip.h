@@ -123,6 +123,21 @@ int qgw_run(qgw_ctx *ctx, int nsteps);
  * the current time. */
 int qgw_diagnostics(qgw_ctx *ctx, double *E_layer, double *Z_layer, double *E_tot, double *Z_tot);
 
+/* Allocation-free form of qgw_diagnostics.  No counterpart in the reference:
+ * it is for callers that cannot allocate or wait (code under HIP graph
+ * capture, tight serving loops).  Enqueues on the context's stream, without
+ * waiting, the per-layer FP64 sums of the current (psi, q) pair into the
+ * caller's DEVICE array sums[2*nl]: sums[2l] = sum psi*q and
+ * sums[2l+1] = sum q^2 over all stored points of layer l (the compensated q
+ * in the QGW_FP32 form), so that E_layer[l] = -sums[2l]*Delta^2/Lx^2 and
+ * Z_layer[l] = sums[2l+1]*Delta^2/Lx^2 are exactly what qgw_diagnostics
+ * reports.  The partial sums live in a fixed scratch array of the library
+ * (256 KiB per GPU, for at most 64 layers; more layers return QGW_ERR_ARG);
+ * it calls no allocator.  Calls that overlap on one GPU (several contexts'
+ * streams at once) race on that scratch: order them on one stream or with
+ * events. */
+int qgw_diagnostics_device(qgw_ctx *ctx, double *sums);
+
 /* Time-step state (the reference globals t, dt, dt0, dt1, it; src/qg.c:63-70,
  * src/timestep.h:14). */
 int qgw_get_time(qgw_ctx *ctx, double *t, double *dt, double *dt0, double *dt1, int *it);

@@ -15,6 +15,7 @@
 #include <string.h>
 
 #include <string>
+#include <mutex>
 #include <utility>
 #include <vector>
 
@@ -105,10 +106,41 @@ struct qgw_ctx {
   std::vector<void *> allocs;  // every device allocation (field pointers may be offset or rotated)
   double prof_ms[QGW_NPASS];
   int prof_cnt[QGW_NPASS];
+  double *scratch;  // this device's copy of the file-scope `scratch` array (qgw_diagnostics_device)
   char err[256];
 };
 
+// ---- scratch of the allocation-free entry point --------------------------
+// Partial sums of qgw_diagnostics_device: 2 doubles per block and layer,
+// kNoAllocBlocks blocks per layer, at most kNoAllocLayers layers: 256 KiB,
+// held by every GPU the library runs on until the process ends.  Host code
+// never uses the array's name as an address: each device's copy is resolved
+// once, with that device current, by hipGetSymbolAddress (scratch_on_device).
+constexpr int kNoAllocBlocks = 256;
+constexpr int kNoAllocLayers = 64;
+static __device__ __align__(256) double scratch[2 * kNoAllocBlocks * kNoAllocLayers];
+
 namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxDevices];
+
+// The device address of `scratch` on device `dev` (the current device),
+// resolved on the first call for that device and kept.
+hipError_t scratch_on_device(int dev, double **out) {
+  *out = nullptr;
+  if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[dev] = (double *)ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return hipSuccess;
+}
 
 int fail_hip(qgw_ctx *c, hipError_t e, const char *what) {
   if (c) snprintf(c->err, sizeof c->err, "%s: %s", what, hipGetErrorString(e));
@@ -875,6 +907,11 @@ qgw_ctx *create_group(const qgw_params *ps, int nb, int precision, hipStream_t e
       return nullptr;
     }
     c->cus = cus;
+    // this device's copy of the allocation-free entry point's scratch
+    if (scratch_on_device(dev, &c->scratch) != hipSuccess) {
+      release(c);
+      return nullptr;
+    }
   }
   c->dk = 3.141592653589793 / p->Lx;
   const size_t field = c->esz * (size_t)g.LS * NL;
@@ -1631,6 +1668,29 @@ const char *qgw_strerror(int status) { return qgwh_strerror(status); }
 
 const char *qgw_last_error(qgw_ctx *c) { return c ? c->err : ""; }
 
+int qgw_diagnostics_device(qgw_ctx *c, double *sums) {
+  if (!c || !sums) return QGW_ERR_ARG;
+  if (c->g.nl > kNoAllocLayers) {
+    snprintf(c->err, sizeof c->err, "qgw_diagnostics_device: %d layers, at most %d", c->g.nl, kNoAllocLayers);
+    return QGW_ERR_ARG;
+  }
+  if (!c->scratch) {
+    snprintf(c->err, sizeof c->err, "qgw_diagnostics_device: no scratch on this device");
+    return QGW_ERR_STATE;
+  }
+  qgw::LaunchArgs a;
+  a.g = c->g;
+  a.prec = c->prec;
+  a.stream = c->stream;
+  a.psi = c->psi;
+  a.q = c->q[c->qcur];
+  a.comp = (c->prec == QGW_FP32) ? c->comp : nullptr;
+  // every partial is written by its block before k_diag_final reads it, so
+  // the scratch needs no clearing between calls
+  int st = qgw::launch_diag(a, c->scratch, kNoAllocBlocks, sums);
+  if (st || (st = check_launch(c, "diag"))) return st;
+  return QGW_OK;
+}
 
 
 qgw_batch *qgw_batch_create(int nmembers, const qgw_params *params, int precision, int *status) {
