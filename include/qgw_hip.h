@@ -56,9 +56,13 @@ typedef struct qgw_ctx qgw_ctx;
  * QGW_FP32_PLAIN.  q, psi and the tendency history start at zero, topography
  * from the bathtub formula.  The vertical modes must be supplied with
  * qgw_set_modes before the first inversion.  Returns NULL and sets *status
- * on failure (1 <= nl <= QGW_DEV_NL_MAX; bounded (bc_fac >= 0): 2(n+1) and
- * 2(m+1) in {2^a: 8..8192} or {3*2^a: 24..6144}; periodic (bc_fac = -1):
- * NX = NY even, in the same length sets; n_hyper even and <= 8; with
+ * on failure (1 <= nl <= QGW_DEV_NL_MAX; bounded (bc_fac >= 0): any grid,
+ * with the FFT sine transforms when 2(n+1) and 2(m+1) are in {2^a: 8..8192}
+ * or {3*2^a: 24..6144}, else dense sine transforms, matrix products with
+ * S_ij = 2 sin(pi i j/(N+1)) on the matrix cores through rocBLAS (O(N)
+ * flops per point instead of O(log N), for any NX, NY the reference's FFTW
+ * plans accept); periodic (bc_fac = -1): NX = NY even, in the FFT length
+ * sets; n_hyper even and <= 8; with
  * stochastic forcing (k_f != 0), NX = NY; with file forcing, dt_forc_period a
  * positive multiple of dt_forc > 0).  When the environment variable
  * QGW_MAX_DEVICE_BYTES holds a positive byte count, a context (or batch group)
@@ -213,7 +217,8 @@ int qgw_profile_read(qgw_ctx *ctx, double *ms, int *count, int npass);
  * m+1 in 64..4096 or 96..3072), 2 (default where 1 is provided) as 1 with
  * the row passes P1/P3 in persistent blocks that load the next row pair
  * while transforming the current one, 0 through complex FFTs of the odd
- * extension, length 2(n+1) (2(m+1)). */
+ * extension, length 2(n+1) (2(m+1)); 3 (reported, and the only choice) for
+ * the dense sine transforms of grids the FFT lengths do not cover. */
 #define QGW_TUNE_DST 4
 /* QGW_TUNE_FFT_E: radix schedule of the half-length FFTs of power-of-two
  * lengths from 1024: 16 (radix-16 stages; the default) or 8 (radix-8 stages,

@@ -630,7 +630,7 @@ class Model:
         """The tuning choices, by name."""
         return {"p4": P4_TILES[self.tune_get(TUNE_P4_TILE)], "p4_waves": self.tune_get(TUNE_P4_WAVES),
                 "t_kernel": ["auto", "valu", "mfma32", "mfma16"][self.tune_get(TUNE_T_KERNEL)],
-                "dst": ["odd-extension", "half-length", "half-length pipelined"][self.tune_get(TUNE_DST)],
+                "dst": ["odd-extension", "half-length", "half-length pipelined", "dense (matrix cores)"][self.tune_get(TUNE_DST)],
                 "fft_e": self.tune_get(TUNE_FFT_E), "fft_e_col": self.tune_get(TUNE_FFT_E_COL)}
 
     def autotune(self, reps=5, cache=True):
@@ -813,7 +813,7 @@ class Batch:
         return {"p4": P4_TILES[self.tune_get(group, TUNE_P4_TILE)],
                 "p4_waves": self.tune_get(group, TUNE_P4_WAVES),
                 "t_kernel": ["auto", "valu", "mfma32", "mfma16"][self.tune_get(group, TUNE_T_KERNEL)],
-                "dst": ["odd-extension", "half-length", "half-length pipelined"][self.tune_get(group, TUNE_DST)],
+                "dst": ["odd-extension", "half-length", "half-length pipelined", "dense (matrix cores)"][self.tune_get(group, TUNE_DST)],
                 "fft_e": self.tune_get(group, TUNE_FFT_E), "fft_e_col": self.tune_get(group, TUNE_FFT_E_COL)}
 
     def profile(self, on=True):

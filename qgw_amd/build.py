@@ -57,7 +57,7 @@ def build(verbose=False):
         objs = list(ex.map(comp, HIP_SOURCES))
     hip_so = os.path.join(HERE, "libqgw_hip.so")
     _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", hip_so] + objs +
-         ["-L" + HERE, "-lqgw_host", "-Wl,-rpath,$ORIGIN"])
+         ["-L" + HERE, "-lqgw_host", "-lrocblas", "-Wl,-rpath,$ORIGIN"])
     drv = os.path.join(HERE, "qg_hip")
     if os.path.exists(os.path.join(HOST, "qg_main.c")):
         _run(["gcc"] + CC_FLAGS + ["-o", drv, os.path.join(HOST, "qg_main.c"),

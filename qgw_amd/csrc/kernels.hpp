@@ -1946,6 +1946,57 @@ k_forcing(Geo g, T *__restrict__ q, comp_t<T> *__restrict__ comp, const T *__res
 }
 
 // ===========================================================================
+// Dense sine transforms (grids whose transform lengths the FFT kernels do not
+// cover): the row and column DST-I are matrix products with S_ij =
+// 2 sin(pi i j / (n+1)) on the matrix cores (library GEMM, see qgw_ctx.hip);
+// these two kernels are the parts between the products.
+//
+// Helmholtz division on a whatT-shaped array (mode k, row a = x frequency
+// 1..n, column b = y frequency 1..m): w /= -(L_a^2 + K_b^2 + 1/Rd_k^2) * norm
+// (src/elliptic.h:120-138), the factor formed in FP64.
+// ===========================================================================
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_helm_div(Geo g, T *__restrict__ w, const double *__restrict__ La2, const double *__restrict__ Kb2,
+           const double *__restrict__ iRd2, double norm) {
+  const int b = blockIdx.x * 256 + threadIdx.x;  // column (y frequency b + 1)
+  const int a = blockIdx.y;                      // row (x frequency a + 1)
+  const int k = blockIdx.z;                      // mode (all members)
+  if (b >= g.m) return;
+  const long o = k * g.WTS + (long)a * g.Pm + b;
+  const double f = 1.0 / (-(La2[a + 1] + Kb2[b + 1] + iRd2[k]) * norm);
+  w[o] = (T)((double)w[o] * f);
+}
+
+// The wall rings after the dense inverse row transform has written the psi
+// interior: psi's ring is zero (adjust_bc(psi, 0)), q's ring (corners left as
+// they are) is 2 bc_fac/Delta^2 times the adjacent interior psi
+// (adjust_bc(q, 1, psi), src/domain.h:308-348).  One thread per ring point.
+template <typename T>
+__global__ void __launch_bounds__(256) k_rings(Geo g, T *__restrict__ psi, T *__restrict__ q, T ring_c) {
+  const int n = g.n, m = g.m;
+  const int per = 2 * (n + 2) + 2 * m;  // rows 0 and m+1 in full, then columns 0 and n+1 of rows 1..m
+  const int e = blockIdx.x * 256 + threadIdx.x;
+  const int k = blockIdx.y;
+  if (e >= per) return;
+  T *P = psi + k * g.LS, *Q = q + k * g.LS;
+  int j, i;
+  if (e < 2 * (n + 2)) {
+    j = (e < n + 2) ? 0 : m + 1;
+    i = (e < n + 2) ? e : e - (n + 2);
+  } else {
+    const int r = e - 2 * (n + 2);
+    j = 1 + (r >> 1);
+    i = (r & 1) ? n + 1 : 0;
+  }
+  P[(long)j * g.P + i] = T(0);
+  if (j == 0 && i >= 1 && i <= n) Q[i] = ring_c * P[(long)g.P + i];
+  if (j == m + 1 && i >= 1 && i <= n) Q[(long)(m + 1) * g.P + i] = ring_c * P[(long)m * g.P + i];
+  if (i == 0 && j >= 1 && j <= m) Q[(long)j * g.P] = ring_c * P[(long)j * g.P + 1];
+  if (i == n + 1 && j >= 1 && j <= m) Q[(long)j * g.P + n + 1] = ring_c * P[(long)j * g.P + n];
+}
+
+// ===========================================================================
 // Conversions between the host FP64 layout and the device arrays.
 // ===========================================================================
 template <typename T>

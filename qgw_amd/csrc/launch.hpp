@@ -67,6 +67,10 @@ template <typename T> int launch_stoch_t(const LaunchArgs &a, const StochArgs &s
 // 4-D forcing on a.q (k_forcing), every layer.
 template <typename T>
 int launch_forcing_t(const LaunchArgs &a, const void *F1, const void *F2, double w1, double w2, double dt);
+// Dense sine-transform helpers: Helmholtz division on a.whatT-shaped array
+// `w`, and the psi/q wall rings after the dense inverse row transform.
+template <typename T> int launch_helm_div_t(const LaunchArgs &a, void *w);
+template <typename T> int launch_rings_t(const LaunchArgs &a);
 bool launch_has_length(int L);
 bool launch_has_half_length(int M);  // lengths M = L/2 of the half-length transforms
 
@@ -86,7 +90,9 @@ bool launch_has_half_length(int M);  // lengths M = L/2 of the half-length trans
                                      double *dst);                                                        \
   template <> int launch_forcing_t<T>(const LaunchArgs &a, const void *F1, const void *F2, double w1,      \
                                       double w2, double dt);                                              \
-  template <> int launch_stoch_t<T>(const LaunchArgs &a, const StochArgs &s, double s_fac);
+  template <> int launch_stoch_t<T>(const LaunchArgs &a, const StochArgs &s, double s_fac);              \
+  template <> int launch_helm_div_t<T>(const LaunchArgs &a, void *w);                                   \
+  template <> int launch_rings_t<T>(const LaunchArgs &a);
 QGW_DECLARE_SPECS(float)
 QGW_DECLARE_SPECS(double)
 #undef QGW_DECLARE_SPECS
@@ -116,5 +122,7 @@ inline int launch_stoch(const LaunchArgs &a, const StochArgs &s, double s_fac) {
 inline int launch_forcing(const LaunchArgs &a, const void *F1, const void *F2, double w1, double w2, double dt) {
   return QGW_DISPATCH(launch_forcing, a, F1, F2, w1, w2, dt);
 }
+inline int launch_helm_div(const LaunchArgs &a, void *w) { return QGW_DISPATCH(launch_helm_div, a, w); }
+inline int launch_rings(const LaunchArgs &a) { return QGW_DISPATCH(launch_rings, a); }
 
 }  // namespace qgw

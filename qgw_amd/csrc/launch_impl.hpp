@@ -370,6 +370,21 @@ int launch_forcing_t<T>(const LaunchArgs &a, const void *F1, const void *F2, dou
   return 0;
 }
 
+template <> int launch_helm_div_t<T>(const LaunchArgs &a, void *w) {
+  if (a.g.n < 1 || a.g.m < 1 || a.g.nl < 1 || a.g.n > 65535 || a.g.nl > 65535) return QGW_ERR_ARG;
+  const dim3 grid((unsigned)((a.g.m + 255) / 256), (unsigned)a.g.n, (unsigned)a.g.nl);
+  hipLaunchKernelGGL((k_helm_div<T>), grid, dim3(256), 0, a.stream, a.g, (T *)w, a.La2, a.Kb2, a.iRd2, a.norm);
+  return 0;
+}
+
+template <> int launch_rings_t<T>(const LaunchArgs &a) {
+  if (a.g.nl < 1 || a.g.nl > 65535) return QGW_ERR_ARG;
+  const int per = 2 * (a.g.n + 2) + 2 * a.g.m;
+  const dim3 grid((unsigned)((per + 255) / 256), (unsigned)a.g.nl);
+  hipLaunchKernelGGL((k_rings<T>), grid, dim3(256), 0, a.stream, a.g, (T *)a.psi, (T *)a.q, (T)a.ring_c);
+  return 0;
+}
+
 template <>
 int launch_to_f64_t<T>(const LaunchArgs &a, int layers, const void *src, const void *resid, double *dst) {
   long tot = (long)(a.g.n + 2) * (a.g.m + 2) * layers;

@@ -19,6 +19,8 @@
 #include <utility>
 #include <vector>
 
+#include <rocblas/rocblas.h>
+
 #include "qgw_hip.h"
 #include "qgw_host.h"
 #include "qgw_nc.h"
@@ -73,6 +75,11 @@ struct qgw_ctx {
   int fft_e;     // elements per thread of the half-length FFTs of the row passes (QGW_TUNE_FFT_E)
   int fft_e_col; // and of the column pass (QGW_TUNE_FFT_E_COL)
   int per;       // periodic domain (bc_fac = -1)
+  // dense sine transforms (bounded grids whose transform lengths the FFT
+  // kernels do not cover): S matrices n x n and m x m, a whatT-shaped temporary
+  int dst_gemm;
+  rocblas_handle blas;
+  void *Sx, *Sy, *wtmp;
   bool modes_set;
   double t, dt, dt0, dt1, DT_max, t_out;
   int it;
@@ -278,6 +285,7 @@ int pitch_of(int w) {
 
 void release(qgw_ctx *c) {
   if (!c) return;
+  if (c->blas) rocblas_destroy_handle(c->blas);
   if (c->rng) qgwh_rng_destroy(c->rng);
   for (int i = 0; i < 2; i++) {
     if (c->st_host[i]) hipHostFree(c->st_host[i]);
@@ -295,6 +303,76 @@ void release(qgw_ctx *c) {
 }
 
 // ---- hot-path sequences ---------------------------------------------------
+
+// C (column-major M x N, ldc) = S (M x K, ld M) * B (K x N, ldb), batched over
+// `batch` matrices B, C with strides sB, sC (elements): a row-major array of
+// rows of K values (pitch ldb) times a symmetric S, i.e. a DST-I of every row.
+int dense_gemm(qgw_ctx *c, const void *S, int M, int N, int K, const void *B, int ldb, long sB, void *C, int ldc,
+               long sC, int batch) {
+  rocblas_status r = rocblas_set_stream(c->blas, c->stream);
+  if (r == rocblas_status_success) {
+    if (c->prec == QGW_FP64) {
+      const double one = 1.0, zero = 0.0;
+      r = rocblas_dgemm_strided_batched(c->blas, rocblas_operation_none, rocblas_operation_none, M, N, K, &one,
+                                        (const double *)S, M, 0, (const double *)B, ldb, sB, &zero, (double *)C,
+                                        ldc, sC, batch);
+    } else {
+      const float one = 1.f, zero = 0.f;
+      r = rocblas_sgemm_strided_batched(c->blas, rocblas_operation_none, rocblas_operation_none, M, N, K, &one,
+                                        (const float *)S, M, 0, (const float *)B, ldb, sB, &zero, (float *)C, ldc,
+                                        sC, batch);
+    }
+  }
+  if (r != rocblas_status_success) {
+    snprintf(c->err, sizeof c->err, "dense sine transform: %s", rocblas_status_to_string(r));
+    return QGW_ERR_HIP;
+  }
+  return QGW_OK;
+}
+
+// invert_pv with dense sine transforms: P1 rows (what[l] = q[l] S_x), T1,
+// P2 columns (S_y on every whatT row, division, S_y again), T2, P3 rows
+// (psi[l] = what[l] S_x) and the wall rings.
+int run_invert_dense(qgw_ctx *c, qgw::LaunchArgs &a) {
+  const qgw::Geo &g = c->g;
+  const size_t e = c->esz;
+  char *q = (char *)c->q[c->qcur], *psi = (char *)c->psi;
+  int st;
+  {
+    PassScope ps(c, QGW_PASS_P1);
+    st = dense_gemm(c, c->Sx, g.n, g.m, g.n, q + e * (g.P + 1), g.P, g.LS, c->what, g.Pn, g.WS, g.nl);
+  }
+  if (st) return st;
+  {
+    PassScope ps(c, QGW_PASS_T1);
+    st = qgw::launch_t1(a);
+  }
+  if (st || (st = check_launch(c, "t1"))) return st;
+  {
+    PassScope ps(c, QGW_PASS_P2);
+    st = dense_gemm(c, c->Sy, g.m, g.n, g.m, c->whatT, g.Pm, g.WTS, c->wtmp, g.Pm, g.WTS, g.nl);
+    if (!st) {
+      st = qgw::launch_helm_div(a, c->wtmp);
+      if (!st) st = check_launch(c, "helm_div");
+    }
+    if (!st) st = dense_gemm(c, c->Sy, g.m, g.n, g.m, c->wtmp, g.Pm, g.WTS, c->whatT, g.Pm, g.WTS, g.nl);
+  }
+  if (st) return st;
+  {
+    PassScope ps(c, QGW_PASS_T2);
+    st = qgw::launch_t2(a);
+  }
+  if (st || (st = check_launch(c, "t2"))) return st;
+  {
+    PassScope ps(c, QGW_PASS_P3);
+    st = dense_gemm(c, c->Sx, g.n, g.m, g.n, c->what, g.Pn, g.WS, psi + e * (g.P + 1), g.P, g.LS, g.nl);
+    if (!st) {
+      st = qgw::launch_rings(a);
+      if (!st) st = check_launch(c, "rings");
+    }
+  }
+  return st;
+}
 
 int run_invert(qgw_ctx *c) {
   if (!c->modes_set) {
@@ -330,6 +408,7 @@ int run_invert(qgw_ctx *c) {
   // normalisation of the transform pair (src/elliptic.h:36,51)
   a.norm = c->per ? (double)c->g.n * (double)c->g.m : 4.0 * (double)(c->g.n + 1) * (double)(c->g.m + 1);
   a.ring_c = 2.0 * c->p.bc_fac / (c->grid.Delta * c->grid.Delta);
+  if (c->dst_gemm) return run_invert_dense(c, a);
   int st;
   {
     PassScope ps(c, QGW_PASS_P1);
@@ -612,7 +691,7 @@ int forcing_records(const qgw_params *p) {
   return (int)r;
 }
 
-int check_params(const qgw_params *p, qgw_grid *grid, int *Lx, int *Ly) {
+int check_params(const qgw_params *p, qgw_grid *grid, int *Lx, int *Ly, int *dense) {
   if (p->nl < 1 || p->nl > qgw::kMaxLayers) return QGW_ERR_ARG;
   if (p->n_hyper < 0 || (p->n_hyper % 2) != 0 || p->n_hyper > 8) return QGW_ERR_ARG;
   // stochastic forcing: the reference's N x N field (N = NX) must be the grid's
@@ -623,7 +702,12 @@ int check_params(const qgw_params *p, qgw_grid *grid, int *Lx, int *Ly) {
   const bool per = (p->bc_fac == -1.0);
   *Lx = per ? grid->n : 2 * (grid->n + 1);
   *Ly = per ? grid->m : 2 * (grid->m + 1);
-  if (!supported_length(*Lx) || !supported_length(*Ly)) return QGW_ERR_ARG;
+  *dense = 0;
+  if (!supported_length(*Lx) || !supported_length(*Ly)) {
+    // bounded grids of any size: dense sine transforms; the periodic branch needs the FFT lengths
+    if (per || grid->n < 1 || grid->m < 1 || grid->n > 65535) return QGW_ERR_ARG;
+    *dense = 1;
+  }
   return QGW_OK;
 }
 
@@ -843,14 +927,41 @@ int setup_stochastic(qgw_ctx *c, const qgw_params *p) {
 
 // A group of nb members (ps[0..nb-1], all same_group, or nb = 1) on stream
 // `ext` (owned by the caller) or on a stream of its own (ext = null).
+// The dense sine transforms: a BLAS handle, S_x (n x n) and S_y (m x m) with
+// S_ij = 2 sin(pi i j / (N+1)) (i, j = 1..N; the unnormalised RODFT00 of
+// src/elliptic.h:63-66, formed in FP64), and the temporary of the column pass.
+int setup_dense(qgw_ctx *c) {
+  if (rocblas_create_handle(&c->blas) != rocblas_status_success) {
+    c->blas = nullptr;
+    snprintf(c->err, sizeof c->err, "rocblas_create_handle failed");
+    return QGW_ERR_HIP;
+  }
+  const int dims[2] = {c->g.n, c->g.m};
+  void **dst[2] = {&c->Sx, &c->Sy};
+  for (int d = 0; d < 2; d++) {
+    const int N = dims[d];
+    std::vector<double> S((size_t)N * N);
+    for (int i = 1; i <= N; i++)
+      for (int j = 1; j <= N; j++) {
+        // sin(pi i j/(N+1)) with the argument reduced exactly: i j mod 2(N+1)
+        const long r = ((long)i * j) % (2L * (N + 1));
+        S[(size_t)(i - 1) * N + (j - 1)] = 2.0 * sin(kPi * (double)r / (double)(N + 1));
+      }
+    int st = dalloc(c, dst[d], c->esz * (size_t)N * N);
+    if (!st) st = upload_typed(c, *dst[d], S.data(), S.size());
+    if (st) return st;
+  }
+  return dalloc(c, &c->wtmp, c->esz * (size_t)c->g.WTS * c->g.nl);
+}
+
 qgw_ctx *create_group(const qgw_params *ps, int nb, int precision, hipStream_t ext, int *status) {
   *status = QGW_ERR_ARG;
   if (!ps || nb < 1) return nullptr;
   if (precision != QGW_FP64 && precision != QGW_FP32 && precision != QGW_FP32_PLAIN) return nullptr;
   const qgw_params *p = &ps[0];
   qgw_grid grid;
-  int Lx, Ly;
-  if (check_params(p, &grid, &Lx, &Ly) != QGW_OK) return nullptr;
+  int Lx, Ly, dense;
+  if (check_params(p, &grid, &Lx, &Ly, &dense) != QGW_OK) return nullptr;
   for (int b = 1; b < nb; b++)
     if (!same_group(ps[0], ps[b])) return nullptr;
   const int n = grid.n, m = grid.m, nl = p->nl;
@@ -946,6 +1057,8 @@ qgw_ctx *create_group(const qgw_params *ps, int nb, int precision, hipStream_t e
     if (nb != 1) st = QGW_ERR_ARG;
     if (!st) st = setup_stochastic(c, p);
   }
+  c->dst_gemm = dense;
+  if (!st && dense) st = setup_dense(c);
   if (!st) {
     if (precision == QGW_FP64) {
       st = make_twiddles<double>(c, Lx, &c->twx);
@@ -1015,7 +1128,7 @@ qgw_ctx *create_group(const qgw_params *ps, int nb, int precision, hipStream_t e
   c->fft_e = 16;
   c->fft_e_col = (precision == QGW_FP64) ? 8 : 16;
   c->per = per ? 1 : 0;
-  c->dst_half = (!per && qgw::launch_has_half_length(Lx / 2) && qgw::launch_has_half_length(Ly / 2)) ? 2 : -1;
+  c->dst_half = (!per && !dense && qgw::launch_has_half_length(Lx / 2) && qgw::launch_has_half_length(Ly / 2)) ? 2 : -1;
   c->modes_set = false;
   // the zero fills above ran on the null stream; the step runs on c->stream
   if (hipDeviceSynchronize() != hipSuccess) {
@@ -1606,6 +1719,7 @@ int qgw_tune_set(qgw_ctx *c, int knob, int value) {
     return QGW_OK;
   }
   if (knob == QGW_TUNE_DST) {
+    if (c->dst_gemm) return value == 3 ? QGW_OK : QGW_ERR_ARG;  // dense transforms only
     if (value < 0 || value > 2 || (value >= 1 && c->dst_half < 0)) return QGW_ERR_ARG;
     if (c->dst_half >= 0) c->dst_half = value;
     return QGW_OK;
@@ -1639,7 +1753,7 @@ int qgw_tune_get(qgw_ctx *c, int knob, int *value) {
     return QGW_OK;
   }
   if (knob == QGW_TUNE_DST) {
-    *value = c->dst_half > 0 ? c->dst_half : 0;
+    *value = c->dst_gemm ? 3 : (c->dst_half > 0 ? c->dst_half : 0);
     return QGW_OK;
   }
   if (knob == QGW_TUNE_FFT_E) {
