@@ -28,10 +28,13 @@
 extern "C" {
 #endif
 
-/* Largest layer count per run the device library accepts (the layer<->mode
- * projections hold a point's layers in registers, or feed 32 x 32 matrix-core
- * tiles). */
-#define QGW_DEV_NL_MAX 32
+/* Largest layer count per run the device library accepts: the reference's
+ * nl_max (src/qg.c:45).  Up to 32 layers the layer<->mode projections hold a
+ * point's layers in registers (or feed 32 x 32 matrix-core tiles in FP32 from
+ * 16 layers); above 32 they run over 32 x 32 blocks of the matrix, each block
+ * of 32 modes stored by its first block of layers and accumulated by the
+ * others (ceil(nl/32)^2 passes over the spectral array). */
+#define QGW_DEV_NL_MAX 1000
 
 /* Arithmetic forms of the step. */
 #define QGW_FP64 0        /* every field and pass in FP64 */
@@ -62,7 +65,8 @@ typedef struct qgw_ctx qgw_ctx;
  * S_ij = 2 sin(pi i j/(N+1)) on the matrix cores through rocBLAS (O(N)
  * flops per point instead of O(log N), for any NX, NY the reference's FFTW
  * plans accept); periodic (bc_fac = -1): NX = NY even, in the FFT length
- * sets; n_hyper even and <= 8; with
+ * sets; n_hyper even and <= 18 (above 8 the update runs in its streaming form
+ * only); with
  * stochastic forcing (k_f != 0), NX = NY; with file forcing, dt_forc_period a
  * positive multiple of dt_forc > 0).  When the environment variable
  * QGW_MAX_DEVICE_BYTES holds a positive byte count, a context (or batch group)

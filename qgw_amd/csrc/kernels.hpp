@@ -140,18 +140,24 @@ k_p1(Geo g, const T *__restrict__ q, T *__restrict__ what, const cx<T> *__restri
 // cm2l[l,k] w_k, src/elliptic.h:140-147).  Both projections commute with the
 // sine transforms, so they sit where the data is transposed anyway.
 // ===========================================================================
-template <typename T, int TC, int TR, int NLC, int KB>
+// Runs over more than 32 layers take the matrix in blocks of at most NLC
+// layers and 32 modes: nl is the number of layers of this block, nk its
+// modes, ms the matrix row stride (the member's layer count); ACC adds the
+// block's contribution to the destination instead of storing it (every
+// layer block after the first).  zs, zd, zm: the member strides of src, dst
+// and M (member blockIdx.z).
+template <typename T, int TC, int TR, int NLC, int KB, bool ACC = false>
 __global__ void __launch_bounds__(256)
-k_tproj(const T *__restrict__ src, T *__restrict__ dst, const T *__restrict__ M, int nl, int R, int C, int Ps,
-        long Ss, int Pd, long Sd) {
+k_tproj(const T *__restrict__ src, T *__restrict__ dst, const T *__restrict__ M, int nl, int nk, int ms, int R,
+        int C, int Ps, long Ss, int Pd, long Sd, long zs, long zd, long zm) {
   static_assert(TC * 8 == 256 && TR % 8 == 0, "tile shape");
   constexpr int U = TR / 8;  // source rows per thread
   // KB output modes per barrier pair: their tiles are formed together, then stored together
   __shared__ T tile[KB][TR][TC + 1];
-  // member blockIdx.z: its nl layers and its own nl x nl matrix
-  src += (long)blockIdx.z * nl * Ss;
-  dst += (long)blockIdx.z * nl * Sd;
-  M += (long)blockIdx.z * nl * nl;
+  // member blockIdx.z: its layers and its own matrix
+  src += (long)blockIdx.z * zs;
+  dst += (long)blockIdx.z * zd;
+  M += (long)blockIdx.z * zm;
   const int c0 = blockIdx.x * TC, r0 = blockIdx.y * TR;
   const int tid = threadIdx.x;
   // every layer of the thread's points, lanes along c (contiguous in src):
@@ -184,17 +190,17 @@ k_tproj(const T *__restrict__ src, T *__restrict__ dst, const T *__restrict__ M,
   // store mapping: lanes along r (contiguous in dst)
   const int rl = tid % TR, cg = tid / TR;
   constexpr int CS = 256 / TR;  // columns per store pass
-  for (int k0 = 0; k0 < nl; k0 += KB) {
+  for (int k0 = 0; k0 < nk; k0 += KB) {
 #pragma unroll
     for (int kb = 0; kb < KB; kb++) {
       const int k = k0 + kb;
-      if (k >= nl) break;  // uniform
+      if (k >= nk) break;  // uniform
       // row k of the matrix: NLC uniform (scalar) loads issued together and
       // waited on once; entries past nl are clamped to a valid address and
       // meet zero layers (v[l][u] = 0 for l >= nl), so their products vanish
       T mk[NLC];
 #pragma unroll
-      for (int l = 0; l < NLC; l++) mk[l] = M[k * nl + ((l < nl) ? l : 0)];
+      for (int l = 0; l < NLC; l++) mk[l] = M[(long)k * ms + ((l < nl) ? l : 0)];
 #pragma unroll
       for (int u = 0; u < U; u++) {
         T acc = T(0);
@@ -208,12 +214,17 @@ k_tproj(const T *__restrict__ src, T *__restrict__ dst, const T *__restrict__ M,
 #pragma unroll
     for (int kb = 0; kb < KB; kb++) {
       const int k = k0 + kb;
-      if (k >= nl) break;
+      if (k >= nk) break;
       T *kd = dst + (long)k * Sd;
 #pragma unroll
       for (int cc = cg; cc < TC; cc += CS) {
         const int c = c0 + cc;
-        if (r < R && c < C) kd[c * Pd + r] = tile[kb][rl][cc];
+        if (r < R && c < C) {
+          if constexpr (ACC)
+            kd[c * Pd + r] += tile[kb][rl][cc];
+          else
+            kd[c * Pd + r] = tile[kb][rl][cc];
+        }
       }
     }
     __syncthreads();
@@ -1119,6 +1130,23 @@ __device__ __forceinline__ void p4_level(const T *src, T *dst, const T *sP, int 
 // than one period on small grids).
 __device__ __forceinline__ int wrap1(int v, int N) { return ((v - 1) % N + N) % N + 1; }
 
+// Levels LEV..NLAP-1 of the nested Laplacians: odd levels land in A, even in
+// B (A holds the largest odd level, B the largest even one).
+template <typename T, int TX, int TY, int H, bool FAST, int NLAP, int LEV, class SM>
+__device__ __forceinline__ void p4_levels(SM &sm, const T *sPp, int x0, int y0, int n, int m, T inv_d2, T ring_c,
+                                          int tx, int ty) {
+  if constexpr (LEV <= NLAP - 1) {
+    constexpr int PW = TX + 2 * H;
+    constexpr int SW = (LEV == 1) ? PW : ((LEV % 2 == 0) ? SM::AW : SM::BW);
+    constexpr int DW = (LEV % 2 == 1) ? SM::AW : SM::BW;
+    const T *src = (LEV == 1) ? sPp : ((LEV % 2 == 0) ? &sm.A[0][0] : &sm.B[0][0]);
+    T *dst = (LEV % 2 == 1) ? &sm.A[0][0] : &sm.B[0][0];
+    p4_level<T, TX, TY, H, FAST, H - LEV, SW, DW>(src, dst, sPp, x0, y0, n, m, inv_d2, ring_c, tx, ty);
+    __syncthreads();
+    p4_levels<T, TX, TY, H, FAST, NLAP, LEV + 1>(sm, sPp, x0, y0, n, m, inv_d2, ring_c, tx, ty);
+  }
+}
+
 template <typename T, int NLAP, int KAHAN, int TX, int TY, bool FAST, bool PER = false>
 __device__ __forceinline__ void p4_body(P4Smem<T, TX, TY, NLAP> &sm, const Geo &g, const T *__restrict__ P,
                                         const T *__restrict__ Q, const T *__restrict__ topo, int k, int x0, int y0,
@@ -1153,26 +1181,9 @@ __device__ __forceinline__ void p4_body(P4Smem<T, TX, TY, NLAP> &sm, const Geo &
   const T inv_d2 = (T)cf.inv_d2;
   const T ring_c = (T)cf.ring_c;
   const T *sPp = &sm.P[0][0];
-  // ---- nested Laplacians of psi down to radius 1 (calc_diff, src/dynamics.h:25-43)
-  if constexpr (NLAP >= 2) {
-    p4_level<T, TX, TY, H, FAST || PER, H - 1, PW, SM::AW>(sPp, &sm.A[0][0], sPp, x0, y0, n, m, inv_d2, ring_c, tx, ty);
-    __syncthreads();
-  }
-  if constexpr (NLAP >= 3) {
-    p4_level<T, TX, TY, H, FAST || PER, H - 2, SM::AW, SM::BW>(&sm.A[0][0], &sm.B[0][0], sPp, x0, y0, n, m, inv_d2,
-                                                         ring_c, tx, ty);
-    __syncthreads();
-  }
-  if constexpr (NLAP >= 4) {
-    p4_level<T, TX, TY, H, FAST || PER, H - 3, SM::BW, SM::AW>(&sm.B[0][0], &sm.A[0][0], sPp, x0, y0, n, m, inv_d2,
-                                                         ring_c, tx, ty);
-    __syncthreads();
-  }
-  if constexpr (NLAP >= 5) {
-    p4_level<T, TX, TY, H, FAST || PER, H - 4, SM::AW, SM::BW>(&sm.A[0][0], &sm.B[0][0], sPp, x0, y0, n, m, inv_d2,
-                                                         ring_c, tx, ty);
-    __syncthreads();
-  }
+  // ---- nested Laplacians of psi down to radius 1 (calc_diff, src/dynamics.h:25-43):
+  // level 1 from psi into A, then alternately A -> B and B -> A (radius H - level)
+  p4_levels<T, TX, TY, H, FAST || PER, NLAP, 1>(sm, sPp, x0, y0, n, m, inv_d2, ring_c, tx, ty);
   // the radius-1 level feeding the last Laplacian: A for even NLAP, B for odd
   constexpr bool lastA = (NLAP % 2) == 0;
   constexpr int LW = lastA ? SM::AW : SM::BW;

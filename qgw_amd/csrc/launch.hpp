@@ -9,8 +9,9 @@
 
 namespace qgw {
 
-// Layers per member handled by the layer<->mode projections.
-constexpr int kMaxLayers = 32;
+// Layers per member handled by the layer<->mode projections (the reference's
+// nl_max, src/qg.c:45); above 32 the projections run in 32 x 32 blocks.
+constexpr int kMaxLayers = 1000;
 
 // P4 tile shapes (columns x rows); the workgroup has 4*columns threads.  Variant
 // kP4TileCount is the streaming form (k_p4s).

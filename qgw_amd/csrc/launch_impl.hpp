@@ -204,18 +204,43 @@ constexpr int kTC = 32;
 constexpr int kTR = (sizeof(T) == 8) ? 16 : 32;
 constexpr int kTKB = 1;  // modes per barrier pair
 
-template <int NLC>
+template <int NLC, bool ACC = false>
 static void tproj_go(const LaunchArgs &a, const void *src, void *dst, const void *M, int R, int C, int Ps, long Ss,
-                     int Pd, long Sd) {
-  const dim3 grid((C + kTC - 1) / kTC, (R + kTR - 1) / kTR, a.g.nl / a.g.nlm);
-  hipLaunchKernelGGL((k_tproj<T, kTC, kTR, NLC, kTKB>), grid, dim3(256), 0, a.stream, (const T *)src, (T *)dst,
-                     (const T *)M, a.g.nlm, R, C, Ps, Ss, Pd, Sd);
+                     int Pd, long Sd, int nlc = 0, int nk = 0) {
+  const int nl = a.g.nlm;
+  const dim3 grid((C + kTC - 1) / kTC, (R + kTR - 1) / kTR, a.g.nl / nl);
+  hipLaunchKernelGGL((k_tproj<T, kTC, kTR, NLC, kTKB, ACC>), grid, dim3(256), 0, a.stream, (const T *)src, (T *)dst,
+                     (const T *)M, nlc ? nlc : nl, nk ? nk : nl, nl, R, C, Ps, Ss, Pd, Sd, (long)nl * Ss,
+                     (long)nl * Sd, (long)nl * nl);
+}
+
+// More than 32 layers: the contraction in blocks of 32 layers x 32 modes, each
+// mode block stored by its first layer block and accumulated by the others.
+static int tproj_blocks(const LaunchArgs &a, const void *src, void *dst, const void *M, int R, int C, int Ps,
+                        long Ss, int Pd, long Sd) {
+  constexpr int B = 32;
+  const int nl = a.g.nlm;
+  for (int k0 = 0; k0 < nl; k0 += B) {
+    const int nk = (nl - k0 < B) ? nl - k0 : B;
+    for (int l0 = 0; l0 < nl; l0 += B) {
+      const int nlc = (nl - l0 < B) ? nl - l0 : B;
+      const T *s0 = (const T *)src + (long)l0 * Ss;
+      T *d0 = (T *)dst + (long)k0 * Sd;
+      const T *m0 = (const T *)M + (long)k0 * nl + l0;
+      if (l0 == 0)
+        tproj_go<B, false>(a, s0, d0, m0, R, C, Ps, Ss, Pd, Sd, nlc, nk);
+      else
+        tproj_go<B, true>(a, s0, d0, m0, R, C, Ps, Ss, Pd, Sd, nlc, nk);
+    }
+  }
+  return 0;
 }
 
 static int tproj(const LaunchArgs &a, const void *src, void *dst, const void *M, int R, int C, int Ps, long Ss,
                  int Pd, long Sd) {
   const int nl = a.g.nlm;
   if (nl < 1 || a.g.nl % nl != 0 || nl > kMaxLayers) return QGW_ERR_ARG;
+  if (nl > 32) return tproj_blocks(a, src, dst, M, R, C, Ps, Ss, Pd, Sd);
   const bool mfma = sizeof(T) == 4 && (a.t_kernel >= 2 || (a.t_kernel == 0 && nl >= 16));
   if (mfma) {
     const int tcb = (a.t_kernel == 3) ? 16 : 32;
@@ -286,14 +311,19 @@ template <int NLAP, int KAHAN> static int p4_tile(const LaunchArgs &a) {
     p4_go<NLAP, KAHAN, kP4TileW[1], kP4TileH[1], true>(a);
     return 0;
   }
-  switch (a.p4_tile) {
-    case kP4TileCount: p4s_go<NLAP, KAHAN>(a); return 0;
-    case 0: p4_go<NLAP, KAHAN, kP4TileW[0], kP4TileH[0]>(a); return 0;
-    case 1: p4_go<NLAP, KAHAN, kP4TileW[1], kP4TileH[1]>(a); return 0;
-    case 2: p4_go<NLAP, KAHAN, kP4TileW[2], kP4TileH[2]>(a); return 0;
-    case 3: p4_go<NLAP, KAHAN, kP4TileW[3], kP4TileH[3]>(a); return 0;
+  if constexpr (NLAP > 5) {  // deep hyperviscosity (n_hyper > 8): the streaming form only
+    p4s_go<NLAP, KAHAN>(a);
+    return 0;
+  } else {
+    switch (a.p4_tile) {
+      case kP4TileCount: p4s_go<NLAP, KAHAN>(a); return 0;
+      case 0: p4_go<NLAP, KAHAN, kP4TileW[0], kP4TileH[0]>(a); return 0;
+      case 1: p4_go<NLAP, KAHAN, kP4TileW[1], kP4TileH[1]>(a); return 0;
+      case 2: p4_go<NLAP, KAHAN, kP4TileW[2], kP4TileH[2]>(a); return 0;
+      case 3: p4_go<NLAP, KAHAN, kP4TileW[3], kP4TileH[3]>(a); return 0;
+    }
+    return QGW_ERR_ARG;
   }
-  return QGW_ERR_ARG;
 }
 
 template <int KAHAN> static int p4_nlap(const LaunchArgs &a) {
@@ -304,6 +334,11 @@ template <int KAHAN> static int p4_nlap(const LaunchArgs &a) {
     case 3: return p4_tile<3, KAHAN>(a);
     case 4: return p4_tile<4, KAHAN>(a);
     case 5: return p4_tile<5, KAHAN>(a);
+    case 6: return p4_tile<6, KAHAN>(a);
+    case 7: return p4_tile<7, KAHAN>(a);
+    case 8: return p4_tile<8, KAHAN>(a);
+    case 9: return p4_tile<9, KAHAN>(a);
+    case 10: return p4_tile<10, KAHAN>(a);
   }
   return QGW_ERR_ARG;
 }

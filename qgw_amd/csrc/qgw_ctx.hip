@@ -693,7 +693,8 @@ int forcing_records(const qgw_params *p) {
 
 int check_params(const qgw_params *p, qgw_grid *grid, int *Lx, int *Ly, int *dense) {
   if (p->nl < 1 || p->nl > qgw::kMaxLayers) return QGW_ERR_ARG;
-  if (p->n_hyper < 0 || (p->n_hyper % 2) != 0 || p->n_hyper > 8) return QGW_ERR_ARG;
+  // n_hyper/2 + 1 nested Laplacians, at most 10 (the streaming update kernel's register windows)
+  if (p->n_hyper < 0 || (p->n_hyper % 2) != 0 || p->n_hyper > 18) return QGW_ERR_ARG;
   // stochastic forcing: the reference's N x N field (N = NX) must be the grid's
   if (p->k_f != 0 && (p->NX != p->NY || !(p->Lx > 0))) return QGW_ERR_ARG;
   if (p->dt_forc_period != 0 && forcing_records(p) < 1) return QGW_ERR_ARG;

@@ -248,6 +248,59 @@ def test_projection_matrix_cores_vs_valu(nl):
     assert rel(runs["mfma16"][1], runs["mfma"][1]) < 1e-5
 
 
+@pytest.mark.parametrize("nl", [33, 48, 70])
+def test_many_layers_vs_oracle(nl):
+    """More than 32 layers (the reference takes up to 1000, src/qg.c:45): the
+    projections run over 32 x 32 blocks of the mode matrices, a partial block
+    at 33 and 70.  FP64 inversion and 4 steps against the oracle (1e-12), and
+    the FP32 form within the FP32 bar (2e-5)."""
+    p = _small(48, nl)
+    _announce("many layers vs oracle", p.NX, nl)
+    modes = Q.eigmode(p)
+    q0 = Q.make_ic(p, modes)[1]
+    o = O.Oracle(p)
+    o.set_modes(*modes)
+    o.set_q(q0)
+    o.invert_pv()
+    psi_o = o.field(O.PSI).copy()
+    o.run(4)
+    q_o = o.field(O.Q).copy()
+    o.close()
+    for prec, tol in ((Q.FP64, 1e-12), (Q.FP32, 2e-5)):
+        m = Q.Model(p, prec, modes)
+        m.set_q(q0)
+        m.invert_pv()
+        assert rel(m.psi, psi_o) < tol, prec
+        m.run(4)
+        assert rel(m.q, q_o) < tol, prec
+        m.close()
+
+
+@pytest.mark.parametrize("n_hyper,periodic", [(10, False), (12, False), (18, False), (10, True)])
+def test_deep_hyperviscosity_vs_oracle(n_hyper, periodic):
+    """n_hyper above 8 (calc_diff nests n_hyper/2 + 1 Laplacians,
+    src/dynamics.h:25-43): the streaming update on bounded grids and the
+    wrapped tile on a periodic one, 4 steps against the oracle (1e-12)."""
+    # |lambda| dt = nu (8/Delta^2)^(n_hyper/2) dt = 0.1 at the grid scale (AB3-stable);
+    # the model sets the sign of nu_hyper itself (src/domain.h:210)
+    kw = dict(n_hyper=n_hyper, nu_hyper=0.1 * (1000e3 / 96) ** n_hyper / (8.0 ** (n_hyper // 2) * 600.0))
+    # free-slip walls: a partial-slip ring re-injected at every nested level
+    # (2 bc_fac/Delta^2 psi) diverges within a step at these depths, in the
+    # reference scheme as here
+    kw.update(bc_fac=-1.0 if periodic else 0.0)
+    p = _small(96, 2, **kw)
+    _announce("deep hyperviscosity vs oracle", n_hyper, "periodic" if periodic else "bounded")
+    m, o, _, _ = _pair(p)
+    m.run(4)
+    o.run(4)
+    assert rel(m.q, o.field(O.Q)) < 1e-12
+    m.invert_pv()
+    o.invert_pv()
+    assert rel(m.psi, o.field(O.PSI)) < 1e-12
+    m.close()
+    o.close()
+
+
 @pytest.mark.parametrize("prec", [Q.FP64, Q.FP32])
 @pytest.mark.parametrize("hyper", [(4, 0.0), (0, 1e3), (2, -1e4), (4, 1e8), (6, -1e9), (8, 1e10)])
 def test_p4_stream_paths(prec, hyper):
