@@ -262,8 +262,14 @@ const char *qgw_last_error(qgw_ctx *ctx);
  * own modes, wind (tau0, forc_mode), topography and state.  Every member needs
  * a fixed step (dt_fixed > 0).  qgw_batch_step replays a hipGraph of six
  * steps of every group (the period of the buffer rotation), captured on first
- * use; remainders run as ordinary launches.  Members are numbered in the
- * order of `params`. */
+ * use; remainders run as ordinary launches.  Members with stochastic
+ * (k_f != 0) or 4-D file forcing (dt_forc_period != 0) form groups of their
+ * own and are stepped launch by launch on the batch stream beside the
+ * replayed graph (their steps take host work: the random draws and the
+ * record reads); each starts the random stream of the C library, as a run of
+ * the reference does (qgw_batch_set_stoch_seed gives it another), and reads
+ * input_vars.nc unless qgw_batch_set_forcing_file names another file.
+ * Members are numbered in the order of `params`. */
 typedef struct qgw_batch qgw_batch;
 
 qgw_batch *qgw_batch_create(int nmembers, const qgw_params *params, int precision, int *status);
@@ -291,6 +297,9 @@ int qgw_batch_synchronize(qgw_batch *b);
 int qgw_batch_diagnostics(qgw_batch *b, int member, double *E_layer, double *Z_layer, double *E_tot,
                           double *Z_tot);
 int qgw_batch_get_time(qgw_batch *b, double *t, int *it);
+/* The forcing of one member (qgw_set_stoch_seed, qgw_set_forcing_file). */
+int qgw_batch_set_stoch_seed(qgw_batch *b, int member, unsigned seed);
+int qgw_batch_set_forcing_file(qgw_batch *b, int member, const char *path);
 /* Graph replay on (default) or off (every step launched one kernel at a time). */
 int qgw_batch_use_graph(qgw_batch *b, int on);
 /* Number of graph replays so far. */

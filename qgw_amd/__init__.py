@@ -191,6 +191,8 @@ def hip_lib():
             ("qgw_batch_upload", [vp, ci, ci, _dp]), ("qgw_batch_download", [vp, ci, ci, _dp]),
             ("qgw_batch_invert_pv", [vp]), ("qgw_batch_step", [vp, ci]),
             ("qgw_batch_time_step", [vp, ci, _dp]), ("qgw_batch_synchronize", [vp]), ("qgw_batch_prepare", [vp]),
+            ("qgw_batch_set_stoch_seed", [vp, ci, ctypes.c_uint]),
+            ("qgw_batch_set_forcing_file", [vp, ci, ctypes.c_char_p]),
             ("qgw_batch_diagnostics", [vp, ci, _dp, _dp, _dp, _dp]),
             ("qgw_batch_get_time", [vp, _dp, ip]), ("qgw_batch_use_graph", [vp, ci]),
             ("qgw_batch_graph_launches", [vp, ip]), ("qgw_batch_autotune", [vp, ci, ip]),
@@ -751,6 +753,12 @@ class Batch:
     def prepare(self):
         """Capture the graph of the current state now (no step is run)."""
         self._chk(self.lib.qgw_batch_prepare(self.b), "prepare")
+
+    def set_stoch_seed(self, member, seed):
+        self._chk(self.lib.qgw_batch_set_stoch_seed(self.b, member, int(seed)), "set_stoch_seed")
+
+    def set_forcing_file(self, member, path):
+        self._chk(self.lib.qgw_batch_set_forcing_file(self.b, member, os.fsencode(path)), "set_forcing_file")
 
     def diagnostics(self, member):
         nl = self.shapes[member][0]

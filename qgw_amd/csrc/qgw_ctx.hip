@@ -677,6 +677,8 @@ int one_step(qgw_ctx *c) {
 // the step kernels, and a fixed step (an adaptive step is per run).
 bool same_group(const qgw_params &a, const qgw_params &b) {
   if (a.dt_fixed <= 0 || b.dt_fixed <= 0) return false;
+  // forced members keep a context of their own (host-fed draws or records)
+  if (a.dt_forc_period != 0 || a.k_f != 0 || b.dt_forc_period != 0 || b.k_f != 0) return false;
   return a.NX == b.NX && a.NY == b.NY && a.nl == b.nl && a.Lx == b.Lx && a.n_hyper == b.n_hyper &&
          a.nu_hyper == b.nu_hyper && a.nu == b.nu && a.beta == b.beta && a.f0 == b.f0 && a.hEkb == b.hEkb &&
          a.dh[a.nl - 1] == b.dh[b.nl - 1] && a.bc_fac == b.bc_fac && a.dt_fixed == b.dt_fixed;
@@ -1359,9 +1361,14 @@ int bpass(qgw_batch *b, qgw_ctx *c, int st) {
   return st;
 }
 
+// A group whose step takes host work (stochastic draws, 4-D forcing records):
+// it is stepped launch by launch beside the replayed graph of the others.
+bool host_fed(const qgw_ctx *c) { return c->nst > 1 || c->nrec4 > 0; }
+
 std::vector<const void *> batch_sig(const qgw_batch *b) {
   std::vector<const void *> sig;
   for (auto *c : b->groups) {
+    if (host_fed(c)) continue;
     sig.push_back(c->fa[0]);
     sig.push_back(c->fa[1]);
     sig.push_back(c->fa[2]);
@@ -1372,11 +1379,13 @@ std::vector<const void *> batch_sig(const qgw_batch *b) {
 
 bool graph_ok(const qgw_batch *b) {
   if (!b->use_graph) return false;
+  int graphed = 0;
   for (auto *c : b->groups) {
     const double dt = c->p.dt_fixed;
     if (c->prof || !(dt > 0) || c->dt != dt || c->dt0 != dt || c->dt1 != dt || !c->modes_set) return false;
+    if (!host_fed(c)) graphed++;
   }
-  return true;
+  return graphed > 0;
 }
 
 int batch_eager_step(qgw_batch *b) {
@@ -1428,16 +1437,19 @@ int batch_capture(qgw_batch *b, hipGraphExec_t *exec) {
   const auto sig0 = batch_sig(b);
   int st = batch_branch_streams(b);
   if (st) return st;
-  const bool branches = b->groups.size() > 1;
+  std::vector<qgw_ctx *> gg;  // the groups the graph holds
+  for (auto *c : b->groups)
+    if (!host_fed(c)) gg.push_back(c);
+  const bool branches = gg.size() > 1;
   hipError_t e = hipStreamBeginCapture(b->stream, hipStreamCaptureModeRelaxed);
   if (e != hipSuccess) return bfail(b, e, "hipStreamBeginCapture");
   if (!branches) {
-    for (int s = 0; s < kGraphSteps && !st; s++) st = batch_eager_step(b);
+    for (int s = 0; s < kGraphSteps && !st; s++) st = bpass(b, gg[0], one_step(gg[0]));
   } else {
     e = hipEventRecord(b->gfork, b->stream);
     if (e != hipSuccess) st = bfail(b, e, "hipEventRecord");
-    for (size_t i = 0; i < b->groups.size() && !st; i++) {
-      qgw_ctx *c = b->groups[i];
+    for (size_t i = 0; i < gg.size() && !st; i++) {
+      qgw_ctx *c = gg[i];
       e = hipStreamWaitEvent(b->gstreams[i], b->gfork, 0);
       if (e != hipSuccess) {
         st = bfail(b, e, "hipStreamWaitEvent");
@@ -1496,11 +1508,19 @@ int batch_run(qgw_batch *b, int nsteps) {
     hipError_t e = hipGraphLaunch(exec, b->stream);
     if (e != hipSuccess) return bfail(b, e, "hipGraphLaunch");
     b->graph_launches++;
-    for (auto *c : b->groups)
+    for (auto *c : b->groups) {
+      if (host_fed(c)) {  // its steps, launch by launch, after the replay on the same stream
+        for (int s = 0; s < kGraphSteps; s++) {
+          st = one_step(c);
+          if (st) return bpass(b, c, st);
+        }
+        continue;
+      }
       for (int s = 0; s < kGraphSteps; s++) {
         c->t = c->t + c->dt;
         c->it++;
       }
+    }
     nsteps -= kGraphSteps;
   }
   for (int s = 0; s < nsteps; s++) {
@@ -1813,9 +1833,12 @@ qgw_batch *qgw_batch_create(int nmembers, const qgw_params *params, int precisio
   if (!status) status = &dummy;
   *status = QGW_ERR_ARG;
   if (nmembers < 1 || !params) return nullptr;
-  for (int i = 0; i < nmembers; i++)
-    if (!(params[i].dt_fixed > 0) || params[i].dt_forc_period != 0 || params[i].k_f != 0)
-      return nullptr;  // the graph step needs a fixed dt and no host-fed forcing
+  for (int i = 0; i < nmembers; i++) {
+    if (!(params[i].dt_fixed > 0)) return nullptr;  // the graph step needs a fixed dt
+    qgw_grid g;
+    int lx, ly, dense;
+    if (check_params(&params[i], &g, &lx, &ly, &dense) != QGW_OK) return nullptr;  // before any device call
+  }
   qgw_batch *b = new qgw_batch();
   memset(b->err, 0, sizeof b->err);
   b->nmem = nmembers;
@@ -1948,6 +1971,20 @@ int qgw_batch_prepare(qgw_batch *b) {
   if (!graph_ok(b)) return QGW_OK;
   hipGraphExec_t exec = nullptr;
   return batch_exec(b, &exec);
+}
+
+int qgw_batch_set_stoch_seed(qgw_batch *b, int member, unsigned seed) {
+  if (!b || member < 0 || member >= b->nmem) return QGW_ERR_ARG;
+  qgw_ctx *c = b->groups[b->gof[member]];
+  int st = qgw_set_stoch_seed(c, seed);
+  return st ? bpass(b, c, st) : QGW_OK;
+}
+
+int qgw_batch_set_forcing_file(qgw_batch *b, int member, const char *path) {
+  if (!b || member < 0 || member >= b->nmem) return QGW_ERR_ARG;
+  qgw_ctx *c = b->groups[b->gof[member]];
+  int st = qgw_set_forcing_file(c, path);
+  return st ? bpass(b, c, st) : QGW_OK;
 }
 
 int qgw_batch_synchronize(qgw_batch *b) {

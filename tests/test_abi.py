@@ -60,8 +60,8 @@ def test_batch_rejects_adaptive_step():
 def test_forcing_parameters_checked_before_device():
     """File forcing needs dt_forc_period a positive multiple of dt_forc > 0
     (src/forcing.h:238-251); stochastic forcing needs the square grid of the
-    reference's N x N transform; a batch takes neither.  All rejected before
-    any device call."""
+    reference's N x N transform; the same checks hold for batch members.
+    All rejected before any device call."""
     lib = Q.hip_lib()
     st = ctypes.c_int(0)
     for kw in (dict(dt_forc=0.0, dt_forc_period=100.0), dict(dt_forc=30.0, dt_forc_period=100.0),
@@ -70,7 +70,7 @@ def test_forcing_parameters_checked_before_device():
         p = Q.default_params(**kw)
         assert not lib.qgw_create(ctypes.byref(p), Q.FP64, ctypes.byref(st)), kw
         assert st.value == -1, kw
-    for kw in (dict(dt_forc=50.0, dt_forc_period=100.0), dict(k_f=10.0, sigma_f=1.0)):
-        p = Q.default_params(NX=64, NY=64, nl=2, dt_fixed=600.0, **kw)
+    for kw in (dict(dt_forc=30.0, dt_forc_period=100.0), dict(NY=48, k_f=10.0, sigma_f=1.0)):
+        p = Q.default_params(**dict(dict(NX=64, NY=64, nl=2, dt_fixed=600.0), **kw))
         assert not lib.qgw_batch_create(1, ctypes.byref(p), Q.FP64, ctypes.byref(st))
         assert st.value == -1

@@ -190,3 +190,54 @@ def test_autotune_choices_persist_across_objects():
         assert m2.tune_get(knob) == m1.tune_get(knob) == int(first[str(knob)])
     m1.close()
     m2.close()
+
+
+def test_batch_with_forced_members(tmp_path):
+    """A batch whose members include a stochastically forced run and a 4-D
+    file-forced run beside two plain ones: the forced members get groups of
+    their own, stepped launch by launch beside the replayed graph of the
+    plain group.  Every member matches the oracle after 12 steps (1e-12),
+    the forced ones with the same realisation and records."""
+    from scipy.io import netcdf_file
+    base = dict(NX=64, NY=64, nl=2, Lx=1000e3, dh=[500.0, 2500.0], N2=[2e-5], f0=1e-4, beta=2e-11, hEkb=3.0,
+                n_hyper=4, nu_hyper=1e8, tau0=5e-5, forc_mode=2.0, bc_fac=0.0, dt_fixed=600.0, ic_urms=0.1,
+                ic_lambda0=150e3)
+    ps = [Q.default_params(**dict(base, seed=11)), Q.default_params(**dict(base, seed=12)),
+          Q.default_params(**dict(base, seed=13, k_f=2 * np.pi * 6 / 1000e3, sigma_f=2e-12)),
+          Q.default_params(**dict(base, seed=14, dt_forc=1000.0, dt_forc_period=3000.0))]
+    _announce("batch with forced members", len(ps))
+    g = Q.grid(ps[3])[0]
+    recs = (2e-10 * np.random.default_rng(3).standard_normal((3, 2, g.nyp2, g.nxp2))).astype(np.float32)
+    recs = recs.astype(np.float64)
+    path = tmp_path / "forc.nc"
+    f = netcdf_file(str(path), "w")
+    f.createDimension("t", None)
+    f.createDimension("level", 2)
+    f.createDimension("y", g.nyp2)
+    f.createDimension("x", g.nxp2)
+    v = f.createVariable("q_forcing_3d", "f", ("t", "level", "y", "x"))
+    for r in range(3):
+        v[r] = recs[r]
+    f.close()
+    modes = [Q.eigmode(p) for p in ps]
+    ics = [Q.make_ic(p, mo)[1] for p, mo in zip(ps, modes)]
+    b = Q.Batch(ps, Q.FP64, modes)
+    assert b.groups()[0] == 3
+    b.set_forcing_file(3, str(path))
+    for i, q in enumerate(ics):
+        b.set_q(i, q)
+    b.step(12)
+    assert b.graph_launches() == 2
+    got = [b.q(i) for i in range(len(ps))]
+    b.close()
+    for i, p in enumerate(ps):
+        o = O.Oracle(p)
+        o.set_modes(*modes[i])
+        o.set_q(ics[i])
+        if i == 2:
+            o.set_stochastic(p.k_f, p.sigma_f)
+        if i == 3:
+            o.set_forcing4d(recs, p.dt_forc)
+        o.run(12)
+        assert rel(got[i], o.field(O.Q)) < 1e-12, i
+        o.close()
