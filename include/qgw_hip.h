@@ -224,6 +224,19 @@ int qgw_profile_read(qgw_ctx *ctx, double *ms, int *count, int npass);
  * extension, length 2(n+1) (2(m+1)); 3 (reported, and the only choice) for
  * the dense sine transforms of grids the FFT lengths do not cover. */
 #define QGW_TUNE_DST 4
+/* QGW_TUNE_FUSE: 1 runs the inversion in four passes over HBM -- the
+ * layer<->mode projections inside the row transforms, the column transforms
+ * on 64-byte column tiles of the mode-major spectral array, no transposed
+ * copy (where the half-length transforms apply on a bounded grid); 0, the
+ * default, in six: the projections fused into two transposes.  Fewer bytes,
+ * but at 2048^2 x 10 the column-tile pass is slower than the transposed
+ * column pass by more than the two transposes cost; the autotuner tries
+ * both. */
+#define QGW_TUNE_FUSE 7
+/* QGW_TUNE_P2M_PAIRS: column pairs per block of the four-pass column pass,
+ * 1, 2, 4 or 8 (capped by the 1024-thread block and the LDS; 0, the default:
+ * the most that fit, 64-byte row segments). */
+#define QGW_TUNE_P2M_PAIRS 8
 /* QGW_TUNE_FFT_E: radix schedule of the half-length FFTs of power-of-two
  * lengths from 1024: 16 (radix-16 stages; the default) or 8 (radix-8 stages,
  * twice the threads per row), for the row passes P1 and P3. */

@@ -54,9 +54,12 @@ __device__ __forceinline__ double wave_incl_scan(double s) {
 // (the real and imaginary parts are independent sums).  The DPP moves carry
 // values from lower to higher lanes only, so a partial last wavefront (NT not
 // a multiple of 64) sums correctly in its active lanes.
-template <int NT> __device__ __forceinline__ cx<double> block_excl_scan(cx<double> v, int t) {
+// wtot: NT/64 entries of LDS for the wavefront totals (its own per transform
+// when a block runs several transforms side by side); null: a static array.
+template <int NT> __device__ __forceinline__ cx<double> block_excl_scan(cx<double> v, int t, cx<double> *wext) {
   constexpr int NW = (NT + 63) / 64;
-  __shared__ cx<double> wtot[NW];
+  __shared__ cx<double> wown[NW];
+  cx<double> *wtot = wext ? wext : wown;
   const int lane = t & 63, w = t >> 6;
   const double sx = wave_incl_scan(v.x), sy = wave_incl_scan(v.y);
   cx<double> e = mkc<double>(dpp_d<0x138, 0xF>(sx), dpp_d<0x138, 0xF>(sy));  // wave_shr:1: lane i - 1's sum
@@ -174,7 +177,7 @@ __device__ __forceinline__ void dst_pre_lds(cx<T> *h, int t, const cx<T> *buf, c
 // order: entry k - 1 + SH holds S_k (k = 1..N) at rslot(k - 1 + SH); with SH = 1
 // entry 0 holds no coefficient (zero or unwritten).  Ends with a barrier.
 template <typename T, int M, int EP, int SH = 0>
-__device__ __forceinline__ void dst_post(cx<T> *h, cx<T> *buf, int t) {
+__device__ __forceinline__ void dst_post(cx<T> *h, cx<T> *buf, int t, cx<double> *wtot = nullptr) {
   constexpr int E = DstHalf<M, EP>::E, NT = DstHalf<M, EP>::NT, KPT = DstHalf<M, EP>::KPT;
   // Addresses as a per-thread base plus compile-time offsets where the pad
   // allows (rslot(B + c) = rslot(B) + c + c/16 while B mod 16 + c mod 16 < 16):
@@ -216,7 +219,7 @@ __device__ __forceinline__ void dst_post(cx<T> *h, cx<T> *buf, int t) {
     run = cadd<double>(run, r);
     cum[i] = run;
   }
-  const cx<double> off = block_excl_scan<NT>(run, t);
+  const cx<double> off = block_excl_scan<NT>(run, t, wtot);
   const int r2 = rslot(2 * KPT * t);  // entry of 2k for i = 0
   __syncthreads();  // every read of Y is done
 #pragma unroll

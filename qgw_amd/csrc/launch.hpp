@@ -6,6 +6,7 @@
 
 #include "qgw_hip.h"
 #include "kernels.hpp"
+#include "fused.hpp"
 
 namespace qgw {
 
@@ -42,6 +43,7 @@ struct LaunchArgs {
   int p4_tile = kP4TileCount;  // P4 variant: a tile index, or kP4TileCount for streaming
   int p4_waves = 40;           // streaming P4: target wavefronts per SIMD (sets the row chunk)
   int t_kernel = 0;            // projection: 0 auto (matrix cores for FP32 at >= 16 layers), 1 VALU, 2 matrix cores
+  int p2m_g = 0;               // four-pass column pass: column pairs per block (0: the most that fit)
   unsigned long long *cfl_max = nullptr;
 };
 
@@ -52,6 +54,11 @@ template <typename T> int launch_p2_t(const LaunchArgs &a);
 template <typename T> int launch_t2_t(const LaunchArgs &a);
 template <typename T> int launch_p3_t(const LaunchArgs &a);
 template <typename T> int launch_p4_t(const LaunchArgs &a);
+// The four-pass inversion (fused.hpp): projections inside the row passes,
+// column tiles in place on a.what.
+template <typename T> int launch_p1m_t(const LaunchArgs &a);
+template <typename T> int launch_p2m_t(const LaunchArgs &a);
+template <typename T> int launch_p3m_t(const LaunchArgs &a);
 template <typename T> int launch_cfl_t(const LaunchArgs &a);
 template <typename T> int launch_diag_t(const LaunchArgs &a, double *part, int nblk, double *out);
 template <typename T> int launch_from_f64_t(const LaunchArgs &a, int layers, const double *src, void *dst, void *resid);
@@ -83,6 +90,9 @@ bool launch_has_half_length(int M);  // lengths M = L/2 of the half-length trans
   template <> int launch_t2_t<T>(const LaunchArgs &a);                                                \
   template <> int launch_p3_t<T>(const LaunchArgs &a);                                                \
   template <> int launch_p4_t<T>(const LaunchArgs &a);                                                \
+  template <> int launch_p1m_t<T>(const LaunchArgs &a);                                               \
+  template <> int launch_p2m_t<T>(const LaunchArgs &a);                                               \
+  template <> int launch_p3m_t<T>(const LaunchArgs &a);                                               \
   template <> int launch_cfl_t<T>(const LaunchArgs &a);                                               \
   template <> int launch_diag_t<T>(const LaunchArgs &a, double *part, int nblk, double *out);         \
   template <> int launch_from_f64_t<T>(const LaunchArgs &a, int layers, const double *src, void *dst, \
@@ -107,6 +117,9 @@ inline int launch_p2(const LaunchArgs &a) { return QGW_DISPATCH(launch_p2, a); }
 inline int launch_t2(const LaunchArgs &a) { return QGW_DISPATCH(launch_t2, a); }
 inline int launch_p3(const LaunchArgs &a) { return QGW_DISPATCH(launch_p3, a); }
 inline int launch_p4(const LaunchArgs &a) { return QGW_DISPATCH(launch_p4, a); }
+inline int launch_p1m(const LaunchArgs &a) { return QGW_DISPATCH(launch_p1m, a); }
+inline int launch_p2m(const LaunchArgs &a) { return QGW_DISPATCH(launch_p2m, a); }
+inline int launch_p3m(const LaunchArgs &a) { return QGW_DISPATCH(launch_p3m, a); }
 inline int launch_cfl(const LaunchArgs &a) { return QGW_DISPATCH(launch_cfl, a); }
 inline int launch_diag(const LaunchArgs &a, double *part, int nblk, double *out) {
   return QGW_DISPATCH(launch_diag, a, part, nblk, out);

@@ -198,6 +198,83 @@ template <> int launch_p2_t<T>(const LaunchArgs &a) {
   return QGW_ERR_ARG;
 }
 
+// ---- four-pass inversion ---------------------------------------------------
+template <int M, int EP> static void p1m_go(const LaunchArgs &a) {
+  static int occ = 0;
+  const long items = (long)a.g.m * (a.g.nl / a.g.nlm) * ((a.g.nlm + 1) / 2);
+  const unsigned pg = persist_grid(k_p1m<T, M, EP>, DstHalf<M, EP>::NT, items, a.cus, occ);
+  hipLaunchKernelGGL((k_p1m<T, M, EP>), dim3(pg), dim3(DstHalf<M, EP>::NT), 0, a.stream, a.g, (const T *)a.q,
+                     (T *)a.what, (const T *)a.cl2m, (const cx<T> *)a.twxh, (const cx<T> *)a.twx);
+}
+template <int M, int EP> static void p3m_go(const LaunchArgs &a) {
+  static int occ = 0;
+  const long items = (long)a.g.m * (a.g.nl / a.g.nlm) * ((a.g.nlm + 1) / 2);
+  const unsigned pg = persist_grid(k_p3m<T, M, EP>, DstHalf<M, EP>::NT, items, a.cus, occ);
+  hipLaunchKernelGGL((k_p3m<T, M, EP>), dim3(pg), dim3(DstHalf<M, EP>::NT), 0, a.stream, a.g, (const T *)a.what,
+                     (T *)a.psi, (T *)a.q, (const T *)a.cm2l, (T)a.ring_c, (const cx<T> *)a.twxh,
+                     (const cx<T> *)a.twx);
+}
+template <int M, int EP, int G> static void p2m_launch(const LaunchArgs &a) {
+  const int ncg = (a.g.n + 2 * G - 1) / (2 * G);
+  const long tiles = (long)ncg * a.g.nl;
+  hipLaunchKernelGGL((k_p2m<T, M, EP, G>), dim3((unsigned)tiles), dim3(G * DstHalf<M, EP>::NT), 0, a.stream, a.g,
+                     (T *)a.what, a.La2, a.iRd2, a.dk, a.norm, (const cx<T> *)a.twyh, (const cx<T> *)a.twy, ncg);
+}
+// a.p2m_g column pairs per block, capped by what the block size and LDS allow
+template <int M, int EP> static void p2m_go(const LaunchArgs &a) {
+  constexpr int GM = p2m_groups<T, M, EP>();
+  const int want = a.p2m_g > 0 ? a.p2m_g : GM;
+  if (want >= 8 && GM >= 8) p2m_launch<M, EP, (GM >= 8 ? 8 : 1)>(a);
+  else if (want >= 4 && GM >= 4) p2m_launch<M, EP, (GM >= 4 ? 4 : 1)>(a);
+  else if (want >= 2 && GM >= 2) p2m_launch<M, EP, (GM >= 2 ? 2 : 1)>(a);
+  else p2m_launch<M, EP, 1>(a);
+}
+
+template <> int launch_p1m_t<T>(const LaunchArgs &a) {
+  if (a.per || a.g.nlm < 1 || a.g.nl % a.g.nlm) return QGW_ERR_ARG;
+  switch (a.Lx / 2) {
+#define QGW_GO(M, EP) p1m_go<M, EP>(a);
+#define QGW_CASE(M) \
+  case M:           \
+    QGW_HALF_EP(M)  \
+    return 0;
+    QGW_HLENGTHS(QGW_CASE)
+#undef QGW_CASE
+#undef QGW_GO
+  }
+  return QGW_ERR_ARG;
+}
+
+template <> int launch_p3m_t<T>(const LaunchArgs &a) {
+  if (a.per || a.g.nlm < 1 || a.g.nl % a.g.nlm) return QGW_ERR_ARG;
+  switch (a.Lx / 2) {
+#define QGW_GO(M, EP) p3m_go<M, EP>(a);
+#define QGW_CASE(M) \
+  case M:           \
+    QGW_HALF_EP(M)  \
+    return 0;
+    QGW_HLENGTHS(QGW_CASE)
+#undef QGW_CASE
+#undef QGW_GO
+  }
+  return QGW_ERR_ARG;
+}
+
+template <> int launch_p2m_t<T>(const LaunchArgs &a) {
+  if (a.per) return QGW_ERR_ARG;
+  switch (a.Ly / 2) {
+#define QGW_GO(M, EP) p2m_go<M, EP>(a);
+#define QGW_CASE(M)                \
+  case M:                          \
+    QGW_HALF_EP_E(M, a.fft_e_col)  \
+    return 0;
+    QGW_HLENGTHS(QGW_CASE)
+#undef QGW_CASE
+#undef QGW_GO
+  }
+  return QGW_ERR_ARG;
+}
+
 // Transpose tiles: 32 columns (contiguous in the source) x kTR rows
 // (contiguous in the destination): 128-byte segments on both sides.
 constexpr int kTC = 32;

@@ -74,6 +74,8 @@ struct qgw_ctx {
   double dk;     // y wavenumber step of the bounded transforms, pi / Lx (src/domain.h:170-173)
   int fft_e;     // elements per thread of the half-length FFTs of the row passes (QGW_TUNE_FFT_E)
   int fft_e_col; // and of the column pass (QGW_TUNE_FFT_E_COL)
+  int fuse;      // QGW_TUNE_FUSE: 1 four-pass inversion, 0 six passes; -1: four passes not available
+  int p2m_g;     // QGW_TUNE_P2M_PAIRS: column pairs per block of the four-pass column pass (0: most)
   int per;       // periodic domain (bc_fac = -1)
   // dense sine transforms (bounded grids whose transform lengths the FFT
   // kernels do not cover): S matrices n x n and m x m, a whatT-shaped temporary
@@ -402,6 +404,7 @@ int run_invert(qgw_ctx *c) {
   a.dst_half = c->dst_half > 0 ? c->dst_half : 0;
   a.fft_e = c->fft_e;
   a.fft_e_col = c->fft_e_col;
+  a.p2m_g = c->p2m_g;
   a.cus = c->cus;
   a.dk = c->dk;
   a.per = c->per;
@@ -410,6 +413,24 @@ int run_invert(qgw_ctx *c) {
   a.ring_c = 2.0 * c->p.bc_fac / (c->grid.Delta * c->grid.Delta);
   if (c->dst_gemm) return run_invert_dense(c, a);
   int st;
+  if (c->fuse == 1 && a.dst_half > 0) {
+    {
+      PassScope ps(c, QGW_PASS_P1);
+      st = qgw::launch_p1m(a);
+    }
+    if (st || (st = check_launch(c, "p1m"))) return st;
+    {
+      PassScope ps(c, QGW_PASS_P2);
+      st = qgw::launch_p2m(a);
+    }
+    if (st || (st = check_launch(c, "p2m"))) return st;
+    {
+      PassScope ps(c, QGW_PASS_P3);
+      st = qgw::launch_p3m(a);
+    }
+    if (st || (st = check_launch(c, "p3m"))) return st;
+    return QGW_OK;
+  }
   {
     PassScope ps(c, QGW_PASS_P1);
     st = qgw::launch_p1(a);
@@ -1132,6 +1153,7 @@ qgw_ctx *create_group(const qgw_params *ps, int nb, int precision, hipStream_t e
   c->fft_e_col = (precision == QGW_FP64) ? 8 : 16;
   c->per = per ? 1 : 0;
   c->dst_half = (!per && !dense && qgw::launch_has_half_length(Lx / 2) && qgw::launch_has_half_length(Ly / 2)) ? 2 : -1;
+  c->fuse = (c->dst_half > 0) ? 0 : -1;  // four passes measured slower at 2048^2 x 10; the autotuner tries both
   c->modes_set = false;
   // the zero fills above ran on the null stream; the step runs on c->stream
   if (hipDeviceSynchronize() != hipSuccess) {
@@ -1257,7 +1279,23 @@ int autotune_group(qgw_ctx *c, int reps, int *p4_tile, double *best_ms) {
   }
   c->p4_waves = warg;
   c->p4_tile = arg;
-  if (!st && c->prec != QGW_FP64) {
+  if (!st && c->fuse >= 0) {
+    // the inversion in four passes or in six
+    double bf = 1e300;
+    int farg = c->fuse;
+    for (int v = 0; v <= 1 && !st; v++) {
+      c->fuse = v;
+      double ms = 0;
+      st = time_steps(c, 1, &ms);  // first-launch code loading
+      if (!st) st = time_candidate(c, reps, &ms);
+      if (!st && ms < bf) {
+        bf = ms;
+        farg = v;
+      }
+    }
+    c->fuse = farg;
+  }
+  if (!st && c->prec != QGW_FP64 && c->fuse != 1) {
     // projection: VALU or matrix cores
     double bt = 1e300;
     int targ = c->t_kernel;
@@ -1745,6 +1783,16 @@ int qgw_tune_set(qgw_ctx *c, int knob, int value) {
     if (c->dst_half >= 0) c->dst_half = value;
     return QGW_OK;
   }
+  if (knob == QGW_TUNE_P2M_PAIRS) {
+    if (value != 0 && value != 1 && value != 2 && value != 4 && value != 8) return QGW_ERR_ARG;
+    c->p2m_g = value;
+    return QGW_OK;
+  }
+  if (knob == QGW_TUNE_FUSE) {
+    if (value < 0 || value > 1 || (value == 1 && c->fuse < 0)) return QGW_ERR_ARG;
+    if (c->fuse >= 0) c->fuse = value;
+    return QGW_OK;
+  }
   if (knob == QGW_TUNE_P4_WAVES) {
     if (value != 2 && value != 4 && value != 8 && value != 16 && value != 32 && value != 40 && value != 48)
       return QGW_ERR_ARG;
@@ -1775,6 +1823,14 @@ int qgw_tune_get(qgw_ctx *c, int knob, int *value) {
   }
   if (knob == QGW_TUNE_DST) {
     *value = c->dst_gemm ? 3 : (c->dst_half > 0 ? c->dst_half : 0);
+    return QGW_OK;
+  }
+  if (knob == QGW_TUNE_FUSE) {
+    *value = c->fuse > 0 ? 1 : 0;
+    return QGW_OK;
+  }
+  if (knob == QGW_TUNE_P2M_PAIRS) {
+    *value = c->p2m_g;
     return QGW_OK;
   }
   if (knob == QGW_TUNE_FFT_E) {

@@ -14,7 +14,8 @@ from conftest import rel
 pytestmark = pytest.mark.gpu
 
 # every knob the autotuner sets
-KNOBS = (Q.TUNE_P4_TILE, Q.TUNE_P4_WAVES, Q.TUNE_T_KERNEL, Q.TUNE_DST, Q.TUNE_FFT_E, Q.TUNE_FFT_E_COL)
+KNOBS = (Q.TUNE_P4_TILE, Q.TUNE_P4_WAVES, Q.TUNE_T_KERNEL, Q.TUNE_DST, Q.TUNE_FFT_E, Q.TUNE_FFT_E_COL,
+         Q.TUNE_FUSE)
 
 
 def _announce(*a):
