@@ -1152,7 +1152,14 @@ qgw_ctx *create_group(const qgw_params *ps, int nb, int precision, hipStream_t e
   c->fft_e = 16;
   c->fft_e_col = (precision == QGW_FP64) ? 8 : 16;
   c->per = per ? 1 : 0;
-  c->dst_half = (!per && !dense && qgw::launch_has_half_length(Lx / 2) && qgw::launch_has_half_length(Ly / 2)) ? 2 : -1;
+  // the pipelined row passes by default, except FP64 rows of a 3 2^k length:
+  // there the full LDS twiddle table of the Stockham leaves one block per CU,
+  // and the one-block form was measured faster (1536^2 x 10: 0.849 against
+  // 0.870 ms per step; FP32 the other way, 0.471 against 0.482)
+  const bool rows_pow2 = ((Lx / 2) & (Lx / 2 - 1)) == 0;
+  c->dst_half = (!per && !dense && qgw::launch_has_half_length(Lx / 2) && qgw::launch_has_half_length(Ly / 2))
+                    ? ((precision == QGW_FP64 && !rows_pow2) ? 1 : 2)
+                    : -1;
   c->fuse = (c->dst_half > 0) ? 0 : -1;  // four passes measured slower at 2048^2 x 10; the autotuner tries both
   c->modes_set = false;
   // the zero fills above ran on the null stream; the step runs on c->stream

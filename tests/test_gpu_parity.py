@@ -526,3 +526,19 @@ def test_pipelined_rows_3x2k_fp32(NX, nl):
         m.close()
         assert rel(out[form], q_o) < 2e-5, form
     assert rel(out[2], out[1]) < 1e-6
+
+
+@pytest.mark.parametrize("NX,prec,form", [(1536, Q.FP64, 1), (1536, Q.FP32, 2), (2048, Q.FP64, 2), (2048, Q.FP32, 2),
+                                          (384, Q.FP64, 1), (96, Q.FP64, 1), (64, Q.FP64, 2)])
+def test_default_sine_transform_form(NX, prec, form):
+    """Default row-transform form: the pipelined persistent blocks, except
+    FP64 rows of a 3*2^k half length, which take one block per row pair (the
+    faster of the two there); both forms stay selectable."""
+    p = _small(NX, 2)
+    _announce("default sine-transform form", NX, prec)
+    m = Q.Model(p, prec, Q.eigmode(p))
+    assert m.tune_get(Q.TUNE_DST) == form
+    for f in (1, 2):
+        m.tune_set(Q.TUNE_DST, f)
+        assert m.tune_get(Q.TUNE_DST) == f
+    m.close()
