@@ -234,8 +234,9 @@ int qgw_profile_read(qgw_ctx *ctx, double *ms, int *count, int npass);
  * both. */
 #define QGW_TUNE_FUSE 7
 /* QGW_TUNE_P2M_PAIRS: column pairs per block of the four-pass column pass,
- * 1, 2, 4 or 8 (capped by the 1024-thread block and the LDS; 0, the default:
- * the most that fit, 64-byte row segments). */
+ * 1, 2, 4 or 8 (capped by the 1024-thread block and the LDS; 0: the most
+ * that fit, 64-byte row segments).  Default: 4 in the FP32 forms, 2 in FP64,
+ * the fastest measured at 2048^2 x 10; the autotuner tries 2, 4 and 8. */
 #define QGW_TUNE_P2M_PAIRS 8
 /* QGW_TUNE_FFT_E: radix schedule of the half-length FFTs of power-of-two
  * lengths from 1024: 16 (radix-16 stages; the default) or 8 (radix-8 stages,

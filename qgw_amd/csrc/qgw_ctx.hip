@@ -1161,6 +1161,10 @@ qgw_ctx *create_group(const qgw_params *ps, int nb, int precision, hipStream_t e
                     ? ((precision == QGW_FP64 && !rows_pow2) ? 1 : 2)
                     : -1;
   c->fuse = (c->dst_half > 0) ? 0 : -1;  // four passes measured slower at 2048^2 x 10; the autotuner tries both
+  // column pairs per block of the four-pass column pass: the fastest measured
+  // at 2048^2 x 10 (FP32 4: 180 us against 191 for 8 and 194 for 2; FP64 2:
+  // 357 us against 390 for 4)
+  c->p2m_g = (precision == QGW_FP64) ? 2 : 4;
   c->modes_set = false;
   // the zero fills above ran on the null stream; the step runs on c->stream
   if (hipDeviceSynchronize() != hipSuccess) {
@@ -1287,20 +1291,25 @@ int autotune_group(qgw_ctx *c, int reps, int *p4_tile, double *best_ms) {
   c->p4_waves = warg;
   c->p4_tile = arg;
   if (!st && c->fuse >= 0) {
-    // the inversion in four passes or in six
+    // the inversion in six passes, or in four with 2, 4 or 8 column pairs per
+    // block of the column pass
     double bf = 1e300;
-    int farg = c->fuse;
-    for (int v = 0; v <= 1 && !st; v++) {
-      c->fuse = v;
+    int farg = c->fuse, garg = c->p2m_g;
+    const int forms[4][2] = {{0, 0}, {1, 2}, {1, 4}, {1, 8}};
+    for (int v = 0; v < 4 && !st; v++) {
+      c->fuse = forms[v][0];
+      if (forms[v][0]) c->p2m_g = forms[v][1];
       double ms = 0;
       st = time_steps(c, 1, &ms);  // first-launch code loading
       if (!st) st = time_candidate(c, reps, &ms);
       if (!st && ms < bf) {
         bf = ms;
-        farg = v;
+        farg = c->fuse;
+        garg = c->p2m_g;
       }
     }
     c->fuse = farg;
+    c->p2m_g = garg;
   }
   if (!st && c->prec != QGW_FP64 && c->fuse != 1) {
     // projection: VALU or matrix cores
