@@ -265,17 +265,24 @@ k_p2m(Geo g, T *__restrict__ w, const double *__restrict__ La2, const double *__
   T *base = w + (long)k * g.WS;
   TwTabFill<T, 2 * M, BT> tfill;
   tfill.load(tw2, tid);
-  // stage the tile: lanes along the 2G columns of a row, BT / 2G rows per pass
-  const int c = tid % W, r0 = tid / W;
-  constexpr int RS = BT / W;
-  const bool cok = a0 + c < n;
-  T *bcol = reinterpret_cast<T *>(&bufs[c >> 1][0]) + (c & 1);
+  // stage the tile: lanes along the G column pairs of a row, one pair (both
+  // columns, 2 sizeof(T) bytes) per lane, BT / G rows per pass; a pair's two
+  // columns land in one complex entry of its buffer
+  const int cp = tid % G, r0 = tid / G;
+  constexpr int RS = BT / G;
+  const bool cok = a0 + 2 * cp < n, cok1 = a0 + 2 * cp + 1 < n;
+  cx<T> *bpair = bufs[cp];
   {
-    const T *src = base + a0 + c;
+    const T *src = base + a0 + 2 * cp;
 #pragma unroll 8
     for (int j = r0; j < m; j += RS) {
-      const T v = cok ? src[(long)j * g.Pn] : T(0);
-      bcol[2 * rslot(j + 1)] = v;  // entry y = j + 1 holds z_y
+      const T *e = src + (long)j * g.Pn;
+      cx<T> v = mkc<T>(T(0), T(0));
+      if (cok1)
+        v = *reinterpret_cast<const cx<T> *>(e);
+      else if (cok)
+        v.x = e[0];
+      bpair[rslot(j + 1)] = v;  // entry y = j + 1 holds z_y
     }
   }
   tfill.store(tb2, tid);
@@ -330,11 +337,15 @@ k_p2m(Geo g, T *__restrict__ w, const double *__restrict__ La2, const double *__
   fft_home<T, M, EP>(h, buf, twh, t);
   dst_post<T, M, EP>(h, buf, t, wtot[gi]);
   {
-    T *dst = base + a0 + c;
+    T *dst = base + a0 + 2 * cp;
 #pragma unroll 8
     for (int j = r0; j < m; j += RS) {
-      const T v = bcol[2 * rslot(j)];
-      if (cok) dst[(long)j * g.Pn] = v;
+      const cx<T> v = bpair[rslot(j)];
+      T *e = dst + (long)j * g.Pn;
+      if (cok1)
+        *reinterpret_cast<cx<T> *>(e) = v;
+      else if (cok)
+        e[0] = v.x;
     }
   }
 }
