@@ -147,6 +147,53 @@ k_p1m(Geo g, const T *__restrict__ q, T *__restrict__ what, const T *__restrict_
   }
 }
 
+// Store of a (psi row j, layer pair) from the natural-order buffer of the
+// inverse row transform (entry i holds x = i + 1): the psi rows with their
+// zero wall columns, the wall rings of psi (zero) and of q (ring_c psi of the
+// adjacent interior point, src/domain.h:308-348).
+template <typename T, int M, int EP>
+__device__ __forceinline__ void p3_store_pair(const Geo &g, const cx<T> *buf, T *__restrict__ psi, T *__restrict__ q,
+                                              T ring_c, int j, long lb, int l0, bool has1, int tl) {
+  constexpr int NT = DstHalf<M, EP>::NT;
+  const int n = g.n, m_ = g.m, l1 = l0 + 1;
+  T *p0 = psi + (lb + l0) * g.LS + (long)j * g.P;
+  T *p1 = psi + (lb + l1) * g.LS + (long)j * g.P;
+  T *q0r = q + (lb + l0) * g.LS;
+  T *q1r = q + (lb + l1) * g.LS;
+#pragma unroll 2
+  for (int i = tl; i < n; i += NT) {
+    const int x = i + 1;
+    const cx<T> c = buf[rslot(i)];
+    p0[x] = c.x;
+    if (has1) p1[x] = c.y;
+    if (x == 1 || x == n) {
+      const long o = (long)j * g.P + (x == 1 ? 0 : n + 1);
+      q0r[o] = ring_c * c.x;
+      if (has1) q1r[o] = ring_c * c.y;
+    }
+    if (j == 1 || j == m_) {
+      const long o = (long)(j == 1 ? 0 : m_ + 1) * g.P + x;
+      q0r[o] = ring_c * c.x;
+      if (has1) q1r[o] = ring_c * c.y;
+    }
+  }
+  if (tl == 0) {
+    p0[0] = T(0);
+    p0[n + 1] = T(0);
+    if (has1) {
+      p1[0] = T(0);
+      p1[n + 1] = T(0);
+    }
+  }
+  if (j == 1 || j == m_) {
+    const long rr = (long)(j == 1 ? 0 : m_ + 1) * g.P;
+    for (int x = tl; x < n + 2; x += NT) {
+      p0[rr - (long)j * g.P + x] = T(0);
+      if (has1) p1[rr - (long)j * g.P + x] = T(0);
+    }
+  }
+}
+
 template <typename T, int M, int EP>
 __global__ void __launch_bounds__((DstHalf<M, EP>::NT))
 k_p3m(Geo g, const T *__restrict__ what, T *__restrict__ psi, T *__restrict__ q, const T *__restrict__ cm2l,
@@ -157,7 +204,6 @@ k_p3m(Geo g, const T *__restrict__ what, T *__restrict__ psi, T *__restrict__ q,
   __shared__ cx<T> twf[twf_len<T, M>()];
   __shared__ TwTab<T, 2 * M> tb2;
   const int t = threadIdx.x;
-  const int n = g.n, m_ = g.m;
   const int nlm = g.nlm, nmem = g.nl / nlm, npair = (nlm + 1) / 2;
   const int nitems = g.m * nmem * npair;
   twtab_fill(tb, twh, t, NT);
@@ -183,44 +229,7 @@ k_p3m(Geo g, const T *__restrict__ what, T *__restrict__ psi, T *__restrict__ q,
     dst_pre_lds<T, M, EP>(h, tl, buf, tb2);
     fft_pipe<T, M, EP>(h, buf, twa, twf, tl);
     dst_post<T, M, EP>(h, buf, tl);
-    const int j = ri.j;
-    const long lb = (long)ri.mb * nlm;
-    T *p0 = psi + (lb + l0) * g.LS + (long)j * g.P;
-    T *p1 = psi + (lb + l1) * g.LS + (long)j * g.P;
-    T *q0r = q + (lb + l0) * g.LS;
-    T *q1r = q + (lb + l1) * g.LS;
-#pragma unroll 2
-    for (int i = tl; i < n; i += NT) {
-      const int x = i + 1;
-      const cx<T> c = buf[rslot(i)];
-      p0[x] = c.x;
-      if (has1) p1[x] = c.y;
-      if (x == 1 || x == n) {
-        const long o = (long)j * g.P + (x == 1 ? 0 : n + 1);
-        q0r[o] = ring_c * c.x;
-        if (has1) q1r[o] = ring_c * c.y;
-      }
-      if (j == 1 || j == m_) {
-        const long o = (long)(j == 1 ? 0 : m_ + 1) * g.P + x;
-        q0r[o] = ring_c * c.x;
-        if (has1) q1r[o] = ring_c * c.y;
-      }
-    }
-    if (tl == 0) {
-      p0[0] = T(0);
-      p0[n + 1] = T(0);
-      if (has1) {
-        p1[0] = T(0);
-        p1[n + 1] = T(0);
-      }
-    }
-    if (j == 1 || j == m_) {
-      const long rr = (long)(j == 1 ? 0 : m_ + 1) * g.P;
-      for (int x = tl; x < n + 2; x += NT) {
-        p0[rr - (long)j * g.P + x] = T(0);
-        if (has1) p1[rr - (long)j * g.P + x] = T(0);
-      }
-    }
+    p3_store_pair<T, M, EP>(g, buf, psi, q, ring_c, ri.j, (long)ri.mb * nlm, l0, has1, tl);
   }
 }
 
