@@ -98,6 +98,34 @@ def test_fp32_budgets_at_2048x10():
 
 
 @pytest.mark.timeout(300)
+def test_fp32_energy_gate_1000_steps_2048x10():
+    """The accuracy bar of the FP32 batched form at 2048^2 x 10: after 1000
+    steps (dt 900 s) its total energy and enstrophy are within 1e-4 of the FP64
+    form's (measured 4.0e-6 and 8.5e-6 with six passes), in the six-pass and
+    the four-pass inversion alike (the autotuner may pick either)."""
+    p = configs.c3()
+    _announce("full size fp32 energy gate, 1000 steps", p.NX, p.nl)
+    modes = Q.eigmode(p)
+    q0 = Q.make_ic(p, modes)[1]
+    m = Q.Model(p, Q.FP64, modes)
+    m.set_q(q0)
+    m.run(1000)
+    m.invert_pv()
+    E64, Z64 = m.diagnostics()[2:]
+    m.close()
+    for fuse in (0, 1):
+        b = Q.Batch([p], Q.FP32, [modes])
+        b.tune_set(0, Q.TUNE_FUSE, fuse)
+        b.set_q(0, q0)
+        b.step(1000)
+        b.invert_pv()
+        E32, Z32 = b.diagnostics(0)[2:]
+        b.close()
+        assert abs(E32 - E64) / abs(E64) < 1e-4, fuse
+        assert abs(Z32 - Z64) / abs(Z64) < 1e-4, fuse
+
+
+@pytest.mark.timeout(300)
 def test_4096x32_inversion():
     """4096^2 x 32 (2.2 GB per FP32 field): the FP64 inversion returns psi to
     the round-off of a 32-layer projection (1e-11; measured 2.4e-12, the
