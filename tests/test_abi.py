@@ -74,3 +74,16 @@ def test_forcing_parameters_checked_before_device():
         p = Q.default_params(**dict(dict(NX=64, NY=64, nl=2, dt_fixed=600.0), **kw))
         assert not lib.qgw_batch_create(1, ctypes.byref(p), Q.FP64, ctypes.byref(st))
         assert st.value == -1
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU: --gpus above 1 exits with status 2 and says
+    so, before any device call."""
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr
+    assert r.stdout == ""
