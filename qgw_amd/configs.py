@@ -99,9 +99,10 @@ C4_CLASSES = [(32, 512, 2, 1e10), (32, 1024, 3, 2e9), (24, 1536, 5, 4e8), (8, 20
 
 
 def c4_member(NX, nl, nu4, member, tau, dt=1200.0):
-    """One ensemble member: C2 physics (L = 4000 km, f0 = 1e-4, beta = 2e-11, drag 1e-7, double-gyre
-    wind) at NX^2 x nl.  nl = 3 uses C2's layers; other layer counts use log-depth layers summing
-    to 4000 m with the seeded exponential stratification of c3.  Seed 20261203 + member."""
+    """One ensemble member: the physics of the 1024^2 x 3 workload (L = 4000 km, f0 = 1e-4,
+    beta = 2e-11, drag 1e-7, double-gyre wind) at NX^2 x nl.  nl = 3 uses that workload's layers;
+    other layer counts use log-depth layers summing to 4000 m with the seeded exponential
+    stratification of the 2048^2 x 10 workload (c3).  Seed 20261203 + member."""
     if nl == 3:
         dh = [350.0, 750.0, 2900.0]
         N2 = _n2_from_gprime(dh, [0.025, 0.0125])
