@@ -60,8 +60,9 @@ typedef struct qgw_ctx qgw_ctx;
  * from the bathtub formula.  The vertical modes must be supplied with
  * qgw_set_modes before the first inversion.  Returns NULL and sets *status
  * on failure (1 <= nl <= QGW_DEV_NL_MAX; bounded (bc_fac >= 0): any grid,
- * with the FFT sine transforms when 2(n+1) and 2(m+1) are in {2^a: 8..8192}
- * or {3*2^a: 24..6144}, else dense sine transforms, matrix products with
+ * with the FFT sine transforms when 2(n+1) and 2(m+1) are in {2^a: 8..8192},
+ * {3*2^a: 24..6144} or {200, 400, 1000, 2000, 4000, 448} (NX = 100, 200,
+ * 500, 1000, 2000, 224), else dense sine transforms, matrix products with
  * S_ij = 2 sin(pi i j/(N+1)) on the matrix cores through rocBLAS (O(N)
  * flops per point instead of O(log N), for any NX, NY the reference's FFTW
  * plans accept); periodic (bc_fac = -1): NX = NY even, in the FFT length

@@ -6,9 +6,10 @@
 // and no bit reversal is needed.  Between stages the data sits in one LDS
 // buffer (read all -> barrier -> write all -> barrier), with one pad slot per
 // 32 entries so that the stride-R writes of the early stages spread over the
-// banks.  Radices are 8 (as many as possible), then 4, 2 and 3, so that the
-// transform lengths 2(n+1) of the model's grids (powers of two and 3*2^k)
-// are covered.  Twiddles come from a table tw[m] = exp(-2 pi i m / L) built
+// banks.  Radices are 8 (as many as possible), then 4, 2, 3, 5 and 7, so that
+// the transform lengths 2(n+1) of the model's grids of the form 2^a 3^b 5^c
+// 7^d are covered (powers of two, 3*2^k, and grids such as NX = 100,
+// 200, 1000, 2000).  Twiddles come from a table tw[m] = exp(-2 pi i m / L) built
 // on the host in FP64 and stored in the arithmetic type of the kernel.
 //
 // One workgroup of NT threads transforms one sequence of length L; the
@@ -43,29 +44,27 @@ __device__ __forceinline__ int lds_slot(int i) { return i + (i >> 5); }
 constexpr int lds_slots(int L) { return L + (L >> 5) + 1; }
 
 // ---- radix plan -----------------------------------------------------------
+constexpr int kPlanRadices[6] = {8, 4, 2, 3, 5, 7};
 constexpr int plan_count(int L) {
   int c = 0;
-  while (L % 8 == 0) { L /= 8; c++; }
-  while (L % 4 == 0) { L /= 4; c++; }
-  while (L % 2 == 0) { L /= 2; c++; }
-  while (L % 3 == 0) { L /= 3; c++; }
+  for (int r : kPlanRadices)
+    while (L % r == 0) { L /= r; c++; }
   return c;
 }
 constexpr int plan_rest(int L) {
-  while (L % 8 == 0) L /= 8;
-  while (L % 4 == 0) L /= 4;
-  while (L % 2 == 0) L /= 2;
-  while (L % 3 == 0) L /= 3;
+  for (int r : kPlanRadices)
+    while (L % r == 0) L /= r;
   return L;
 }
 constexpr int plan_radix(int L, int s) {
-  int r = 0;
   int c = 0;
-  while (L % 8 == 0) { if (c == s) return 8; L /= 8; c++; }
-  while (L % 4 == 0) { if (c == s) return 4; L /= 4; c++; }
-  while (L % 2 == 0) { if (c == s) return 2; L /= 2; c++; }
-  while (L % 3 == 0) { if (c == s) return 3; L /= 3; c++; }
-  return r;
+  for (int r : kPlanRadices)
+    while (L % r == 0) {
+      if (c == s) return r;
+      L /= r;
+      c++;
+    }
+  return 0;
 }
 constexpr int plan_ns(int L, int s) {  // product of the radices before stage s
   int p = 1;
@@ -96,6 +95,55 @@ template <typename T> struct Bfly<T, 3> {
     v[0] = cadd<T>(v[0], s);
     v[1] = cadd<T>(m, r);
     v[2] = csub<T>(m, r);
+  }
+};
+
+// X_k = sum_j x_j W5^(jk), W5 = exp(-2 pi i/5): with t = x1 + x4, x2 + x3 and
+// u = x1 - x4, x2 - x3, X_1,4 = x0 + c1 t1 + c2 t2 -/+ i (s1 u1 + s2 u2) and
+// X_2,3 = x0 + c2 t1 + c1 t2 -/+ i (s2 u1 - s1 u2), c_k = cos(2 pi k/5), s_k = sin(2 pi k/5).
+template <typename T> struct Bfly<T, 5> {
+  static __device__ __forceinline__ void run(cx<T> *v) {
+    const T c1 = T(0.30901699437494742410229341718281906), c2 = T(-0.80901699437494742410229341718281906);
+    const T s1 = T(0.95105651629515357211643933337938214), s2 = T(0.58778525229247312916870595463907277);
+    const cx<T> t1 = cadd<T>(v[1], v[4]), t2 = cadd<T>(v[2], v[3]);
+    const cx<T> u1 = csub<T>(v[1], v[4]), u2 = csub<T>(v[2], v[3]);
+    const cx<T> a1 = mkc<T>(v[0].x + c1 * t1.x + c2 * t2.x, v[0].y + c1 * t1.y + c2 * t2.y);
+    const cx<T> a2 = mkc<T>(v[0].x + c2 * t1.x + c1 * t2.x, v[0].y + c2 * t1.y + c1 * t2.y);
+    const cx<T> b1 = mkc<T>(s1 * u1.x + s2 * u2.x, s1 * u1.y + s2 * u2.y);
+    const cx<T> b2 = mkc<T>(s2 * u1.x - s1 * u2.x, s2 * u1.y - s1 * u2.y);
+    v[0] = cadd<T>(v[0], cadd<T>(t1, t2));
+    v[1] = cadd<T>(a1, cnegi<T>(b1));
+    v[4] = csub<T>(a1, cnegi<T>(b1));
+    v[2] = cadd<T>(a2, cnegi<T>(b2));
+    v[3] = csub<T>(a2, cnegi<T>(b2));
+  }
+};
+
+// The same for W7 = exp(-2 pi i/7): t_j = x_j + x_(7-j), u_j = x_j - x_(7-j)
+// (j = 1..3), X_k and X_(7-k) = x0 + sum_j cos(2 pi jk/7) t_j -/+ i sum_j
+// sin(2 pi jk/7) u_j, the products jk reduced mod 7.
+template <typename T> struct Bfly<T, 7> {
+  static __device__ __forceinline__ void run(cx<T> *v) {
+    const T c1 = T(0.62348980185873353052500488400423981), c2 = T(-0.22252093395631440428890256449679476),
+            c3 = T(-0.90096886790241912623610231950744505);
+    const T s1 = T(0.78183148246802980870844452667405775), s2 = T(0.97492791218182360701813168299393122),
+            s3 = T(0.43388373911755812047576833284835875);
+    const cx<T> t1 = cadd<T>(v[1], v[6]), t2 = cadd<T>(v[2], v[5]), t3 = cadd<T>(v[3], v[4]);
+    const cx<T> u1 = csub<T>(v[1], v[6]), u2 = csub<T>(v[2], v[5]), u3 = csub<T>(v[3], v[4]);
+    auto lin3 = [](const cx<T> &x0, T a, const cx<T> &p, T b, const cx<T> &q, T c, const cx<T> &r) {
+      return mkc<T>(x0.x + a * p.x + b * q.x + c * r.x, x0.y + a * p.y + b * q.y + c * r.y);
+    };
+    const cx<T> z = mkc<T>(T(0), T(0));
+    const cx<T> a1 = lin3(v[0], c1, t1, c2, t2, c3, t3), b1 = lin3(z, s1, u1, s2, u2, s3, u3);
+    const cx<T> a2 = lin3(v[0], c2, t1, c3, t2, c1, t3), b2 = lin3(z, s2, u1, -s3, u2, -s1, u3);
+    const cx<T> a3 = lin3(v[0], c3, t1, c1, t2, c2, t3), b3 = lin3(z, s3, u1, -s1, u2, s2, u3);
+    v[0] = cadd<T>(v[0], cadd<T>(t1, cadd<T>(t2, t3)));
+    v[1] = cadd<T>(a1, cnegi<T>(b1));
+    v[6] = csub<T>(a1, cnegi<T>(b1));
+    v[2] = cadd<T>(a2, cnegi<T>(b2));
+    v[5] = csub<T>(a2, cnegi<T>(b2));
+    v[3] = cadd<T>(a3, cnegi<T>(b3));
+    v[4] = csub<T>(a3, cnegi<T>(b3));
   }
 };
 
@@ -176,7 +224,7 @@ template <typename T, int L, int NT, int S> struct Stages {
 // threads of the workgroup.
 template <typename T, int L, int NT>
 __device__ __forceinline__ void fft_lds(cx<T> *buf, const cx<T> *__restrict__ tw, int t) {
-  static_assert(plan_rest(L) == 1, "transform length must be 2^a 3^b");
+  static_assert(plan_rest(L) == 1, "transform length must be 2^a 3^b 5^c 7^d");
   Stages<T, L, NT, 0>::run(buf, tw, t);
 }
 

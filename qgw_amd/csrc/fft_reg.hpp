@@ -10,9 +10,10 @@
 // their sequence straight from global memory and store the result straight
 // back, with no extra LDS pass at either end.
 //
-// Lengths 2^a use E = 16 (L >= 1024) or E = 8 (radices 16/8/4/2); lengths
-// 3*2^a use E = 8 with the LDS Stockham of fft.hpp between a write and a read
-// of the home positions.
+// Lengths 2^a use E = 16 (L >= 1024) or E = 8 (radices 16/8/4/2); other
+// lengths (factors 2, 3, 5, 7) use E = 8 (E = 4 where 8 does not divide L) with
+// the LDS Stockham of fft.hpp between a write and a read of the home
+// positions.
 #pragma once
 #include "fft.hpp"
 
@@ -23,7 +24,8 @@ constexpr bool is_pow2(int L) { return L > 0 && (L & (L - 1)) == 0; }
 // EP: elements per thread (0: 16 for powers of two from 1024, else 8).  A
 // radix schedule of the transform: E = 16 runs radix-16 stages, E = 8 radix-8.
 template <int L, int EP = 0> struct RegPlan {
-  static constexpr int E = (EP > 0 && is_pow2(L)) ? EP : (is_pow2(L) ? ((L >= 1024) ? 16 : 8) : 8);
+  static constexpr int E = (EP > 0 && is_pow2(L)) ? EP : (is_pow2(L) ? ((L >= 1024) ? 16 : 8) : (L % 8 == 0 ? 8 : 4));
+  static_assert(L % E == 0, "home positions");
   static constexpr int NT = L / E;
   // radix of stage s (powers of two): E while the remaining length allows, then the rest
   static constexpr int radix(int s) {

@@ -262,7 +262,7 @@ k_p2m(Geo g, T *__restrict__ w, const double *__restrict__ La2, const double *__
   static_assert(GG >= 1 && GG <= p2m_groups<T, M, EP>(), "pairs per block");
   constexpr int BT = G * NT, W = 2 * G, SL = p2m_slots(DstHalf<M, EP>::slots);
   constexpr int NWV = (NT + 63) / 64;
-  static_assert(BT % W == 0, "tile rows per pass");
+  static_assert(BT % G == 0, "tile rows per pass");
   __shared__ cx<T> bufs[G][SL];
   __shared__ TwTab<T, 2 * M> tb2;  // sines of the pre passes
   __shared__ cx<double> wtot[G][NWV];

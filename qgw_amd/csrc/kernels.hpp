@@ -591,7 +591,7 @@ k_p3h(Geo g, const T *__restrict__ what, T *__restrict__ psi, T *__restrict__ q,
 // it = blockIdx.x, blockIdx.x + gridDim.x, ... (a row and layer pair).  The
 // rows of item it + gridDim.x are loaded into registers while item it is
 // transformed; the twiddles (two-level tables for powers of two, the full
-// table for 3 2^k) and the pre-pass sines come from LDS tables filled once per
+// table for other lengths) and the pre-pass sines come from LDS tables filled once per
 // block, so the global traffic inside the loop is the row loads and stores
 // alone and no wait on a table load also waits on the next item's rows.  Each
 // item's row pair is staged in LDS (entry x holds z_x, entry 0 the zero z_0),
@@ -641,7 +641,7 @@ template <typename T, int M, int EP> struct PipeRows {
 };
 
 // Transform of the pipelined row passes: the register plan with the two-level
-// LDS table for powers of two; for 3 2^k the LDS Stockham of fft.hpp on buf,
+// LDS table for powers of two; for other lengths the LDS Stockham of fft.hpp on buf,
 // with its full twiddle table in LDS (twf), so that no global twiddle load
 // waits behind the prefetched rows.
 template <typename T, int M, int EP>
@@ -659,7 +659,7 @@ __device__ __forceinline__ void fft_pipe(cx<T> *h, cx<T> *buf, const TwLds<T, M>
     for (int m = 0; m < E; m++) h[m] = buf[lds_slot(t + m * NT)];
   }
 }
-// the full length-M table of the LDS Stockham (3 2^k lengths; one entry otherwise)
+// the full length-M table of the LDS Stockham (lengths other than 2^k; one entry otherwise)
 template <typename T, int M> constexpr int twf_len() { return is_pow2(M) ? 1 : M; }
 template <typename T, int M>
 __device__ __forceinline__ void twf_fill(cx<T> *twf, const cx<T> *__restrict__ tw, int t, int nt) {

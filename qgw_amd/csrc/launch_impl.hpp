@@ -11,13 +11,19 @@ namespace qgw {
 
 using T = QGW_T;
 
-// Supported transform lengths L = 2(n+1): 2^a (8..8192) and 3*2^a (24..6144).
-#define QGW_LENGTHS(X) \
+// Supported transform lengths L = 2(n+1) (bounded) or n (periodic): 2^a
+// (8..8192), 3*2^a (24..6144), the 5^b 2^a lengths of the grids NX = 100,
+// 200, 500, 1000 and 2000 (L = 200, 400, 1000, 2000, 4000; periodic NX = 100
+// is L = 100) and 7*2^6 (NX = 224).
+#define QGW_LENGTHS(X)                                                           \
   X(8) X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(2048) X(4096) X(8192) \
-  X(24) X(48) X(96) X(192) X(384) X(768) X(1536) X(3072) X(6144)
+  X(24) X(48) X(96) X(192) X(384) X(768) X(1536) X(3072) X(6144)              \
+  X(100) X(200) X(400) X(1000) X(2000) X(4000) X(448)
 
-// Lengths M of the half-length transforms (M = L/2 >= 64).
-#define QGW_HLENGTHS(X) X(64) X(128) X(256) X(512) X(1024) X(2048) X(4096) X(96) X(192) X(384) X(768) X(1536) X(3072)
+// Lengths M of the half-length transforms (M = L/2 >= 64; M even).
+#define QGW_HLENGTHS(X)                                                                                  \
+  X(64) X(128) X(256) X(512) X(1024) X(2048) X(4096) X(96) X(192) X(384) X(768) X(1536) X(3072) \
+  X(100) X(200) X(500) X(1000) X(2000) X(224)
 // Radix schedule of a half-length transform: powers of two from 1024 run with
 // 16 (radix-16 stages) or 8 (radix-8) elements per thread: a.fft_e for the
 // row passes, a.fft_e_col for the column pass.

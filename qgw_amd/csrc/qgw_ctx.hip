@@ -36,6 +36,8 @@ bool supported_length(int L) {
   int r = L;
   while (r % 2 == 0) r /= 2;
   while (r % 3 == 0) r /= 3;
+  while (r % 5 == 0) r /= 5;
+  while (r % 7 == 0) r /= 7;
   if (r != 1) return false;
   return qgw::launch_has_length(L);
 }
@@ -1152,7 +1154,7 @@ qgw_ctx *create_group(const qgw_params *ps, int nb, int precision, hipStream_t e
   c->fft_e = 16;
   c->fft_e_col = (precision == QGW_FP64) ? 8 : 16;
   c->per = per ? 1 : 0;
-  // the pipelined row passes by default, except FP64 rows of a 3 2^k length:
+  // the pipelined row passes by default, except FP64 rows of a length other than 2^k:
   // there the full LDS twiddle table of the Stockham leaves one block per CU,
   // and the one-block form was measured faster (1536^2 x 10: 0.849 against
   // 0.870 ms per step; FP32 the other way, 0.471 against 0.482)

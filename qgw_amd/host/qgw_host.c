@@ -502,6 +502,7 @@ static int hplan_make(hplan *pl, int L) {
   while (r % 2 == 0) { pl->radix[pl->nr++] = 2; r /= 2; }
   while (r % 3 == 0) { pl->radix[pl->nr++] = 3; r /= 3; }
   while (r % 5 == 0) { pl->radix[pl->nr++] = 5; r /= 5; }
+  while (r % 7 == 0) { pl->radix[pl->nr++] = 7; r /= 7; }
   if (r != 1) return QGW_ERR_ARG;
   pl->tw = (cplx *)malloc(sizeof(cplx) * L);
   if (!pl->tw) return QGW_ERR_NOMEM;
@@ -523,7 +524,7 @@ static cplx *hfft(const hplan *pl, cplx *x, cplx *y) {
     int twstep = L / (Ns * R);
     for (int j = 0; j < stride; j++) {
       int k = j % Ns;
-      cplx v[5], o[5];
+      cplx v[7], o[7];
       for (int r = 0; r < R; r++) v[r] = src[j + r * stride] * pl->tw[(r * k * twstep) % L];
       if (R == 4) {
         cplx a0 = v[0] + v[2], a1 = v[0] - v[2], a2 = v[1] + v[3], a3 = (v[1] - v[3]) * (-I);

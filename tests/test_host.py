@@ -175,6 +175,13 @@ def test_host_dst2d_golden():
     rng = np.random.default_rng(11)
     a = rng.standard_normal((95, 47))
     assert rel(Q.dst2d(a), O.dst2d(a)) < 1e-14
+    # radix-5 and radix-7 stages of the host FFT: lengths 2(n+1) = 448 (2^6 7),
+    # 280 (2^3 5 7) and 200 (2^3 5^2), against the direct sum
+    for m, n in ((223, 139), (99, 223)):
+        a = rng.standard_normal((m, n))
+        Sx = 2.0 * np.sin(np.pi * np.outer(np.arange(1, n + 1), np.arange(1, n + 1)) / (n + 1))
+        Sy = 2.0 * np.sin(np.pi * np.outer(np.arange(1, m + 1), np.arange(1, m + 1)) / (m + 1))
+        assert rel(Q.dst2d(a), Sy @ a @ Sx) < 1e-13, (m, n)
 
 
 def test_initial_condition_properties():
