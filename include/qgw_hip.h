@@ -207,7 +207,9 @@ int qgw_profile_read(qgw_ctx *ctx, double *ms, int *count, int npass);
 
 /* Tuning knobs.  QGW_TUNE_P4_TILE selects the tile of the tendency/AB3
  * kernel: 0: 64x8, 1: 64x16, 2: 64x32, 3: 128x16 (columns x rows) LDS
- * tiles, 4: register-streaming wavefront strips (default). */
+ * tiles, 4: register-streaming wavefront strips (default), 5: the same
+ * strips in blocks of 4 over 224 output columns whose boundaries fall on
+ * 128-byte lines (faster at 4096^2). */
 #define QGW_TUNE_P4_TILE 1
 /* QGW_TUNE_T_KERNEL selects the layer<->mode projection: 0 automatic (matrix
  * cores for FP32 at 16 or more layers, else VALU), 1 VALU, 2 matrix cores on

@@ -36,7 +36,7 @@ TUNE_FFT_E = 5
 TUNE_FFT_E_COL = 6
 TUNE_FUSE = 7
 TUNE_P2M_PAIRS = 8
-P4_TILES = ["64x8", "64x16", "64x32", "128x16", "stream"]
+P4_TILES = ["64x8", "64x16", "64x32", "128x16", "stream", "stream-aligned"]
 TUNE_KNOBS = (TUNE_P4_TILE, TUNE_P4_WAVES, TUNE_T_KERNEL, TUNE_DST, TUNE_FFT_E, TUNE_FFT_E_COL, TUNE_FUSE,
               TUNE_P2M_PAIRS)
 _TUNE_CACHE = {}

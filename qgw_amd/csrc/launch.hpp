@@ -15,9 +15,11 @@ namespace qgw {
 constexpr int kMaxLayers = 1000;
 
 // P4 tile shapes (columns x rows); the workgroup has 4*columns threads.  Variant
-// kP4TileCount is the streaming form (k_p4s).
+// kP4TileCount is the streaming form (k_p4s), kP4Aligned the streaming form
+// with aligned row stores (k_p4a).
 constexpr int kP4TileCount = 4;
-constexpr int kP4Variants = kP4TileCount + 1;
+constexpr int kP4Aligned = kP4TileCount + 1;
+constexpr int kP4Variants = kP4TileCount + 2;
 constexpr int kP4TileW[kP4TileCount] = {64, 64, 64, 128};
 constexpr int kP4TileH[kP4TileCount] = {8, 16, 32, 16};
 
@@ -40,7 +42,7 @@ struct LaunchArgs {
   int cus = 256;    // compute units of the device (persistent grids)
   StepCoef cf{};
   int nlap = 0;
-  int p4_tile = kP4TileCount;  // P4 variant: a tile index, or kP4TileCount for streaming
+  int p4_tile = kP4TileCount;  // P4 variant: a tile index, kP4TileCount (streaming) or kP4Aligned
   int p4_waves = 40;           // streaming P4: target wavefronts per SIMD (sets the row chunk)
   int t_kernel = 0;            // projection: 0 auto (matrix cores for FP32 at >= 16 layers), 1 VALU, 2 matrix cores
   int p2m_g = 0;               // four-pass column pass: column pairs per block (0: the most that fit)

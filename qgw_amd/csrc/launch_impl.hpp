@@ -388,13 +388,33 @@ template <int NLAP, int KAHAN> static void p4s_go(const LaunchArgs &a) {
                      (const T *)a.wind, a.cf, nstrip, chunk, nchunk);
 }
 
+// the streaming form over aligned column blocks: blocks of 4 wavefronts over
+// p4a_cols output columns, the row chunk set as for k_p4s
+template <int NLAP, int KAHAN> static void p4a_go(const LaunchArgs &a) {
+  constexpr int BW = p4a_cols<NLAP>();
+  const int nbx = (a.g.n + BW - 1) / BW;
+  const long lanes = (long)nbx * 4 * a.g.nl;
+  const long target = 1024L * (a.p4_waves > 0 ? a.p4_waves : 40);
+  int chunk = (int)(((long)a.g.m * lanes + target - 1) / target);
+  chunk = chunk < 16 ? 16 : (chunk > 512 ? 512 : chunk);
+  const int nchunk = (a.g.m + chunk - 1) / chunk;
+  const dim3 grid((unsigned)((long)nbx * nchunk * a.g.nl));
+  hipLaunchKernelGGL((k_p4a<T, NLAP, KAHAN>), grid, dim3(256), 0, a.stream, a.g, (const T *)a.psi,
+                     (const T *)a.q, (T *)a.qn, (T *)a.fnew, (const T *)a.f1, (const T *)a.f2, (comp_t<T> *)a.comp,
+                     (const T *)a.topo, (const T *)a.wind, a.cf, nbx, chunk, nchunk);
+}
+
 template <int NLAP, int KAHAN> static int p4_tile(const LaunchArgs &a) {
   static_assert(kP4TileCount == 4, "tile table");
   if (a.per) {  // periodic: the 64x16 tile with wrapped halo loads
     p4_go<NLAP, KAHAN, kP4TileW[1], kP4TileH[1], true>(a);
     return 0;
   }
-  if constexpr (NLAP > 5) {  // deep hyperviscosity (n_hyper > 8): the streaming form only
+  if (a.p4_tile == kP4Aligned) {
+    p4a_go<NLAP, KAHAN>(a);
+    return 0;
+  }
+  if constexpr (NLAP > 5) {  // deep hyperviscosity (n_hyper > 8): the streaming forms only
     p4s_go<NLAP, KAHAN>(a);
     return 0;
   } else {

@@ -1276,7 +1276,7 @@ int autotune_group(qgw_ctx *c, int reps, int *p4_tile, double *best_ms) {
   static const int kWaves[] = {4, 8, 16, 32, 40, 48};
   constexpr int kNW = sizeof(kWaves) / sizeof(kWaves[0]);
   for (int v = 0; v < (c->per ? 0 : qgw::kP4Variants) && !st; v++) {  // periodic: one P4 kernel
-    const bool stream = (v == qgw::kP4TileCount);
+    const bool stream = (v >= qgw::kP4TileCount);  // the streaming forms: also the row chunk
     for (int wi = 0; wi < (stream ? kNW : 1) && !st; wi++) {
       c->p4_tile = v;
       if (stream) c->p4_waves = kWaves[wi];
