@@ -48,25 +48,30 @@ def _announce(*a):
 @pytest.mark.parametrize("name", ["c2", "c3"])
 def test_fp64_inversion_and_steps_against_oracle(name):
     """1024^2 x 3 and 2048^2 x 10 (FP64): inversion round trip (1e-12) and two
-    steps against the oracle (1e-12)."""
+    steps against the oracle (1e-12), with the streaming P4 and with the P4
+    over aligned column blocks."""
     p = getattr(configs, name)()
     _announce("full size fp64 vs oracle", name, p.NX, p.nl)
     modes = Q.eigmode(p)
     psi0, q0 = Q.make_ic(p, modes)
-    m = Q.Model(p, Q.FP64, modes)
-    m.set_q(q0)
-    m.invert_pv()
-    assert rel(m.psi, psi0) < 1e-12
-    m.run(2)
-    q_dev = m.q
-    m.close()
+    q_dev = {}
+    for tile in (4, 5):
+        m = Q.Model(p, Q.FP64, modes)
+        m.tune_set(Q.TUNE_P4_TILE, tile)
+        m.set_q(q0)
+        m.invert_pv()
+        assert rel(m.psi, psi0) < 1e-12
+        m.run(2)
+        q_dev[tile] = m.q
+        m.close()
     o = O.Oracle(p)
     o.set_modes(*modes)
     o.set_q(q0)
     o.run(2)
     q_orc = o.field(O.Q).copy()
     o.close()
-    assert rel(q_dev, q_orc) < 1e-12
+    for tile in (4, 5):
+        assert rel(q_dev[tile], q_orc) < 1e-12
 
 
 @pytest.mark.timeout(300)
@@ -168,6 +173,13 @@ def test_4096x32_steps_against_oracle():
     m.invert_pv()
     q64 = m.q
     E64, Z64 = m.diagnostics()[2:]
+    m.close()
+    # the P4 over aligned column blocks (the faster form at this width)
+    m = Q.Model(p, Q.FP64, modes)
+    m.tune_set(Q.TUNE_P4_TILE, 5)
+    m.set_q(q0)
+    m.run(2)
+    assert rel(m.q, q64) < 1e-12
     m.close()
     f = Q.Batch([p], Q.FP32, [modes])
     f.tune_set(0, Q.TUNE_T_KERNEL, 2)
