@@ -48,8 +48,11 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
 PASS_WORDS = {"P1": 2, "T1": 2, "P2": 2, "T2": 2, "P3": 2, "P4": 6}
 # Kernel name prefixes of each pass, in order of preference (the profile holds
 # whichever form ran).
-PASS_KERNELS = {"P1": ("k_p1hp<", "k_p1h<", "k_p1<"), "T1": ("k_tproj_mfma<", "k_tproj<"), "P2": ("k_p2h<", "k_p2<"),
-                "T2": ("k_tproj_mfma<", "k_tproj<"), "P3": ("k_p3hp<", "k_p3h<", "k_p3<"), "P4": ("k_p4s<", "k_p4<")}
+PASS_KERNELS = {"P1": ("k_p1hp<", "k_p1m<", "k_p1h<", "k_p1<"), "T1": ("k_tproj_mfma<", "k_tproj<"),
+                "P2": ("k_p2h<", "k_p2m<", "k_p2<"), "T2": ("k_tproj_mfma<", "k_tproj<"),
+                "P3": ("k_p3hp<", "k_p3m<", "k_p3h<", "k_p3<"), "P4": ("k_p4s<", "k_p4a<", "k_p4<")}
+# the P4 kernel of each QGW_TUNE_P4_TILE choice (Model.tuning()["p4"])
+P4_KERNEL = {"stream": "k_p4s<", "stream-aligned": "k_p4a<"}
 
 
 def log(*a):
@@ -189,9 +192,11 @@ def c5_leg(Q, configs, steps, acc_steps):
     storage = g.nxp2 * g.nyp2 * p.nl
     out = {"workload": "4096^2 x 32-layer QG step (L=8000 km, dt=600 s, nu4=2e8, double-gyre wind)",
            "initial_condition_s": t_ic}
-    # FP32 batched, matrix-core projections (the default at >= 16 layers in FP32)
+    # FP32 batched, matrix-core projections (the default at >= 16 layers in FP32);
+    # both forms autotuned as at 2048^2 x 10 (choices kept in the tuning cache)
     b = Q.Batch([p], Q.FP32, [modes])
     b.set_q(0, q0)
+    b.autotune(reps=2)
     b.step(6)
     b.synchronize()
     ms32 = b.time_step(steps) / steps
@@ -203,6 +208,8 @@ def c5_leg(Q, configs, steps, acc_steps):
     b.close()
     m = Q.Model(p, Q.FP64, modes)
     m.set_q(q0)
+    m.autotune(reps=2)
+    tuning64 = m.tuning()
     m.run(2)
     m.synchronize()
     ms64 = m.time_run(steps) / steps
@@ -239,7 +246,7 @@ def c5_leg(Q, configs, steps, acc_steps):
     out.update({
         "fp32": {"ms_per_step": ms32, "value": work / (ms32 * 1e-3), "passes": t32, "tuning": tuning32,
                  "form": "fp32+kahan, batched (hipGraph replay), projections on the matrix cores"},
-        "fp64": {"ms_per_step": ms64, "value": work / (ms64 * 1e-3), "passes": t64},
+        "fp64": {"ms_per_step": ms64, "value": work / (ms64 * 1e-3), "passes": t64, "tuning": tuning64},
         "fp32_over_fp64": ms64 / ms32,
         "accuracy_vs_fp64": {"steps": acc_steps, "energy_rel_err": abs(E32 - E64) / abs(E64),
                              "enstrophy_rel_err": abs(Z32 - Z64) / abs(Z64),
@@ -391,14 +398,19 @@ def hbm_copy_rate(nbytes, reps=20):
     return 2.0 * nbytes * reps / (ms.value * 1e-3) / 1e9
 
 
-def kernel_traffic(dom, head):
-    """HBM bytes per launch of the pass's kernel from the committed counter profile."""
+def kernel_traffic(dom, head, tuning=None):
+    """HBM bytes per launch of the pass's kernel from the committed counter
+    profile: the kernel the tuning selected for the pass (P4), else the first
+    of the pass's kernels the profile holds."""
     tpath = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(tpath):
         return None, None
     kernels = json.load(open(tpath))["kernels"]
     ty = "double" if head == "fp64" else "float"
-    for prefix in PASS_KERNELS[dom]:
+    prefixes = PASS_KERNELS[dom]
+    if dom == "P4" and tuning:
+        prefixes = (P4_KERNEL.get(tuning.get("p4"), "k_p4<"),)
+    for prefix in prefixes:
         for kname, tv in kernels.items():
             if kname.startswith(prefix + ty):
                 return tv["fetch_bytes"] + tv["write_bytes"], kname
@@ -483,7 +495,7 @@ def main():
                   for k in per_launch}
     total_alg = sum(words.values()) * s * work
     step_gbs = total_alg / (ms_step * 1e-3) / 1e9
-    traffic, tkernel = kernel_traffic(dom, head)
+    traffic, tkernel = kernel_traffic(dom, head, tunes.get(head))
     out = {
         "metric": "grid-pt*layer*steps/s at 2048^2x10, FP32 batched vs FP64 plain, <=1e-4 energy err",
         "value": value,
