@@ -327,7 +327,7 @@ def test_p4_stream_paths(prec, hyper):
         assert rel(out[tile][0], out[1][0]) < tol
 
 
-@pytest.mark.parametrize("prec", [Q.FP64, Q.FP32])
+@pytest.mark.parametrize("prec", [Q.FP64, Q.FP32, Q.FP32_PLAIN])
 @pytest.mark.parametrize("case", [(600, 320, 4, 1e8), (500, 200, 12, None), (1024, 96, 2, -1e4)])
 def test_p4_aligned_blocks(prec, case):
     """The streaming P4 over aligned column blocks (4 wavefronts over 224
