@@ -1438,9 +1438,6 @@ template <int W> __device__ constexpr int wslot(int ph, int i) { return ((ph - i
 // is a fixed register: no moves shift the windows.  EDGE = false: the strip
 // and its rows touch no wall, so every load is in bounds and no wall rule or
 // ring carry applies.
-// LDS ring of one wavefront's FP32 interior walk: 4 rows of 7 streams of 64 dwords.
-constexpr int kP4RingWave = 4 * 7 * 64;
-
 // OWN: the walk stores only the output columns X0 .. X0 + ownw - 1 (the
 // block's own columns in k_p4a); the other outputs of its strip belong to a
 // neighbouring block, which computes them as well.
@@ -1450,8 +1447,7 @@ __device__ __forceinline__ void p4s_walk(const Geo &g, const T *__restrict__ P, 
                                          const T *__restrict__ f2, comp_t<T> *__restrict__ comp,
                                          const T *__restrict__ topo, const T *__restrict__ wind, const StepCoef &cf,
                                          int x, int lane, int km,
-                                         int y0, int y1, bool ringstrip, int X0 = 0, int ownw = 0,
-                                         float *lring = nullptr) {
+                                         int y0, int y1, bool ringstrip, int X0 = 0, int ownw = 0) {
   constexpr int H = (NLAP > 1) ? NLAP : 1;       // halo, and the delay of the output row
   constexpr int NP = H + 2;                      // psi rows r, r-1, ..., r-H-1
   constexpr int D = 3;                           // prefetch depth (iterations)
@@ -1460,20 +1456,6 @@ __device__ __forceinline__ void p4s_walk(const Geo &g, const T *__restrict__ P, 
   static_assert(D < RING && W % 3 == 0, "unroll");
   constexpr int NL = (NLAP > 1) ? NLAP - 1 : 1;  // stored levels 1..NLAP-1
   static_assert(NP <= W, "psi window");
-  // FP32 interior walks load their rows into an LDS ring (buffer loads with an
-  // LDS destination), RL slots of NSTR streams of 64 dwords per wavefront
-  // (psi, q, topography, f1, f2, wind and the compensation): the compiler does
-  // not track those loads, so the walk waits for them itself, at each row for
-  // the row it uses, with the D rows issued after it still in flight.  Every
-  // iteration issues the same NSTR loads (a stream or row the walk does not use
-  // gets an out-of-range offset, which the hardware's range check turns into
-  // no access), so the count of loads issued after a row is fixed.  With
-  // register destinations the compiler waits for all of a wavefront's loads at
-  // every row (the loads per iteration differ by branch), which leaves one row
-  // in flight.
-  constexpr bool DMA = !EDGE && sizeof(T) == 4;
-  constexpr int NSTR = 7, RL = D + 1;
-  static_assert((RL & (RL - 1)) == 0 && RL * NSTR * 64 == kP4RingWave, "LDS ring slots");
   const int n = g.n, m = g.m;
   const bool xin = !EDGE || (x >= 0 && x <= n + 1);
   const bool xring = EDGE && (x == 0 || x == n + 1);
@@ -1569,52 +1551,6 @@ __device__ __forceinline__ void p4s_walk(const Geo &g, const T *__restrict__ P, 
       }
     }
   };
-  // the LDS-ring form of fetch: stream k of row r into slot (r - rbeg) mod RL;
-  // rows past the walk and streams the row does not use read nothing
-  auto fetch_dma = [&](int r) {
-    if constexpr (DMA) {
-      typedef __attribute__((address_space(3))) void lds_void;
-      constexpr unsigned OOR = 0x80000000u;  // past every view's record size
-      const int sl = __builtin_amdgcn_readfirstlane((r - rbeg) & (RL - 1));
-      float *sb = lring + sl * NSTR * 64;
-      const bool live = r < rend;
-      const bool qrow = live && r + 1 - H >= y0 - 1;
-      const bool hist = live && r - H >= y0;
-      auto ld = [&](const BufView &v, int k, auto SZ, unsigned vo, bool on, unsigned so) {
-        __builtin_amdgcn_raw_ptr_buffer_load_lds(v.r, (lds_void *)(sb + k * 64), decltype(SZ)::value, on ? vo : OOR,
-                                                 on ? so : 0u, 0, 0);
-      };
-      ld(vP, 0, IC<4>{}, xb, live, (unsigned)r * rowb);
-      ld(vQ, 1, IC<4>{}, xb, qrow, (unsigned)(r + 1 - H) * rowb);
-      ld(vT, 2, IC<4>{}, xb, qrow && bottom, (unsigned)(r + 1 - H) * rowb);
-      ld(v1, 3, IC<4>{}, xb, hist, (unsigned)(r - H) * rowb);
-      ld(v2, 4, IC<4>{}, xb, hist, (unsigned)(r - H) * rowb);
-      ld(vW, 5, IC<4>{}, 0u, live && km == 0, (unsigned)(r - H) * (unsigned)sizeof(T));
-      ld(vC, 6, IC<2>{}, xbc, hist && KAHAN, (unsigned)(r - H) * rowbc);
-    }
-  };
-  // wait until the loads of row r have reached the ring (the NSTR D loads of
-  // the D rows after it may still be in flight; stores issued in between only
-  // make the wait longer), then take the row from the ring into the slots of
-  // the register windows
-  auto take_dma = [&](auto Sc, int r) {
-    if constexpr (DMA) {
-      constexpr int s = decltype(Sc)::value;
-      constexpr int N = NSTR * D;
-      static_assert(N < 64, "vmcnt");
-      __builtin_amdgcn_s_waitcnt((N & 0xF) | ((N >> 4) << 14) | (0x7 << 4) | (0xF << 8));
-      asm volatile("" ::: "memory");
-      const int sl = __builtin_amdgcn_readfirstlane((r - rbeg) & (RL - 1));
-      const float *sb = lring + sl * NSTR * 64 + lane;
-      bp[s] = sb[0];
-      bq[s] = sb[64];
-      bt[s] = sb[128];
-      b1[s] = sb[192];
-      b2[s] = sb[256];
-      bw[s] = sb[320];
-      if constexpr (KAHAN) bc[s] = (comp_t<T>)__builtin_bit_cast(unsigned, sb[384]);
-    }
-  };
   const BufView vF = buf_view(fnew, lbytes), vN = buf_view(qn, lbytes);
   T nfa = T(0);  // non-finite detection (nf_acc)
   auto body = [&](auto PHc, int r) {
@@ -1622,12 +1558,7 @@ __device__ __forceinline__ void p4s_walk(const Geo &g, const T *__restrict__ P, 
     constexpr int cur = ph % RING;
     constexpr int j0 = ph % 3, j1 = (ph + 2) % 3, j2 = (ph + 1) % 3;  // rows y+1, y, y-1
     const int y = r - H;  // output row of this iteration
-    if constexpr (DMA) {
-      fetch_dma(r + D);
-      take_dma(IC<cur>{}, r);
-    } else {
-      if (r + D < rend) fetch(IC<(ph + D) % RING>{}, r + D);
-    }
+    if (r + D < rend) fetch(IC<(ph + D) % RING>{}, r + D);
     const T p0 = bp[cur];
     pw[wslot<W>(ph, 0)] = p0;
     if constexpr (EDGE) {
@@ -1735,12 +1666,9 @@ __device__ __forceinline__ void p4s_walk(const Geo &g, const T *__restrict__ P, 
       if (x == n) qn[y * g.P + n + 1] = jqr[j1];
     }
   };
-#define QGW_P4S_PRE(k)                                     \
-  if constexpr ((k) < D) {                                 \
-    if constexpr (DMA)                                     \
-      fetch_dma(rbeg + (k));                               \
-    else if (rbeg + (k) < rend)                            \
-      fetch(IC<(k)>{}, rbeg + (k));                        \
+#define QGW_P4S_PRE(k)                          \
+  if constexpr ((k) < D) {                      \
+    if (rbeg + (k) < rend) fetch(IC<(k)>{}, rbeg + (k)); \
   }
   QGW_P4S_PRE(0)
   QGW_P4S_PRE(1)
@@ -1778,7 +1706,6 @@ k_p4s(Geo g, const T *__restrict__ psi, const T *__restrict__ q, T *__restrict__
       const T *__restrict__ wind, StepCoef cf, int nstrip, int chunk, int nchunk) {
   constexpr int H = (NLAP > 1) ? NLAP : 1;  // halo
   constexpr int OW = 64 - 2 * H;            // output columns per strip
-  __shared__ float p4ring[(sizeof(T) == 4) ? 4 * kP4RingWave : 1];  // the FP32 interior walks' load rings
   const int n = g.n, m = g.m;
   const int lane = threadIdx.x & 63;
   // Blocks are dealt to the 8 XCDs round-robin; the remap gives each XCD one
@@ -1809,8 +1736,7 @@ k_p4s(Geo g, const T *__restrict__ psi, const T *__restrict__ q, T *__restrict__
   // walk loads lies inside the layer
   const bool interior = xlo >= 1 && xlo + 63 <= n && y0 - H - NLAP >= 1 && y1 + H - 1 <= m;
   if (interior)
-    p4s_walk<T, NLAP, KAHAN, false>(g, P, Qf, qn, fnew, f1, f2, comp, topo, wind, cf, x, lane, km, y0, y1, false, 0,
-                                    0, p4ring + (threadIdx.x >> 6) * kP4RingWave);
+    p4s_walk<T, NLAP, KAHAN, false>(g, P, Qf, qn, fnew, f1, f2, comp, topo, wind, cf, x, lane, km, y0, y1, false);
   else
     p4s_walk<T, NLAP, KAHAN, true>(g, P, Qf, qn, fnew, f1, f2, comp, topo, wind, cf, x, lane, km, y0, y1,
                                    ringstrip);
@@ -1840,7 +1766,6 @@ k_p4a(Geo g, const T *__restrict__ psi, const T *__restrict__ q, T *__restrict__
   constexpr int H = (NLAP > 1) ? NLAP : 1;  // halo
   constexpr int BW = p4a_cols<NLAP>();      // output columns per block
   constexpr int OWU = BW / 4;               // output columns per wavefront
-  __shared__ float p4ring[(sizeof(T) == 4) ? 4 * kP4RingWave : 1];  // the FP32 interior walks' load rings
   const int n = g.n, m = g.m;
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -1869,7 +1794,7 @@ k_p4a(Geo g, const T *__restrict__ psi, const T *__restrict__ q, T *__restrict__
   const bool interior = xlo >= 1 && xlo + 63 <= n && y0 - H - NLAP >= 1 && y1 + H - 1 <= m;
   if (interior)
     p4s_walk<T, NLAP, KAHAN, false, true>(g, P, Qf, qn, fnew, f1, f2, comp, topo, wind, cf, x, lane, km, y0, y1,
-                                          false, X0, OWU, p4ring + w * kP4RingWave);
+                                          false, X0, OWU);
   else
     p4s_walk<T, NLAP, KAHAN, true, true>(g, P, Qf, qn, fnew, f1, f2, comp, topo, wind, cf, x, lane, km, y0, y1,
                                          ringstrip, X0, OWU);
