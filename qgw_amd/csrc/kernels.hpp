@@ -1752,8 +1752,11 @@ k_p4s(Geo g, const T *__restrict__ psi, const T *__restrict__ q, T *__restrict__
 // block, about together, on one CU; in k_p4s (blocks of 4 OW columns) the
 // lines across a block boundary are completed by a wavefront of another block
 // at another time.  Measured on the access pattern alone (five loaded and
-// three stored streams per row, 109 VGPRs, FP32): 235 against 244 us at
-// 2048^2 x 10, 440 against 484 us at 4096^2 x 10.
+// three stored streams per row, 109 VGPRs, FP32, 10 layers of 2050 rows): 235
+// against 244 us at 2048 columns, 440 against 484 us at 4096.  In the stencil
+// kernel, which computes 4% more strip columns here: faster at 4096^2 x 32
+// (FP32 3983 against 4095 us, FP64 5766 against 6170 us), slower at 2048^2 x
+// 10 (311 against 302 us); the autotuner picks.
 template <int NLAP> constexpr int p4a_cols() {
   constexpr int H = (NLAP > 1) ? NLAP : 1, OW = 64 - 2 * H;
   return 32 * ((4 * OW) / 32);
