@@ -241,6 +241,12 @@ int qgw_profile_read(qgw_ctx *ctx, double *ms, int *count, int npass);
  * that fit, 64-byte row segments).  Default: 4 in the FP32 forms, 2 in FP64,
  * the fastest measured at 2048^2 x 10; the autotuner tries 2, 4 and 8. */
 #define QGW_TUNE_P2M_PAIRS 8
+/* QGW_TUNE_PSI_NT: 1 (default) stores psi from the half-length inverse row
+ * passes (six- and four-pass forms) with non-temporal stores, 0 with plain
+ * ones.  The tendency pass that reads psi next runs faster after the
+ * non-temporal stores at 2048^2 x 10 (by more than the row pass loses), not at
+ * 4096^2 x 32; the autotuner tries both.  Results are the same either way. */
+#define QGW_TUNE_PSI_NT 9
 /* QGW_TUNE_FFT_E: radix schedule of the half-length FFTs of power-of-two
  * lengths from 1024: 16 (radix-16 stages; the default) or 8 (radix-8 stages,
  * twice the threads per row), for the row passes P1 and P3. */

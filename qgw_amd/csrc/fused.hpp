@@ -153,7 +153,7 @@ k_p1m(Geo g, const T *__restrict__ q, T *__restrict__ what, const T *__restrict_
 // adjacent interior point, src/domain.h:308-348).
 template <typename T, int M, int EP>
 __device__ __forceinline__ void p3_store_pair(const Geo &g, const cx<T> *buf, T *__restrict__ psi, T *__restrict__ q,
-                                              T ring_c, int j, long lb, int l0, bool has1, int tl) {
+                                              T ring_c, int j, long lb, int l0, bool has1, int tl, int psi_nt) {
   constexpr int NT = DstHalf<M, EP>::NT;
   const int n = g.n, m_ = g.m, l1 = l0 + 1;
   T *p0 = psi + (lb + l0) * g.LS + (long)j * g.P;
@@ -164,8 +164,8 @@ __device__ __forceinline__ void p3_store_pair(const Geo &g, const cx<T> *buf, T 
   for (int i = tl; i < n; i += NT) {
     const int x = i + 1;
     const cx<T> c = buf[rslot(i)];
-    p0[x] = c.x;
-    if (has1) p1[x] = c.y;
+    psi_st(p0, x, c.x, psi_nt);
+    if (has1) psi_st(p1, x, c.y, psi_nt);
     if (x == 1 || x == n) {
       const long o = (long)j * g.P + (x == 1 ? 0 : n + 1);
       q0r[o] = ring_c * c.x;
@@ -197,7 +197,7 @@ __device__ __forceinline__ void p3_store_pair(const Geo &g, const cx<T> *buf, T 
 template <typename T, int M, int EP>
 __global__ void __launch_bounds__((DstHalf<M, EP>::NT))
 k_p3m(Geo g, const T *__restrict__ what, T *__restrict__ psi, T *__restrict__ q, const T *__restrict__ cm2l,
-      T ring_c, const cx<T> *__restrict__ twh, const cx<T> *__restrict__ tw2) {
+      T ring_c, const cx<T> *__restrict__ twh, const cx<T> *__restrict__ tw2, int psi_nt) {
   constexpr int E = DstHalf<M, EP>::E, NT = DstHalf<M, EP>::NT;
   __shared__ cx<T> buf[DstHalf<M, EP>::slots];
   __shared__ TwTab<T, M> tb;
@@ -229,7 +229,7 @@ k_p3m(Geo g, const T *__restrict__ what, T *__restrict__ psi, T *__restrict__ q,
     dst_pre_lds<T, M, EP>(h, tl, buf, tb2);
     fft_pipe<T, M, EP>(h, buf, twa, twf, tl);
     dst_post<T, M, EP>(h, buf, tl);
-    p3_store_pair<T, M, EP>(g, buf, psi, q, ring_c, ri.j, (long)ri.mb * nlm, l0, has1, tl);
+    p3_store_pair<T, M, EP>(g, buf, psi, q, ring_c, ri.j, (long)ri.mb * nlm, l0, has1, tl, psi_nt);
   }
 }
 

@@ -528,10 +528,30 @@ k_p2h(Geo g, T *__restrict__ whatT, const double *__restrict__ La2, const double
   }
 }
 
+// Element x of a psi row: a plain store, or (nt) a non-temporal one, written
+// as a buffer store with the nt cache-policy bit (a plain store and a
+// non-temporal one to the same address under a uniform branch would be merged
+// by the compiler into one plain store).  At 2048^2 x 10 the non-temporal psi
+// stores of the inverse row pass make the tendency pass that reads psi next
+// faster by more than they slow the row pass itself (QGW_TUNE_PSI_NT).
+template <typename T> __device__ __forceinline__ void psi_st(T *row, int x, T v, int nt) {
+  if (nt) {
+    const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(row, (short)0, 0x7ffffff0, 0x00020000);
+    if constexpr (sizeof(T) == 8) {
+      typedef unsigned u2 __attribute__((ext_vector_type(2)));
+      __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), r, x * 8, 0, 2);
+    } else {
+      __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, x * 4, 0, 2);
+    }
+  } else {
+    row[x] = v;
+  }
+}
+
 template <typename T, int M, int EP>
 __global__ void __launch_bounds__((DstHalf<M, EP>::NT))
 k_p3h(Geo g, const T *__restrict__ what, T *__restrict__ psi, T *__restrict__ q, T ring_c,
-      const cx<T> *__restrict__ twh, const cx<T> *__restrict__ tw2) {
+      const cx<T> *__restrict__ twh, const cx<T> *__restrict__ tw2, int psi_nt) {
   constexpr int E = DstHalf<M, EP>::E, NT = DstHalf<M, EP>::NT;
   __shared__ cx<T> buf[DstHalf<M, EP>::slots];
   const int t = threadIdx.x;
@@ -555,8 +575,8 @@ k_p3h(Geo g, const T *__restrict__ what, T *__restrict__ psi, T *__restrict__ q,
   for (int i = t; i < n; i += NT) {
     const int x = i + 1;
     const cx<T> c = buf[rslot(i)];
-    p0[x] = c.x;
-    if (has1) p1[x] = c.y;
+    psi_st(p0, x, c.x, psi_nt);
+    if (has1) psi_st(p1, x, c.y, psi_nt);
     if (x == 1 || x == n) {
       const long o = (long)j * g.P + (x == 1 ? 0 : n + 1);
       q0r[o] = ring_c * c.x;
@@ -722,7 +742,7 @@ k_p1hp(Geo g, const T *__restrict__ q, T *__restrict__ what, const cx<T> *__rest
 template <typename T, int M, int EP>
 __global__ void __launch_bounds__((DstHalf<M, EP>::NT))
 k_p3hp(Geo g, const T *__restrict__ what, T *__restrict__ psi, T *__restrict__ q, T ring_c,
-       const cx<T> *__restrict__ twh, const cx<T> *__restrict__ tw2) {
+       const cx<T> *__restrict__ twh, const cx<T> *__restrict__ tw2, int psi_nt) {
   constexpr int E = DstHalf<M, EP>::E, NT = DstHalf<M, EP>::NT;
   __shared__ cx<T> buf[DstHalf<M, EP>::slots];
   __shared__ TwTab<T, M> tb;
@@ -768,8 +788,8 @@ k_p3hp(Geo g, const T *__restrict__ what, T *__restrict__ psi, T *__restrict__ q
     for (int i = tl; i < n; i += NT) {
       const int x = i + 1;
       const cx<T> c = buf[rslot(i)];
-      p0[x] = c.x;
-      if (has1) p1[x] = c.y;
+      psi_st(p0, x, c.x, psi_nt);
+      if (has1) psi_st(p1, x, c.y, psi_nt);
       if (x == 1 || x == n) {
         const long o = (long)j * g.P + (x == 1 ? 0 : n + 1);
         q0r[o] = ring_c * c.x;

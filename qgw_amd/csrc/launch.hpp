@@ -46,6 +46,7 @@ struct LaunchArgs {
   int p4_waves = 40;           // streaming P4: target wavefronts per SIMD (sets the row chunk)
   int t_kernel = 0;            // projection: 0 auto (matrix cores for FP32 at >= 16 layers), 1 VALU, 2 matrix cores
   int p2m_g = 0;               // four-pass column pass: column pairs per block (0: the most that fit)
+  int psi_nt = 1;              // the half-length inverse row passes store psi non-temporally
   unsigned long long *cfl_max = nullptr;
 };
 

@@ -73,11 +73,11 @@ template <int M, int EP> static void p3h_go(const LaunchArgs &a, dim3 grid) {
     const unsigned pg = persist_grid(k_p3hp<T, M, EP>, DstHalf<M, EP>::NT, (long)grid.x, a.cus, occ);
     hipLaunchKernelGGL((k_p3hp<T, M, EP>), dim3(pg), dim3(DstHalf<M, EP>::NT), 0, a.stream, a.g,
                        (const T *)a.what, (T *)a.psi, (T *)a.q, (T)a.ring_c, (const cx<T> *)a.twxh,
-                       (const cx<T> *)a.twx);
+                       (const cx<T> *)a.twx, a.psi_nt);
     return;
   }
   hipLaunchKernelGGL((k_p3h<T, M, EP>), grid, dim3(DstHalf<M, EP>::NT), 0, a.stream, a.g, (const T *)a.what,
-                     (T *)a.psi, (T *)a.q, (T)a.ring_c, (const cx<T> *)a.twxh, (const cx<T> *)a.twx);
+                     (T *)a.psi, (T *)a.q, (T)a.ring_c, (const cx<T> *)a.twxh, (const cx<T> *)a.twx, a.psi_nt);
 }
 
 template <int M, int EP> static void p2h_go(const LaunchArgs &a, dim3 grid) {
@@ -218,7 +218,7 @@ template <int M, int EP> static void p3m_go(const LaunchArgs &a) {
   const unsigned pg = persist_grid(k_p3m<T, M, EP>, DstHalf<M, EP>::NT, items, a.cus, occ);
   hipLaunchKernelGGL((k_p3m<T, M, EP>), dim3(pg), dim3(DstHalf<M, EP>::NT), 0, a.stream, a.g, (const T *)a.what,
                      (T *)a.psi, (T *)a.q, (const T *)a.cm2l, (T)a.ring_c, (const cx<T> *)a.twxh,
-                     (const cx<T> *)a.twx);
+                     (const cx<T> *)a.twx, a.psi_nt);
 }
 template <int M, int EP, int G> static void p2m_launch(const LaunchArgs &a) {
   const int ncg = (a.g.n + 2 * G - 1) / (2 * G);

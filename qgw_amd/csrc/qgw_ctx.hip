@@ -78,6 +78,7 @@ struct qgw_ctx {
   int fft_e_col; // and of the column pass (QGW_TUNE_FFT_E_COL)
   int fuse;      // QGW_TUNE_FUSE: 1 four-pass inversion, 0 six passes; -1: four passes not available
   int p2m_g;     // QGW_TUNE_P2M_PAIRS: column pairs per block of the four-pass column pass (0: most)
+  int psi_nt;    // QGW_TUNE_PSI_NT: non-temporal psi stores of the half-length inverse row passes
   int per;       // periodic domain (bc_fac = -1)
   // dense sine transforms (bounded grids whose transform lengths the FFT
   // kernels do not cover): S matrices n x n and m x m, a whatT-shaped temporary
@@ -407,6 +408,7 @@ int run_invert(qgw_ctx *c) {
   a.fft_e = c->fft_e;
   a.fft_e_col = c->fft_e_col;
   a.p2m_g = c->p2m_g;
+  a.psi_nt = c->psi_nt;
   a.cus = c->cus;
   a.dk = c->dk;
   a.per = c->per;
@@ -1167,6 +1169,7 @@ qgw_ctx *create_group(const qgw_params *ps, int nb, int precision, hipStream_t e
   // at 2048^2 x 10 (FP32 4: 180 us against 191 for 8 and 194 for 2; FP64 2:
   // 357 us against 390 for 4)
   c->p2m_g = (precision == QGW_FP64) ? 2 : 4;
+  c->psi_nt = 1;
   c->modes_set = false;
   // the zero fills above ran on the null stream; the step runs on c->stream
   if (hipDeviceSynchronize() != hipSuccess) {
@@ -1294,13 +1297,15 @@ int autotune_group(qgw_ctx *c, int reps, int *p4_tile, double *best_ms) {
   c->p4_tile = arg;
   if (!st && c->fuse >= 0) {
     // the inversion in six passes, or in four with 2, 4 or 8 column pairs per
-    // block of the column pass
+    // block of the column pass, each with plain and non-temporal psi stores
+    // (which change the time of the tendency pass that follows)
     double bf = 1e300;
-    int farg = c->fuse, garg = c->p2m_g;
+    int farg = c->fuse, garg = c->p2m_g, narg = c->psi_nt;
     const int forms[4][2] = {{0, 0}, {1, 2}, {1, 4}, {1, 8}};
-    for (int v = 0; v < 4 && !st; v++) {
-      c->fuse = forms[v][0];
-      if (forms[v][0]) c->p2m_g = forms[v][1];
+    for (int v = 0; v < 8 && !st; v++) {
+      c->fuse = forms[v / 2][0];
+      if (forms[v / 2][0]) c->p2m_g = forms[v / 2][1];
+      c->psi_nt = v % 2;
       double ms = 0;
       st = time_steps(c, 1, &ms);  // first-launch code loading
       if (!st) st = time_candidate(c, reps, &ms);
@@ -1308,10 +1313,26 @@ int autotune_group(qgw_ctx *c, int reps, int *p4_tile, double *best_ms) {
         bf = ms;
         farg = c->fuse;
         garg = c->p2m_g;
+        narg = c->psi_nt;
       }
     }
     c->fuse = farg;
     c->p2m_g = garg;
+    c->psi_nt = narg;
+  } else if (!st) {
+    // plain or non-temporal psi stores
+    double bn = 1e300;
+    int narg = c->psi_nt;
+    for (int v = 0; v < 2 && !st; v++) {
+      c->psi_nt = v;
+      double ms = 0;
+      st = time_candidate(c, reps, &ms);
+      if (!st && ms < bn) {
+        bn = ms;
+        narg = v;
+      }
+    }
+    c->psi_nt = narg;
   }
   if (!st && c->prec != QGW_FP64 && c->fuse != 1) {
     // projection: VALU or matrix cores
@@ -1806,6 +1827,11 @@ int qgw_tune_set(qgw_ctx *c, int knob, int value) {
     c->p2m_g = value;
     return QGW_OK;
   }
+  if (knob == QGW_TUNE_PSI_NT) {
+    if (value < 0 || value > 1) return QGW_ERR_ARG;
+    c->psi_nt = value;
+    return QGW_OK;
+  }
   if (knob == QGW_TUNE_FUSE) {
     if (value < 0 || value > 1 || (value == 1 && c->fuse < 0)) return QGW_ERR_ARG;
     if (c->fuse >= 0) c->fuse = value;
@@ -1849,6 +1875,10 @@ int qgw_tune_get(qgw_ctx *c, int knob, int *value) {
   }
   if (knob == QGW_TUNE_P2M_PAIRS) {
     *value = c->p2m_g;
+    return QGW_OK;
+  }
+  if (knob == QGW_TUNE_PSI_NT) {
+    *value = c->psi_nt;
     return QGW_OK;
   }
   if (knob == QGW_TUNE_FFT_E) {

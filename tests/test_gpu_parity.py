@@ -369,6 +369,39 @@ def test_p4_aligned_blocks(prec, case):
         o.close()
 
 
+@pytest.mark.parametrize("prec", [Q.FP64, Q.FP32])
+@pytest.mark.parametrize("fuse,dst", [(0, 2), (0, 1), (1, 2)])
+def test_psi_store_policy(prec, fuse, dst):
+    """Plain and non-temporal psi stores of the inverse row passes
+    (QGW_TUNE_PSI_NT; pipelined and one-block rows, six and four passes) give
+    the same results bit for bit, and the oracle's (FP64)."""
+    p = _small(256, 3, NY=192, bc_fac=0.5)
+    _announce("psi store policy", prec, fuse, dst)
+    modes = Q.eigmode(p)
+    q0 = Q.make_ic(p, modes)[1]
+    out = {}
+    for nt in (0, 1):
+        m = Q.Model(p, prec, modes)
+        m.tune_set(Q.TUNE_FUSE, fuse)
+        m.tune_set(Q.TUNE_DST, dst)
+        m.tune_set(Q.TUNE_PSI_NT, nt)
+        assert m.tune_get(Q.TUNE_PSI_NT) == nt
+        m.set_q(q0)
+        m.run(3)
+        q = m.q
+        m.invert_pv()  # psi of the new q (the inversion also rewrites q's wall ring)
+        out[nt] = (q, m.psi)
+        m.close()
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+    if prec == Q.FP64:
+        o = O.Oracle(p)
+        o.set_modes(*modes)
+        o.set_q(q0)
+        o.run(3)
+        assert rel(out[1][0], o.field(O.Q)) < 1e-12
+        o.close()
+
+
 @pytest.mark.parametrize("shape", [(64, 64, 3), (96, 128, 2), (256, 192, 5), (384, 256, 2)])
 def test_dst_half_vs_full(shape):
     """Sine transforms through half-length FFTs (one block per row pair, and
