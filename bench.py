@@ -493,7 +493,8 @@ def main():
     pass_table = {k: {"ms_per_launch": round(per_launch[k], 5),
                       "GBs_algorithmic": round(words[k] * s * work / (per_launch[k] * 1e-3) / 1e9, 1)}
                   for k in per_launch}
-    total_alg = sum(words.values()) * s * work
+    # the passes the step ran: 68 B per point.layer in six passes, 52 B in four (FP32 + compensation)
+    total_alg = sum(words[k] for k in per_launch) * s * work
     step_gbs = total_alg / (ms_step * 1e-3) / 1e9
     traffic, tkernel = kernel_traffic(dom, head, tunes.get(head))
     out = {
