@@ -229,12 +229,13 @@ int qgw_profile_read(qgw_ctx *ctx, double *ms, int *count, int npass);
 #define QGW_TUNE_DST 4
 /* QGW_TUNE_FUSE: 1 runs the inversion in four passes over HBM -- the
  * layer<->mode projections inside the row transforms, the column transforms
- * on 64-byte column tiles of the mode-major spectral array, no transposed
- * copy (where the half-length transforms apply on a bounded grid); 0, the
+ * on column tiles of the mode-major spectral array, no transposed copy
+ * (where the half-length transforms apply on a bounded grid); 0, the
  * default, in six: the projections fused into two transposes.  Fewer bytes,
- * but at 2048^2 x 10 the column-tile pass is slower than the transposed
- * column pass by more than the two transposes cost; the autotuner tries
- * both. */
+ * but the column-tile pass is slower than the transposed column pass: at
+ * 2048^2 x 10 the two forms take about the same time with plain psi stores,
+ * the four-pass form is the faster (by 2%) with the non-temporal psi stores
+ * of QGW_TUNE_PSI_NT; the autotuner tries both. */
 #define QGW_TUNE_FUSE 7
 /* QGW_TUNE_P2M_PAIRS: column pairs per block of the four-pass column pass,
  * 1, 2, 4 or 8 (capped by the 1024-thread block and the LDS; 0: the most
@@ -247,6 +248,13 @@ int qgw_profile_read(qgw_ctx *ctx, double *ms, int *count, int npass);
  * non-temporal stores at 2048^2 x 10 (by more than the row pass loses), not at
  * 4096^2 x 32; the autotuner tries both.  Results are the same either way. */
 #define QGW_TUNE_PSI_NT 9
+/* QGW_TUNE_P4_REV: 1 (default) runs the interior walks of the odd row chunks
+ * of the streaming tendency pass (QGW_TUNE_P4_TILE 4 and 5) up their chunk,
+ * 0 every walk down.  Two neighbouring chunks both load the psi and q rows
+ * around their common boundary; walking in opposite directions they load
+ * them at about the same time, so the second load meets L2.  Every stencil
+ * is evaluated in the same order either way, so results are the same bits. */
+#define QGW_TUNE_P4_REV 10
 /* QGW_TUNE_FFT_E: radix schedule of the half-length FFTs of power-of-two
  * lengths from 1024: 16 (radix-16 stages; the default) or 8 (radix-8 stages,
  * twice the threads per row), for the row passes P1 and P3. */

@@ -1461,7 +1461,7 @@ template <int W> __device__ constexpr int wslot(int ph, int i) { return ((ph - i
 // OWN: the walk stores only the output columns X0 .. X0 + ownw - 1 (the
 // block's own columns in k_p4a); the other outputs of its strip belong to a
 // neighbouring block, which computes them as well.
-template <typename T, int NLAP, int KAHAN, bool EDGE, bool OWN = false>
+template <typename T, int NLAP, int KAHAN, bool EDGE, bool OWN = false, bool REV = false>
 __device__ __forceinline__ void p4s_walk(const Geo &g, const T *__restrict__ P, const T *__restrict__ Qf,
                                          T *__restrict__ qn, T *__restrict__ fnew, const T *__restrict__ f1,
                                          const T *__restrict__ f2, comp_t<T> *__restrict__ comp,
@@ -1489,6 +1489,12 @@ __device__ __forceinline__ void p4s_walk(const Geo &g, const T *__restrict__ P, 
   const T c0 = (T)cf.c0, c1 = (T)cf.c1, c2 = (T)cf.c2;
   const T drag = (T)cf.drag, tfac = (T)cf.topo_fac;
   const bool use_nu = cf.nu != 0.0;
+  // REV: the walk runs up its chunk, virtual row v being grid row y0 + y1 - 1 - v
+  // (an interior walk only); every stencil below is written for grid rows u =
+  // y + 1 and d = y - 1, so both directions give the same bits
+  static_assert(!(REV && EDGE), "reversed walks are interior walks");
+  const int ymir = y0 + y1 - 1;
+  auto prow = [&](int v) -> unsigned { return (unsigned)(REV ? ymir - v : v); };
 
   T pw[W], nbw[W];                                    // psi rows; psi of the ring columns' neighbour
   T lv[NL][3];                                        // level a: rows r-a, r-a-1, r-a-2 (circular)
@@ -1535,7 +1541,7 @@ __device__ __forceinline__ void p4s_walk(const Geo &g, const T *__restrict__ P, 
       if constexpr (EDGE)
         bw[s] = (r - H >= 0 && r - H <= m + 1) ? wind[r - H] : T(0);
       else
-        bw[s] = buf_ld<T>(vW, 0u, (unsigned)(r - H) * (unsigned)sizeof(T));
+        bw[s] = buf_ld<T>(vW, 0u, prow(r - H) * (unsigned)sizeof(T));
     }
     if constexpr (EDGE) {
       auto ldrow = [&](const T *base, int row) -> T {
@@ -1557,17 +1563,17 @@ __device__ __forceinline__ void p4s_walk(const Geo &g, const T *__restrict__ P, 
     } else {
       // every row an interior walk touches lies inside the layer
       if (r >= rend) return;  // past the walk (wavefront-uniform)
-      bp[s] = buf_ld<T>(vP, xb, (unsigned)r * rowb);
+      bp[s] = buf_ld<T>(vP, xb, prow(r) * rowb);
       // the Jacobians and nu lap q use q (and the topography) at rows y0 - 1
       // .. y1 only: the first 2H - 2 warm-up rows load psi alone (wavefront-uniform)
       if (r + 1 - H >= y0 - 1) {
-        bq[s] = buf_ld<T>(vQ, xb, (unsigned)(r + 1 - H) * rowb);
-        if (bottom) bt[s] = buf_ld<T>(vT, xb, (unsigned)(r + 1 - H) * rowb);
+        bq[s] = buf_ld<T>(vQ, xb, prow(r + 1 - H) * rowb);
+        if (bottom) bt[s] = buf_ld<T>(vT, xb, prow(r + 1 - H) * rowb);
       }
       if (r - H >= y0) {  // the AB3 history only for output rows (wavefront-uniform)
-        b1[s] = buf_ld<T>(v1, xb, (unsigned)(r - H) * rowb);
-        b2[s] = buf_ld<T>(v2, xb, (unsigned)(r - H) * rowb);
-        if (KAHAN) bc[s] = buf_ld<comp_t<T>>(vC, xbc, (unsigned)(r - H) * rowbc);
+        b1[s] = buf_ld<T>(v1, xb, prow(r - H) * rowb);
+        b2[s] = buf_ld<T>(v2, xb, prow(r - H) * rowb);
+        if (KAHAN) bc[s] = buf_ld<comp_t<T>>(vC, xbc, prow(r - H) * rowbc);
       }
     }
   };
@@ -1576,7 +1582,8 @@ __device__ __forceinline__ void p4s_walk(const Geo &g, const T *__restrict__ P, 
   auto body = [&](auto PHc, int r) {
     constexpr int ph = decltype(PHc)::value;
     constexpr int cur = ph % RING;
-    constexpr int j0 = ph % 3, j1 = (ph + 2) % 3, j2 = (ph + 1) % 3;  // rows y+1, y, y-1
+    constexpr int j0 = ph % 3, j1 = (ph + 2) % 3, j2 = (ph + 1) % 3;  // virtual rows y+1, y, y-1
+    constexpr int jU = REV ? j2 : j0, jD = REV ? j0 : j2;              // grid rows y+1, y-1
     const int y = r - H;  // output row of this iteration
     if (r + D < rend) fetch(IC<(ph + D) % RING>{}, r + D);
     const T p0 = bp[cur];
@@ -1593,7 +1600,8 @@ __device__ __forceinline__ void p4s_walk(const Geo &g, const T *__restrict__ P, 
       T s0 = pw[wslot<W>(ph, 0)], s1 = pw[wslot<W>(ph, 1)], s2 = pw[wslot<W>(ph, 2)];
 #pragma unroll
       for (int a = 1; a <= NLAP; a++) {
-        T v = (dpp_left(s1) + dpp_right(s1) + s0 + s2 - T(4) * s1) * inv_d2;
+        const T su = REV ? s2 : s0, sd = REV ? s0 : s2;  // grid rows above and below
+        T v = (dpp_left(s1) + dpp_right(s1) + su + sd - T(4) * s1) * inv_d2;
         if constexpr (EDGE) {
           const int yl = r - a;
           if (yl == 0 || yl == m + 1) {
@@ -1630,23 +1638,23 @@ __device__ __forceinline__ void p4s_walk(const Geo &g, const T *__restrict__ P, 
     }
     if (y < y0) return;  // warm-up row (wavefront-uniform)
     // ---- tendency at row y (rhs, src/dynamics.h:52-71)
-    const T jpq = arakawa12<T>(jpl[j1], jpr[j1], jpc[j0], jpc[j2], jpl[j0], jpr[j0], jpl[j2], jpr[j2], jql[j1],
-                               jqr[j1], jqc[j0], jqc[j2], jql[j0], jqr[j0], jql[j2], jqr[j2]) *
+    const T jpq = arakawa12<T>(jpl[j1], jpr[j1], jpc[jU], jpc[jD], jpl[jU], jpr[jU], jpl[jD], jpr[jD], jql[j1],
+                               jqr[j1], jqc[jU], jqc[jD], jql[jU], jqr[jU], jql[jD], jqr[jD]) *
                   i12;
     T f = -jpq - beta * (jpr[j1] - jpl[j1]) * i2d;
-    if (use_nu) f += nu * ((jql[j1] + jqr[j1] + jqc[j0] + jqc[j2] - T(4) * jqc[j1]) * inv_d2);
+    if (use_nu) f += nu * ((jql[j1] + jqr[j1] + jqc[jU] + jqc[jD] - T(4) * jqc[j1]) * inv_d2);
     if constexpr (NLAP >= 1) f += nuh * hyp;
     if (km == 0) f += bw[cur];
     if (bottom) {
-      const T lap1 = (jpl[j1] + jpr[j1] + jpc[j0] + jpc[j2] - T(4) * jpc[j1]) * inv_d2;
-      const T jt = arakawa12<T>(jpl[j1], jpr[j1], jpc[j0], jpc[j2], jpl[j0], jpr[j0], jpl[j2], jpr[j2], jtl[j1],
-                                jtr[j1], jtc[j0], jtc[j2], jtl[j0], jtr[j0], jtl[j2], jtr[j2]) *
+      const T lap1 = (jpl[j1] + jpr[j1] + jpc[jU] + jpc[jD] - T(4) * jpc[j1]) * inv_d2;
+      const T jt = arakawa12<T>(jpl[j1], jpr[j1], jpc[jU], jpc[jD], jpl[jU], jpr[jU], jpl[jD], jpr[jD], jtl[j1],
+                                jtr[j1], jtc[jU], jtc[jD], jtl[jU], jtr[jU], jtl[jD], jtr[jD]) *
                    i12;
       f += drag * lap1 - jt * tfac;
     }
     if (!outl) return;
-    const int ro = y * g.P;  // output row offset (wavefront-uniform)
-    const unsigned rb = (unsigned)y * rowb;
+    const int ro = y * g.P;  // output row offset (wavefront-uniform; edge walks)
+    const unsigned rb = prow(y) * rowb;
     if constexpr (OWN) {
       if ((unsigned)(x - X0) >= (unsigned)ownw) return;  // a neighbouring block's column
     }
@@ -1675,7 +1683,7 @@ __device__ __forceinline__ void p4s_walk(const Geo &g, const T *__restrict__ P, 
       if constexpr (KAHAN) st_at(comp + ro, xbc, cs);
       st_at(qn + ro, xb, qv);
     } else {
-      if constexpr (KAHAN) buf_st(vC, xbc, (unsigned)y * rowbc, cs);
+      if constexpr (KAHAN) buf_st(vC, xbc, prow(y) * rowbc, cs);
       buf_st(vN, xb, rb, qv);
     }
     if constexpr (EDGE) {
@@ -1723,7 +1731,7 @@ template <typename T, int NLAP, int KAHAN>
 __global__ void __launch_bounds__(256)
 k_p4s(Geo g, const T *__restrict__ psi, const T *__restrict__ q, T *__restrict__ qn, T *__restrict__ fnew,
       const T *__restrict__ f1, const T *__restrict__ f2, comp_t<T> *__restrict__ comp, const T *__restrict__ topo,
-      const T *__restrict__ wind, StepCoef cf, int nstrip, int chunk, int nchunk) {
+      const T *__restrict__ wind, StepCoef cf, int nstrip, int chunk, int nchunk, int rev) {
   constexpr int H = (NLAP > 1) ? NLAP : 1;  // halo
   constexpr int OW = 64 - 2 * H;            // output columns per strip
   const int n = g.n, m = g.m;
@@ -1755,7 +1763,14 @@ k_p4s(Geo g, const T *__restrict__ psi, const T *__restrict__ q, T *__restrict__
   // interior: no wall column or row in reach of the stencils, so every row the
   // walk loads lies inside the layer
   const bool interior = xlo >= 1 && xlo + 63 <= n && y0 - H - NLAP >= 1 && y1 + H - 1 <= m;
-  if (interior)
+  // rev: the interior walks of odd chunks run up their chunk, so that the rows
+  // two neighbouring chunks both load (the psi and q rows around their common
+  // boundary) are loaded by both at about the same time, at the start or the
+  // end of both walks, and the second load meets L2
+  if (interior && rev && (cidx & 1))
+    p4s_walk<T, NLAP, KAHAN, false, false, true>(g, P, Qf, qn, fnew, f1, f2, comp, topo, wind, cf, x, lane, km, y0,
+                                                 y1, false);
+  else if (interior)
     p4s_walk<T, NLAP, KAHAN, false>(g, P, Qf, qn, fnew, f1, f2, comp, topo, wind, cf, x, lane, km, y0, y1, false);
   else
     p4s_walk<T, NLAP, KAHAN, true>(g, P, Qf, qn, fnew, f1, f2, comp, topo, wind, cf, x, lane, km, y0, y1,
@@ -1785,7 +1800,7 @@ template <typename T, int NLAP, int KAHAN>
 __global__ void __launch_bounds__(256)
 k_p4a(Geo g, const T *__restrict__ psi, const T *__restrict__ q, T *__restrict__ qn, T *__restrict__ fnew,
       const T *__restrict__ f1, const T *__restrict__ f2, comp_t<T> *__restrict__ comp, const T *__restrict__ topo,
-      const T *__restrict__ wind, StepCoef cf, int nbx, int chunk, int nchunk) {
+      const T *__restrict__ wind, StepCoef cf, int nbx, int chunk, int nchunk, int rev) {
   constexpr int H = (NLAP > 1) ? NLAP : 1;  // halo
   constexpr int BW = p4a_cols<NLAP>();      // output columns per block
   constexpr int OWU = BW / 4;               // output columns per wavefront
@@ -1815,7 +1830,10 @@ k_p4a(Geo g, const T *__restrict__ psi, const T *__restrict__ q, T *__restrict__
   if (KAHAN) comp += lo;
   const bool ringstrip = (xlo <= 0 || xlo + 63 >= n + 1);
   const bool interior = xlo >= 1 && xlo + 63 <= n && y0 - H - NLAP >= 1 && y1 + H - 1 <= m;
-  if (interior)
+  if (interior && rev && (cidx & 1))  // as in k_p4s
+    p4s_walk<T, NLAP, KAHAN, false, true, true>(g, P, Qf, qn, fnew, f1, f2, comp, topo, wind, cf, x, lane, km, y0, y1,
+                                                false, X0, OWU);
+  else if (interior)
     p4s_walk<T, NLAP, KAHAN, false, true>(g, P, Qf, qn, fnew, f1, f2, comp, topo, wind, cf, x, lane, km, y0, y1,
                                           false, X0, OWU);
   else

@@ -47,6 +47,7 @@ struct LaunchArgs {
   int t_kernel = 0;            // projection: 0 auto (matrix cores for FP32 at >= 16 layers), 1 VALU, 2 matrix cores
   int p2m_g = 0;               // four-pass column pass: column pairs per block (0: the most that fit)
   int psi_nt = 1;              // the half-length inverse row passes store psi non-temporally
+  int p4_rev = 1;              // streaming P4: the interior walks of odd row chunks run up their chunk
   unsigned long long *cfl_max = nullptr;
 };
 

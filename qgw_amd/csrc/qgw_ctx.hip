@@ -79,6 +79,7 @@ struct qgw_ctx {
   int fuse;      // QGW_TUNE_FUSE: 1 four-pass inversion, 0 six passes; -1: four passes not available
   int p2m_g;     // QGW_TUNE_P2M_PAIRS: column pairs per block of the four-pass column pass (0: most)
   int psi_nt;    // QGW_TUNE_PSI_NT: non-temporal psi stores of the half-length inverse row passes
+  int p4_rev;    // QGW_TUNE_P4_REV: streaming P4 walks of odd row chunks run up their chunk
   int per;       // periodic domain (bc_fac = -1)
   // dense sine transforms (bounded grids whose transform lengths the FFT
   // kernels do not cover): S matrices n x n and m x m, a whatT-shaped temporary
@@ -409,6 +410,7 @@ int run_invert(qgw_ctx *c) {
   a.fft_e_col = c->fft_e_col;
   a.p2m_g = c->p2m_g;
   a.psi_nt = c->psi_nt;
+  a.p4_rev = c->p4_rev;
   a.cus = c->cus;
   a.dk = c->dk;
   a.per = c->per;
@@ -1170,6 +1172,7 @@ qgw_ctx *create_group(const qgw_params *ps, int nb, int precision, hipStream_t e
   // 357 us against 390 for 4)
   c->p2m_g = (precision == QGW_FP64) ? 2 : 4;
   c->psi_nt = 1;
+  c->p4_rev = 1;
   c->modes_set = false;
   // the zero fills above ran on the null stream; the step runs on c->stream
   if (hipDeviceSynchronize() != hipSuccess) {
@@ -1832,6 +1835,11 @@ int qgw_tune_set(qgw_ctx *c, int knob, int value) {
     c->psi_nt = value;
     return QGW_OK;
   }
+  if (knob == QGW_TUNE_P4_REV) {
+    if (value < 0 || value > 1) return QGW_ERR_ARG;
+    c->p4_rev = value;
+    return QGW_OK;
+  }
   if (knob == QGW_TUNE_FUSE) {
     if (value < 0 || value > 1 || (value == 1 && c->fuse < 0)) return QGW_ERR_ARG;
     if (c->fuse >= 0) c->fuse = value;
@@ -1879,6 +1887,10 @@ int qgw_tune_get(qgw_ctx *c, int knob, int *value) {
   }
   if (knob == QGW_TUNE_PSI_NT) {
     *value = c->psi_nt;
+    return QGW_OK;
+  }
+  if (knob == QGW_TUNE_P4_REV) {
+    *value = c->p4_rev;
     return QGW_OK;
   }
   if (knob == QGW_TUNE_FFT_E) {

@@ -369,6 +369,50 @@ def test_p4_aligned_blocks(prec, case):
         o.close()
 
 
+@pytest.mark.parametrize("prec", [Q.FP64, Q.FP32, Q.FP32_PLAIN])
+@pytest.mark.parametrize("case", [(600, 320, 4, 1e8, {}), (500, 200, 12, None, {}),
+                                  (1024, 96, 2, -1e4, {}), (448, 256, 0, 0.0, dict(nu=20.0, nl=3))])
+def test_p4_reversed_walks(prec, case):
+    """The interior walks of odd row chunks run up their chunk
+    (QGW_TUNE_P4_REV 1, the default) or down (0), in the streaming P4 and over
+    aligned column blocks: the same bits either way (q and the tendency after
+    three steps: every stencil -- both Jacobians, the nested Laplacians, nu
+    lap q, the drag -- is evaluated in the same order in both directions;
+    a y difference taken the wrong way round changes the bits), and the
+    oracle's results (FP64).  Row chunks of 16 rows on these grids, so most
+    walks are interior and half of them run up."""
+    NX, NY, n_hyper, nu_h, kw = case
+    kw = dict(kw)
+    nl = kw.pop("nl", 2)
+    bc = 0.5
+    if n_hyper > 8:
+        nu_h, bc = 0.1 * (1000e3 / NX) ** n_hyper / (8.0 ** (n_hyper // 2) * 600.0), 0.0
+    p = _small(NX, nl, NY=NY, n_hyper=n_hyper, nu_hyper=nu_h, bc_fac=bc, h_topo=150.0, w_topo=80e3, **kw)
+    _announce("p4 reversed walks", prec, case)
+    modes = Q.eigmode(p)
+    q0 = Q.make_ic(p, modes)[1]
+    for tile in (4, 5):
+        out = {}
+        for rev in (0, 1):
+            m = Q.Model(p, prec, modes)
+            m.tune_set(Q.TUNE_P4_TILE, tile)
+            m.tune_set(Q.TUNE_P4_REV, rev)
+            assert m.tune_get(Q.TUNE_P4_REV) == rev
+            m.set_q(q0)
+            m.run(3)
+            out[rev] = (m.q, m.download(Q.FIELD_F0))
+            m.close()
+        assert np.array_equal(out[1][0], out[0][0]), tile
+        assert np.array_equal(out[1][1], out[0][1]), tile
+        if prec == Q.FP64:
+            o = O.Oracle(p)
+            o.set_modes(*modes)
+            o.set_q(q0)
+            o.run(3)
+            assert rel(out[1][0], o.field(O.Q)) < 1e-12
+            o.close()
+
+
 @pytest.mark.parametrize("prec", [Q.FP64, Q.FP32])
 @pytest.mark.parametrize("fuse,dst", [(0, 2), (0, 1), (1, 2)])
 def test_psi_store_policy(prec, fuse, dst):
