@@ -217,7 +217,10 @@ int qgw_profile_read(qgw_ctx *ctx, double *ms, int *count, int npass);
  * only). */
 #define QGW_TUNE_T_KERNEL 2
 /* QGW_TUNE_P4_WAVES sets the row chunk of the streaming P4 so that about
- * 2, 4, 8, 16, 32, 40 (default) or 48 wavefronts per SIMD are launched. */
+ * 2, 4, 8, 16, 32, 40 (default), 48, 64, 96, 128, 192, 256, 384, 512 or 768
+ * wavefronts per SIMD are launched (chunks of at least 16 rows).  Shorter
+ * chunks are faster down to about 16 rows at 2048^2 x 10 (40-48) and 26 rows
+ * at 4096^2 x 32 (384); the autotuner tries 16 to 384. */
 #define QGW_TUNE_P4_WAVES 3
 /* QGW_TUNE_DST selects the sine transforms of P1/P2/P3: 1 through complex
  * FFTs of length n+1 (m+1) with a pre/post pass (both half lengths n+1 and

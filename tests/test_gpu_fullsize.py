@@ -174,13 +174,17 @@ def test_4096x32_steps_against_oracle():
     q64 = m.q
     E64, Z64 = m.diagnostics()[2:]
     m.close()
-    # the P4 over aligned column blocks (the faster form at this width)
-    m = Q.Model(p, Q.FP64, modes)
-    m.tune_set(Q.TUNE_P4_TILE, 5)
-    m.set_q(q0)
-    m.run(2)
-    assert rel(m.q, q64) < 1e-12
-    m.close()
+    # the P4 over aligned column blocks (the faster form at this width), in
+    # 244-row chunks (40 wavefronts per SIMD) and in 26-row chunks (384, the
+    # autotuner's pick at this size)
+    for waves in (40, 384):
+        m = Q.Model(p, Q.FP64, modes)
+        m.tune_set(Q.TUNE_P4_TILE, 5)
+        m.tune_set(Q.TUNE_P4_WAVES, waves)
+        m.set_q(q0)
+        m.run(2)
+        assert rel(m.q, q64) < 1e-12, waves
+        m.close()
     f = Q.Batch([p], Q.FP32, [modes])
     f.tune_set(0, Q.TUNE_T_KERNEL, 2)
     f.set_q(0, q0)
