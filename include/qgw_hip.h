@@ -258,6 +258,13 @@ int qgw_profile_read(qgw_ctx *ctx, double *ms, int *count, int npass);
  * them at about the same time, so the second load meets L2.  Every stencil
  * is evaluated in the same order either way, so results are the same bits. */
 #define QGW_TUNE_P4_REV 10
+/* QGW_TUNE_P4_ORDER: the order in which the walks of the streaming tendency
+ * pass (QGW_TUNE_P4_TILE 4 and 5) are dealt, strips fastest: 1 the layers of
+ * a row chunk before the next chunk, 0 the chunks of a layer before the next
+ * layer.  Default 1 up to 16 layers (4% faster at 2048^2 x 10), else 0 (3-4%
+ * faster at 4096^2 x 32); the autotuner tries both.  Results are the same
+ * either way. */
+#define QGW_TUNE_P4_ORDER 11
 /* QGW_TUNE_FFT_E: radix schedule of the half-length FFTs of power-of-two
  * lengths from 1024: 16 (radix-16 stages; the default) or 8 (radix-8 stages,
  * twice the threads per row), for the row passes P1 and P3. */

@@ -38,9 +38,10 @@ TUNE_FUSE = 7
 TUNE_P2M_PAIRS = 8
 TUNE_PSI_NT = 9
 TUNE_P4_REV = 10
+TUNE_P4_ORDER = 11
 P4_TILES = ["64x8", "64x16", "64x32", "128x16", "stream", "stream-aligned"]
 TUNE_KNOBS = (TUNE_P4_TILE, TUNE_P4_WAVES, TUNE_T_KERNEL, TUNE_DST, TUNE_FFT_E, TUNE_FFT_E_COL, TUNE_FUSE,
-              TUNE_P2M_PAIRS, TUNE_PSI_NT, TUNE_P4_REV)
+              TUNE_P2M_PAIRS, TUNE_PSI_NT, TUNE_P4_REV, TUNE_P4_ORDER)
 _TUNE_CACHE = {}
 
 QGW_OK = 0
@@ -640,7 +641,8 @@ class Model:
                 "dst": ["odd-extension", "half-length", "half-length pipelined", "dense (matrix cores)"][self.tune_get(TUNE_DST)],
                 "fft_e": self.tune_get(TUNE_FFT_E), "fft_e_col": self.tune_get(TUNE_FFT_E_COL),
                 "passes": 4 if self.tune_get(TUNE_FUSE) else 6, "p2m_pairs": self.tune_get(TUNE_P2M_PAIRS),
-                "psi_nt": self.tune_get(TUNE_PSI_NT), "p4_rev": self.tune_get(TUNE_P4_REV)}
+                "psi_nt": self.tune_get(TUNE_PSI_NT), "p4_rev": self.tune_get(TUNE_P4_REV),
+                "p4_order": ["chunks first", "layers first"][self.tune_get(TUNE_P4_ORDER)]}
 
     def autotune(self, reps=5, cache=True):
         """Pick the tuning of this model by timing each variant on its own
@@ -832,7 +834,8 @@ class Batch:
                 "fft_e": self.tune_get(group, TUNE_FFT_E), "fft_e_col": self.tune_get(group, TUNE_FFT_E_COL),
                 "passes": 4 if self.tune_get(group, TUNE_FUSE) else 6,
                 "p2m_pairs": self.tune_get(group, TUNE_P2M_PAIRS),
-                "psi_nt": self.tune_get(group, TUNE_PSI_NT), "p4_rev": self.tune_get(group, TUNE_P4_REV)}
+                "psi_nt": self.tune_get(group, TUNE_PSI_NT), "p4_rev": self.tune_get(group, TUNE_P4_REV),
+                "p4_order": ["chunks first", "layers first"][self.tune_get(group, TUNE_P4_ORDER)]}
 
     def profile(self, on=True):
         self._chk(self.lib.qgw_batch_profile_enable(self.b, 1 if on else 0), "profile_enable")

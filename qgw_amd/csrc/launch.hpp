@@ -48,6 +48,7 @@ struct LaunchArgs {
   int p2m_g = 0;               // four-pass column pass: column pairs per block (0: the most that fit)
   int psi_nt = 1;              // the half-length inverse row passes store psi non-temporally
   int p4_rev = 1;              // streaming P4: the interior walks of odd row chunks run up their chunk
+  int p4_order = 1;            // streaming P4: 1 the layers of a row chunk before the next chunk, 0 the chunks first
   unsigned long long *cfl_max = nullptr;
 };
 

@@ -385,7 +385,7 @@ template <int NLAP, int KAHAN> static void p4s_go(const LaunchArgs &a) {
   const dim3 grid((unsigned)((waves + 3) / 4));
   hipLaunchKernelGGL((k_p4s<T, NLAP, KAHAN>), grid, dim3(256), 0, a.stream, a.g, (const T *)a.psi, (const T *)a.q,
                      (T *)a.qn, (T *)a.fnew, (const T *)a.f1, (const T *)a.f2, (comp_t<T> *)a.comp, (const T *)a.topo,
-                     (const T *)a.wind, a.cf, nstrip, chunk, nchunk, a.p4_rev);
+                     (const T *)a.wind, a.cf, nstrip, chunk, nchunk, a.p4_rev, a.p4_order);
 }
 
 // the streaming form over aligned column blocks: blocks of 4 wavefronts over
@@ -401,7 +401,7 @@ template <int NLAP, int KAHAN> static void p4a_go(const LaunchArgs &a) {
   const dim3 grid((unsigned)((long)nbx * nchunk * a.g.nl));
   hipLaunchKernelGGL((k_p4a<T, NLAP, KAHAN>), grid, dim3(256), 0, a.stream, a.g, (const T *)a.psi,
                      (const T *)a.q, (T *)a.qn, (T *)a.fnew, (const T *)a.f1, (const T *)a.f2, (comp_t<T> *)a.comp,
-                     (const T *)a.topo, (const T *)a.wind, a.cf, nbx, chunk, nchunk, a.p4_rev);
+                     (const T *)a.topo, (const T *)a.wind, a.cf, nbx, chunk, nchunk, a.p4_rev, a.p4_order);
 }
 
 template <int NLAP, int KAHAN> static int p4_tile(const LaunchArgs &a) {

@@ -80,6 +80,7 @@ struct qgw_ctx {
   int p2m_g;     // QGW_TUNE_P2M_PAIRS: column pairs per block of the four-pass column pass (0: most)
   int psi_nt;    // QGW_TUNE_PSI_NT: non-temporal psi stores of the half-length inverse row passes
   int p4_rev;    // QGW_TUNE_P4_REV: streaming P4 walks of odd row chunks run up their chunk
+  int p4_order;  // QGW_TUNE_P4_ORDER: streaming P4 walks dealt layers (1) or row chunks (0) first
   int per;       // periodic domain (bc_fac = -1)
   // dense sine transforms (bounded grids whose transform lengths the FFT
   // kernels do not cover): S matrices n x n and m x m, a whatT-shaped temporary
@@ -411,6 +412,7 @@ int run_invert(qgw_ctx *c) {
   a.p2m_g = c->p2m_g;
   a.psi_nt = c->psi_nt;
   a.p4_rev = c->p4_rev;
+  a.p4_order = c->p4_order;
   a.cus = c->cus;
   a.dk = c->dk;
   a.per = c->per;
@@ -1155,6 +1157,9 @@ qgw_ctx *create_group(const qgw_params *ps, int nb, int precision, hipStream_t e
   c->qcur = 0;
   c->p4_tile = qgw::kP4TileCount;  // the register-streaming form
   c->p4_waves = 40;
+  // the faster dealing order measured: layers first at 2048^2 x 10, chunks
+  // first at 4096^2 x 32 (the autotuner tries both)
+  c->p4_order = (c->g.nl <= 16) ? 1 : 0;
   c->fft_e = 16;
   c->fft_e_col = (precision == QGW_FP64) ? 8 : 16;
   c->per = per ? 1 : 0;
@@ -1301,6 +1306,17 @@ int autotune_group(qgw_ctx *c, int reps, int *p4_tile, double *best_ms) {
   }
   c->p4_waves = warg;
   c->p4_tile = arg;
+  if (!st && !c->per && arg >= qgw::kP4TileCount) {
+    // the streaming forms: the other dealing order of the walks
+    const int oarg = c->p4_order;
+    c->p4_order = 1 - oarg;
+    double ms = 0;
+    st = time_candidate(c, reps, &ms);
+    if (!st && ms < best)
+      best = ms;
+    else
+      c->p4_order = oarg;
+  }
   if (!st && c->fuse >= 0) {
     // the inversion in six passes, or in four with 2, 4 or 8 column pairs per
     // block of the column pass, each with plain and non-temporal psi stores
@@ -1843,6 +1859,11 @@ int qgw_tune_set(qgw_ctx *c, int knob, int value) {
     c->p4_rev = value;
     return QGW_OK;
   }
+  if (knob == QGW_TUNE_P4_ORDER) {
+    if (value < 0 || value > 1) return QGW_ERR_ARG;
+    c->p4_order = value;
+    return QGW_OK;
+  }
   if (knob == QGW_TUNE_FUSE) {
     if (value < 0 || value > 1 || (value == 1 && c->fuse < 0)) return QGW_ERR_ARG;
     if (c->fuse >= 0) c->fuse = value;
@@ -1896,6 +1917,10 @@ int qgw_tune_get(qgw_ctx *c, int knob, int *value) {
   }
   if (knob == QGW_TUNE_P4_REV) {
     *value = c->p4_rev;
+    return QGW_OK;
+  }
+  if (knob == QGW_TUNE_P4_ORDER) {
+    *value = c->p4_order;
     return QGW_OK;
   }
   if (knob == QGW_TUNE_FFT_E) {

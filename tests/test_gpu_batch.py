@@ -15,7 +15,7 @@ pytestmark = pytest.mark.gpu
 
 # every knob the autotuner sets
 KNOBS = (Q.TUNE_P4_TILE, Q.TUNE_P4_WAVES, Q.TUNE_T_KERNEL, Q.TUNE_DST, Q.TUNE_FFT_E, Q.TUNE_FFT_E_COL,
-         Q.TUNE_FUSE, Q.TUNE_P2M_PAIRS, Q.TUNE_PSI_NT)
+         Q.TUNE_FUSE, Q.TUNE_P2M_PAIRS, Q.TUNE_PSI_NT, Q.TUNE_P4_ORDER)
 
 
 def _announce(*a):

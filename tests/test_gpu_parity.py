@@ -374,13 +374,15 @@ def test_p4_aligned_blocks(prec, case):
                                   (1024, 96, 2, -1e4, {}), (448, 256, 0, 0.0, dict(nu=20.0, nl=3))])
 def test_p4_reversed_walks(prec, case):
     """The interior walks of odd row chunks run up their chunk
-    (QGW_TUNE_P4_REV 1, the default) or down (0), in the streaming P4 and over
-    aligned column blocks: the same bits either way (q and the tendency after
-    three steps: every stencil -- both Jacobians, the nested Laplacians, nu
-    lap q, the drag -- is evaluated in the same order in both directions;
-    a y difference taken the wrong way round changes the bits), and the
-    oracle's results (FP64).  Row chunks of 16 rows on these grids, so most
-    walks are interior and half of them run up."""
+    (QGW_TUNE_P4_REV 1, the default) or down (0), and the walks are dealt
+    layers or chunks first (QGW_TUNE_P4_ORDER), in the streaming P4 and over
+    aligned column blocks: the same bits in all four combinations (q and the
+    tendency after three steps: every stencil -- both Jacobians, the nested
+    Laplacians, nu lap q, the drag -- is evaluated in the same order in both
+    directions; a y difference taken the wrong way round, or a walk given the
+    wrong layer or chunk, changes the bits), and the oracle's results (FP64).
+    Row chunks of 16 rows on these grids, so most walks are interior and half
+    of them run up."""
     NX, NY, n_hyper, nu_h, kw = case
     kw = dict(kw)
     nl = kw.pop("nl", 2)
@@ -393,17 +395,20 @@ def test_p4_reversed_walks(prec, case):
     q0 = Q.make_ic(p, modes)[1]
     for tile in (4, 5):
         out = {}
-        for rev in (0, 1):
+        for rev, order in ((0, 0), (1, 0), (0, 1), (1, 1)):
             m = Q.Model(p, prec, modes)
             m.tune_set(Q.TUNE_P4_TILE, tile)
             m.tune_set(Q.TUNE_P4_REV, rev)
-            assert m.tune_get(Q.TUNE_P4_REV) == rev
+            m.tune_set(Q.TUNE_P4_ORDER, order)
+            assert m.tune_get(Q.TUNE_P4_REV) == rev and m.tune_get(Q.TUNE_P4_ORDER) == order
             m.set_q(q0)
             m.run(3)
-            out[rev] = (m.q, m.download(Q.FIELD_F0))
+            out[rev, order] = (m.q, m.download(Q.FIELD_F0))
             m.close()
-        assert np.array_equal(out[1][0], out[0][0]), tile
-        assert np.array_equal(out[1][1], out[0][1]), tile
+        for key in ((1, 0), (0, 1), (1, 1)):
+            assert np.array_equal(out[key][0], out[0, 0][0]), (tile, key)
+            assert np.array_equal(out[key][1], out[0, 0][1]), (tile, key)
+        out[1] = out[1, 1]
         if prec == Q.FP64:
             o = O.Oracle(p)
             o.set_modes(*modes)
